@@ -1,0 +1,167 @@
+/*
+ * pepc_gpu.h -- C-ABI of the MI355X Coulomb Barnes-Hut force solve.
+ *
+ * Drop-in replacement for the frontends' call into PEPC's treecode
+ * (pepc_grow_tree + pepc_traverse_tree on one rank) for the Coulomb backend
+ * with the XYZQ particle type.  Plain C types only: double arrays in
+ * structure-of-arrays form, 64-bit sizes, int status returns.  A Fortran
+ * frontend binds these with iso_c_binding (pepc_amd/fortran/module_pepc_gpu.f90);
+ * Python binds them with ctypes (pepc_amd/_lib.py).
+ *
+ * Status codes: 0 = success, negative = error (pepc_gpu_last_error() gives
+ * the message).  The reference has no status returns at its Fortran API; its
+ * fatal errors abort through DEBUG_ERROR (pepc_debug.h:28-35) and its C calls
+ * return int checked by ERROR_ON_FAIL (pepc_debug.h:54-61).  The Fortran shim
+ * maps a nonzero status to the same kind of abort.
+ *
+ * Ordering.  Like the reference on one rank (module_domains.f90:204-234),
+ * grow sorts the particles by space-filling key; results are returned in that
+ * sorted order and perm[i] names the input index of sorted particle i
+ * (the reference's indxl(i) - 1).  pepc-mini/pepc-essential keep the sorted
+ * order (src/frontends/pepc-mini/pepc.f90:85); pepc_restore_particles undoes it.
+ *
+ * Threading.  One context is used by one host thread at a time, as the
+ * reference API is (MPI_THREAD_SERIALIZED, module_pepc.f90:88).  All work of
+ * a context is ordered on its HIP stream.
+ */
+#ifndef PEPC_GPU_H
+#define PEPC_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PEPC_GPU_OK = 0,
+    PEPC_GPU_EINVAL = -1,     /* bad argument or call order */
+    PEPC_GPU_EHIP = -2,       /* HIP runtime error (no device, launch failure) */
+    PEPC_GPU_ENOMEM = -3,     /* device allocation failed */
+    PEPC_GPU_EOVERFLOW = -4,  /* node pool too small (raise node_factor, cf. np_mult) */
+    PEPC_GPU_EKEYS = -5,      /* key space exhausted / unresolvable duplicate keys */
+    PEPC_GPU_EBOX = -6,       /* non-finite coordinates or a degenerate box */
+    PEPC_GPU_EWALK = -7       /* walk stack overflow */
+};
+
+enum { PEPC_GPU_FP32_KAHAN = 0, PEPC_GPU_FP64 = 1 };
+
+/* Solver parameters; the &calc_force_coulomb and &libpepc namelist entries
+ * of the reference (coulomb/module_interaction_specific.f90:35-43,
+ * module_libpepc_main.f90:44) that apply to this path. */
+typedef struct {
+    double theta2;          /* BH opening parameter squared (default 0.36) */
+    double eps2;            /* Plummer softening squared (default 0) */
+    double sign;            /* +1 Coulomb; -1 attractive (gravity-like) */
+    double node_factor;     /* node capacity per particle (np_mult analogue, default 2.5) */
+    int32_t mac_select;     /* 0 = Barnes-Hut (the only value supported) */
+    int32_t group_size;     /* walk tile size G, 1..64 (16 = module_tree_walk_simple TILE_SIZE) */
+    int32_t precision;      /* PEPC_GPU_FP32_KAHAN or PEPC_GPU_FP64 */
+    int32_t force_cubic_domain;  /* module_box.f90:29 */
+    int32_t walk_blocks;    /* persistent walk grid; 0 = fill the device */
+    int32_t device;         /* HIP device ordinal */
+} pepc_gpu_config;
+
+/* Counters of the last grow/traverse (tree_stats, module_tree.f90:622-699;
+ * tree_walk_statistics, module_tree_walk_simple.f90:63-115). */
+typedef struct {
+    int64_t n;
+    int64_t nnodes, nleaf, ntwig, ngroups;
+    int64_t dup_fixed;          /* particles moved by the duplicate-key fix-up */
+    uint64_t interactions;      /* sum over targets of P2P + M2P (self included) */
+    uint64_t mac_evaluations;   /* per-tile MAC tests, first failure ends a node */
+    uint64_t p2p, m2p;
+    uint32_t flags;
+    int32_t maxlevel;
+    uint32_t level_offset[22];  /* first node of each level in the device layout */
+} pepc_gpu_stats;
+
+/* Fills *cfg with the defaults above. */
+void pepc_gpu_config_default(pepc_gpu_config *cfg);
+
+/* Replaces pepc_initialize + pepc_prepare (module_pepc.f90:70-175, 269-288):
+ * binds a device and creates the context's stream. */
+int pepc_gpu_create(void **ctx, const pepc_gpu_config *cfg);
+
+/* Replaces pepc_finalize (module_pepc.f90:177-200). */
+int pepc_gpu_destroy(void *ctx);
+
+/* Copies the last error message (NUL-terminated) into buf. */
+int pepc_gpu_last_error(const void *ctx, char *buf, size_t len);
+
+/* Orders the context's work on a caller stream (hipStream_t) instead of its own. */
+int pepc_gpu_set_stream(void *ctx, void *hip_stream);
+
+/* Replaces pepc_grow_tree (module_pepc.f90:435-446 -> tree_grow,
+ * module_tree_grow.f90:37-138), host arrays.  Inputs x,y,z,q[n] in any
+ * order; outputs (each may be NULL): keys_out[n] sorted keys, perm_out[n],
+ * xs/ys/zs[n] sorted positions after the duplicate-key fix-up
+ * (module_domains.f90:299-305 rewrites moved particles).  Synchronous. */
+int pepc_gpu_grow(void *ctx, int64_t n, const double *x, const double *y, const double *z,
+                  const double *q, uint64_t *keys_out, int64_t *perm_out,
+                  double *xs, double *ys, double *zs);
+
+/* Replaces pepc_traverse_tree (module_pepc.f90:459-468 ->
+ * libpepc_traverse_tree, module_libpepc_main.f90:161-200) for the grown
+ * particles.  Outputs in sorted order (each may be NULL): field ex,ey,ez,
+ * potential pot (particle%results), work = 1 + interactions (particle%work,
+ * module_domains.f90:318 + walk increments).  Synchronous. */
+int pepc_gpu_traverse(void *ctx, double *ex, double *ey, double *ez, double *pot, double *work);
+
+/* Device-pointer forms of grow and traverse: asynchronous on the context's
+ * stream; call pepc_gpu_sync() to wait and check for errors. */
+int pepc_gpu_grow_device(void *ctx, int64_t n, const double *x, const double *y,
+                         const double *z, const double *q);
+int pepc_gpu_traverse_device(void *ctx, double *ex, double *ey, double *ez, double *pot,
+                             double *work);
+
+/* Waits for the context's work, reads its counters into *st (may be NULL)
+ * and reports errors raised by the kernels. */
+int pepc_gpu_sync(void *ctx, pepc_gpu_stats *st);
+
+/* Replaces pepc_timber_tree (module_pepc.f90:538): releases nothing but
+ * marks the tree invalid (device buffers are kept for the next step). */
+int pepc_gpu_timber(void *ctx);
+
+/* Device pointers to the sorted particle arrays of the last grow (valid
+ * until the next grow): keys[n] (uint64), perm[n] (uint32), x,y,z,q[n]. */
+int pepc_gpu_sorted_device(void *ctx, const uint64_t **keys, const uint32_t **perm,
+                           const double **x, const double **y, const double **z,
+                           const double **q);
+
+/* Host copies of the sorted particle arrays of the last grow (each may be
+ * NULL): keys[n], perm[n] (input index of sorted particle i), x,y,z,q[n]. */
+int pepc_gpu_read_sorted(void *ctx, uint64_t *keys, int64_t *perm, double *x, double *y,
+                         double *z, double *q);
+
+/* Copies the tree of the last grow to host arrays of capacity cap nodes:
+ * link[4*i] = {first child or particle, #children (0 = leaf), #particles,
+ * level}, first[i] = first particle, mom[16*i] = moments in the reference
+ * layout {coc(3), charge, abs_charge, dip(3), quad(3), xy, yz, zx, bmax, 0}
+ * (coulomb/module_interaction_specific_types_XYZQ.f90:45-55).  Returns the
+ * node count in *nnodes. */
+int pepc_gpu_read_tree(void *ctx, int64_t cap, uint32_t *link, uint32_t *first, double *mom,
+                       int64_t *nnodes);
+
+/* Copies the walk groups and per-target counts of the last traverse:
+ * gfirst/gcount/gmac[ngroups] and p2p/m2p[n] (sorted order). */
+int pepc_gpu_read_walk(void *ctx, int64_t gcap, uint32_t *gfirst, uint32_t *gcount,
+                       uint32_t *gmac, uint32_t *p2p, uint32_t *m2p, int64_t *ngroups);
+
+/* FP64 direct sum (src/utils/module_directsum.f90:34-172) on device arrays:
+ * fields at the ntest targets testidx[] from all n sources, self excluded by
+ * index.  Asynchronous on the context's stream. */
+int pepc_gpu_direct_device(void *ctx, int64_t n, const double *x, const double *y,
+                           const double *z, const double *q, int64_t ntest,
+                           const int64_t *testidx, double eps2, double sign, double *ex,
+                           double *ey, double *ez, double *pot);
+
+/* Device timing of the last grow and traverse in milliseconds (HIP events
+ * on the context's stream, recorded by the *_device and host forms). */
+int pepc_gpu_timings(void *ctx, double *t_grow_ms, double *t_walk_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1 @@
+"""Oracle package: CPU restatement of the reference path (test infrastructure only)."""

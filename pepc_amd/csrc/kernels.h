@@ -1,0 +1,99 @@
+// kernels.h -- host-side launch entry points of the device kernels.
+#pragma once
+
+#include "common.h"
+
+namespace pepc {
+
+// keys.hip
+int box_partial_blocks(int64_t n);
+void launch_box(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
+                int force_cubic, double* partial, DevBox* box, DevScalars* sc);
+void launch_keys(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
+                 const DevBox* box, unsigned long long* keys, uint32_t* idx);
+
+// sort.hip
+int64_t scan_blocks(int64_t n);
+int64_t sort_blocks(int64_t n);
+void launch_scan(hipStream_t s, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* bsum,
+                 uint32_t* total);
+void launch_radix_pass(hipStream_t s, const unsigned long long* kin, const uint32_t* vin,
+                       unsigned long long* kout, uint32_t* vout, int64_t n, int shift, int nbits,
+                       uint32_t* hist, uint32_t* bsum);
+void launch_radix_pass_dev(hipStream_t s, const unsigned long long* kin, const uint32_t* vin,
+                           unsigned long long* kout, uint32_t* vout, int64_t n, const uint32_t* ndev,
+                           int shift, int nbits, uint32_t* hist, uint32_t* bsum);
+void launch_sort_keys(hipStream_t s, unsigned long long* k0, uint32_t* v0, unsigned long long* k1,
+                      uint32_t* v1, int64_t n, uint32_t* hist, uint32_t* bsum);
+void launch_gather4(hipStream_t s, int64_t n, const uint32_t* perm, const double* x, const double* y,
+                    const double* z, const double* q, double* xs, double* ys, double* zs,
+                    double* qs);
+
+// tree.hip
+struct TreeBuffers {
+  int64_t n;
+  int64_t cap;                       // node capacity
+  const unsigned long long* keys;    // sorted particle keys
+  const double *x, *y, *z, *q;       // sorted particles
+  uint8_t* lo;                       // L(a): level of the first node of particle a
+  uint8_t* hi;                       // leaf level of particle a
+  uint32_t* cnt;                     // nodes per particle -> DFS offsets
+  unsigned long long* dkey[2];       // level keys (sort ping-pong)
+  uint32_t* dval[2];                 // DFS indices (sort ping-pong)
+  uint32_t* dpart;                   // DFS node -> particle
+  uint32_t* inv;                     // DFS index -> BFS index
+  uint32_t* flag;                    // twig flags -> twig ranks
+  uint32_t* fc_rank;                 // first child by twig rank (+1 sentinel)
+  uint32_t* node_rank;               // node index by twig rank
+  uint32_t* nfirst;                  // first particle of each node
+  NodeLink* link;
+  NodeGeom* geom;
+  double* mom64;                     // 16 doubles per node, reference layout
+  NodeMoments<float>* wm32;          // walk form, FP32
+  NodeMoments<double>* wm64;         // walk form, FP64
+  uint32_t* gmark;                   // group marks per particle -> ranks
+  uint32_t* gfirst;
+  uint32_t* gcount;
+  uint32_t* hist;
+  uint32_t* bsum;
+  DevBox* box;
+  DevScalars* sc;
+};
+
+void launch_dup_detect(hipStream_t s, int64_t n, const unsigned long long* keys, DevScalars* sc);
+void launch_dupfix(hipStream_t s, int64_t n, unsigned long long* keys, double* x, double* y,
+                   double* z, const DevBox* box, long long* kd, unsigned long long* oldkeys,
+                   uint32_t* dflag, uint32_t* dlist, uint32_t* bsum, DevScalars* sc);
+void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size);
+
+// walk.hip
+struct WalkArgs {
+  int64_t n;
+  const double *x, *y, *z, *q;
+  const NodeGeom* geom;
+  const NodeLink* link;
+  const NodeMoments<float>* wm32;
+  const NodeMoments<double>* wm64;
+  const uint32_t* gfirst;
+  const uint32_t* gcount;
+  DevScalars* sc;
+  double theta2, eps2, sign;
+  double* ex;
+  double* ey;
+  double* ez;
+  double* pot;
+  double* work;
+  uint32_t* np2p;
+  uint32_t* nm2p;
+  uint32_t* gmac;
+};
+enum WalkPrecision { kWalkFP32Kahan = 0, kWalkFP64 = 1 };
+void launch_walk(hipStream_t s, const WalkArgs& a, int precision, int blocks);
+int walk_default_blocks();
+
+// direct.hip
+void launch_direct(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
+                   const double* q, int64_t ntest, const int64_t* testidx, double eps2, double sign,
+                   double* ex, double* ey, double* ez, double* pot);
+
+}  // namespace pepc

@@ -1,0 +1,458 @@
+// tree.hip -- duplicate-key fix-up, level-synchronous octree build, FP64
+// upward pass and walk groups.
+//
+// Build (replaces tree_build_from_particles / insert_helper,
+// module_tree_grow.f90:552-697).  For sorted distinct keys let L(a) be the
+// first level at which particles a-1 and a fall into different cells (0 at
+// both ends).  Particle a then owns the nodes of levels L(a)..max(L(a),L(a+1)):
+// twigs down to the level where it separates from its right neighbour, and
+// its leaf at the deepest of those levels.  Numbering the nodes particle by
+// particle gives the reference's pre-order (insert_helper provisions a twig,
+// then its children in key order).  One stable counting pass keyed on the
+// level turns that into a level-by-level (BFS) order in which the children of
+// every twig are contiguous; the k-th run of children belongs to the k-th twig.
+//
+// Upward pass (tree_node_update_from_children, :762-821, with
+// shift_multipoles_up, coulomb/module_interaction_specific.f90:84-140): one
+// launch per level, deepest first, FP64, the reference's operation order with
+// contraction off, so moments are bit-identical to the reference formula.
+//
+// Groups (walk tiles, module_tree_walk_simple.f90:135-212): the maximal key
+// cells holding at most G particles = children with count <= G of twigs with
+// count > G (or the root).
+#include "kernels.h"
+
+namespace pepc {
+
+namespace {
+
+constexpr int kT = 256;
+
+// ------------------------------------------------------------ duplicate keys
+__global__ void dup_detect_kernel(int64_t n, const unsigned long long* __restrict__ keys,
+                                  DevScalars* __restrict__ sc) {
+  uint32_t c = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i + 1 < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    c += keys[i + 1] <= keys[i] ? 1u : 0u;
+  c = wave_sum(c);
+  if (lane_id() == 0 && c) atomicAdd(&sc->dup_count, c);
+}
+
+__global__ void dup_prepare_kernel(int64_t n, const unsigned long long* __restrict__ keys,
+                                   long long* __restrict__ kd, unsigned long long* __restrict__ old,
+                                   uint32_t* __restrict__ flag, const DevScalars* __restrict__ sc) {
+  if (sc->dup_count == 0) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    long long d = (i + 1 < n) ? (long long)(keys[i + 1] - keys[i]) : 1ll;  // last rank
+    kd[i] = d;
+    old[i] = keys[i];
+    flag[i] = d < 1 ? 1u : 0u;
+  }
+}
+
+__global__ void dup_compact_kernel(int64_t n, const long long* __restrict__ kd,
+                                   const uint32_t* __restrict__ rank, uint32_t* __restrict__ list,
+                                   const DevScalars* __restrict__ sc) {
+  if (sc->dup_count == 0) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    if (kd[i] < 1) list[rank[i]] = (uint32_t)i;
+}
+
+// module_domains.f90:248-297, one thread, in the reference's loop order.
+// Only positions that start as duplicates are ever modified to a smaller
+// key difference, so visiting the compacted list in order equals visiting
+// every position.
+__global__ void dup_serial_kernel(int64_t n, unsigned long long* __restrict__ keys,
+                                  long long* __restrict__ kd, const uint32_t* __restrict__ list,
+                                  DevScalars* __restrict__ sc) {
+  const uint32_t nd = sc->dup_count;
+  if (nd == 0 || threadIdx.x != 0) return;
+  if ((long long)(keys[n - 1] - keys[0]) + 1 < n) {   // :254-262
+    atomicOr(&sc->flags, kFlagDupKeys);
+    return;
+  }
+  for (uint32_t r = 0; r < nd; ++r) {
+    const long long i1 = (long long)list[r] + 1;        // 1-based as in the source
+    if (kd[i1 - 1] >= 1) continue;
+    const long long jmax = (n - i1) > i1 ? (n - i1) : i1;
+    for (long long j = 1; j <= jmax; ++j) {
+      const long long dn = i1 - j < 1 ? 1 : i1 - j;
+      const long long up = i1 + j < n - 1 ? i1 + j : n - 1;
+      if (kd[dn - 1] > 1 && i1 - j >= 1) {
+        for (long long t = i1 - j + 1; t <= i1; ++t) keys[t - 1] -= 1ull;
+        kd[i1 - 1] += 1;
+        kd[i1 - j - 1] -= 1;
+        break;
+      } else if (kd[up - 1] > 1 && i1 + j <= n - 1) {
+        for (long long t = i1 + 1; t <= i1 + j; ++t) keys[t - 1] += 1ull;
+        kd[i1 - 1] += 1;
+        kd[i1 + j - 1] -= 1;
+        break;
+      }
+    }
+  }
+}
+
+// key_to_intcoord_hilbert / key_to_coord, module_spacefilling.f90:445-565.
+constexpr int kC3[8] = {0, 1, 3, 2, 6, 7, 5, 4};
+constexpr int kGX[8] = {5, 6, 0, 5, 5, 0, 6, 5};
+constexpr int kGR[8] = {0, 0, 0, 5, 0, 0, 6, 5};
+
+__device__ void key_to_intcoord(unsigned long long key, long long ic[3]) {
+  const int lev = level_of_key(key);
+  ic[0] = ic[1] = ic[2] = 0;
+  for (int i = 0; i < lev; ++i) {
+    const int h = (int)((key >> (3 * i)) & 7ull);
+    const int cval = kC3[h], rev = kGR[h], ex = kGX[h];
+    if (i > 0) {
+      for (int j = 0; j < 3; ++j)
+        if ((rev >> j) & 1) ic[j] = (~ic[j]) & ((1ll << i) - 1);
+      for (int j = 0; j < 2; ++j) {
+        if (!((ex >> j) & 1)) continue;
+        for (int k = j + 1; k < 3; ++k)
+          if ((ex >> k) & 1) { long long t = ic[j]; ic[j] = ic[k]; ic[k] = t; }
+      }
+    }
+    for (int j = 0; j < 3; ++j) ic[j] |= (long long)((cval >> j) & 1) << i;
+  }
+  for (int j = 0; j < 3; ++j) ic[j] <<= (kMaxLevel - lev);
+}
+
+__global__ void dup_recoord_kernel(int64_t n, const unsigned long long* __restrict__ keys,
+                                   const unsigned long long* __restrict__ old,
+                                   double* __restrict__ x, double* __restrict__ y,
+                                   double* __restrict__ z, const DevBox* __restrict__ box,
+                                   const DevScalars* __restrict__ sc) {
+  if (sc->dup_count == 0) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (keys[i] == old[i]) continue;
+    long long ic[3];
+    key_to_intcoord(keys[i], ic);
+    x[i] = ((double)ic[0] + 0.5) * box->cell[0] + box->boxmin[0];
+    y[i] = ((double)ic[1] + 0.5) * box->cell[1] + box->boxmin[1];
+    z[i] = ((double)ic[2] + 0.5) * box->cell[2] + box->boxmin[2];
+  }
+}
+
+// ------------------------------------------------------------ build
+// Every kernel after the node count is known bails out once an earlier stage
+// raised a flag that makes the node arrays inconsistent, so no index read
+// from those arrays is ever used.
+__device__ __forceinline__ bool tree_broken(const DevScalars* sc) {
+  return (sc->flags & (kFlagNodeOverflow | kFlagDupKeys | kFlagBadBox)) != 0u;
+}
+
+__global__ void levels_kernel(int64_t n, const unsigned long long* __restrict__ keys,
+                              uint8_t* __restrict__ lo, uint8_t* __restrict__ hi,
+                              uint32_t* __restrict__ cnt, DevScalars* __restrict__ sc) {
+  for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < n;
+       a += (int64_t)gridDim.x * blockDim.x) {
+    int la = 0, lb = 0;
+    bool dup = false;
+    if (a > 0) {
+      if (keys[a - 1] == keys[a]) dup = true; else la = split_level(keys[a - 1], keys[a]);
+    }
+    if (a + 1 < n) {
+      if (keys[a + 1] == keys[a]) dup = true; else lb = split_level(keys[a], keys[a + 1]);
+    }
+    if (dup) {
+      atomicOr(&sc->flags, kFlagDupKeys);
+      la = lb = 0;
+    }
+    const int lam = la > lb ? la : lb;
+    lo[a] = (uint8_t)la;
+    hi[a] = (uint8_t)lam;
+    cnt[a] = (uint32_t)(lam - la + 1);
+  }
+}
+
+__global__ void dfs_emit_kernel(int64_t n, int64_t cap, const uint8_t* __restrict__ lo,
+                                const uint8_t* __restrict__ hi, const uint32_t* __restrict__ pre,
+                                unsigned long long* __restrict__ dkey, uint32_t* __restrict__ dval,
+                                uint32_t* __restrict__ dpart, DevScalars* __restrict__ sc) {
+  if ((int64_t)sc->nnodes > cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&sc->flags, kFlagNodeOverflow);
+    return;
+  }
+  if (tree_broken(sc)) return;
+  for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < n;
+       a += (int64_t)gridDim.x * blockDim.x) {
+    const int la = lo[a], lam = hi[a];
+    const uint32_t base = pre[a];
+    for (int l = la; l <= lam; ++l) {
+      const uint32_t d = base + (uint32_t)(l - la);
+      dkey[d] = (unsigned long long)l;
+      dval[d] = d;
+      dpart[d] = (uint32_t)a;
+    }
+  }
+}
+
+__global__ void level_offsets_kernel(const uint32_t* __restrict__ hist, int nblocks,
+                                     DevScalars* __restrict__ sc) {
+  const int l = threadIdx.x;
+  const bool broken = tree_broken(sc);
+  __syncthreads();
+  if (l <= kMaxLevel + 1) {
+    uint32_t off = (l <= kMaxLevel) ? hist[(int64_t)l * nblocks] : sc->nnodes;
+    sc->level_offset[l] = broken ? 0u : off;
+  }
+}
+
+__global__ void inv_kernel(const uint32_t* __restrict__ dval, uint32_t* __restrict__ inv,
+                           const DevScalars* __restrict__ sc) {
+  if (tree_broken(sc)) return;
+  const int64_t m = sc->nnodes;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x)
+    inv[dval[i]] = (uint32_t)i;
+}
+
+__global__ void node_fill_kernel(int64_t cap, const unsigned long long* __restrict__ dkey,
+                                 const uint32_t* __restrict__ dval, const uint32_t* __restrict__ dpart,
+                                 const uint32_t* __restrict__ inv, const uint8_t* __restrict__ hi,
+                                 NodeLink* __restrict__ link, uint32_t* __restrict__ nfirst,
+                                 uint32_t* __restrict__ flag, const DevScalars* __restrict__ sc) {
+  const int64_t m = tree_broken(sc) ? 0 : (int64_t)sc->nnodes;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cap;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (i >= m) { flag[i] = 0; continue; }
+    const uint32_t d = dval[i];
+    const uint32_t l = (uint32_t)dkey[i];
+    const uint32_t a = dpart[d];
+    NodeLink lk;
+    lk.level = l;
+    nfirst[i] = a;
+    if (l == hi[a]) {          // leaf
+      lk.first = a; lk.nchild = 0; lk.count = 1;
+      flag[i] = 0;
+    } else {                   // twig: first child is this particle's next node
+      lk.first = inv[d + 1]; lk.nchild = 0xFFFFFFFFu; lk.count = 0;
+      flag[i] = 1;
+    }
+    link[i] = lk;
+  }
+}
+
+__global__ void twig_rank_kernel(const NodeLink* __restrict__ link, const uint32_t* __restrict__ rank,
+                                 uint32_t* __restrict__ fc_rank, uint32_t* __restrict__ node_rank,
+                                 const DevScalars* __restrict__ sc) {
+  if (tree_broken(sc)) return;
+  const int64_t m = sc->nnodes;
+  if (blockIdx.x == 0 && threadIdx.x == 0) fc_rank[sc->ntwig] = (uint32_t)m;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (link[i].nchild != 0u) {
+      const uint32_t r = rank[i];
+      fc_rank[r] = link[i].first;
+      node_rank[r] = (uint32_t)i;
+    }
+  }
+}
+
+__global__ void nchild_kernel(NodeLink* __restrict__ link, const uint32_t* __restrict__ fc_rank,
+                              const uint32_t* __restrict__ node_rank, const DevScalars* __restrict__ sc) {
+  if (tree_broken(sc)) return;
+  const int64_t nt = sc->ntwig;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nt;
+       r += (int64_t)gridDim.x * blockDim.x)
+    link[node_rank[r]].nchild = fc_rank[r + 1] - fc_rank[r];
+}
+
+// Moment record slots, reference layout (t_tree_node_interaction_data order).
+enum { M_CX = 0, M_CY, M_CZ, M_Q, M_AQ, M_PX, M_PY, M_PZ, M_QX, M_QY, M_QZ, M_XY, M_YZ, M_ZX, M_BMAX };
+
+__device__ void write_walk_forms(int64_t i, const double* m, double b2, NodeGeom* geom,
+                                 NodeMoments<float>* wm32, NodeMoments<double>* wm64) {
+  NodeGeom g;
+  g.cx = m[M_CX]; g.cy = m[M_CY]; g.cz = m[M_CZ]; g.b2 = b2;
+  geom[i] = g;
+  NodeMoments<double> w;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) w.v[k] = 0.0;
+  w.v[0] = m[M_Q];
+  w.v[1] = m[M_PX]; w.v[2] = m[M_PY]; w.v[3] = m[M_PZ];
+  w.v[4] = 2.0 * m[M_QX]; w.v[5] = 2.0 * m[M_QY]; w.v[6] = 2.0 * m[M_QZ];
+  w.v[7] = m[M_XY]; w.v[8] = m[M_YZ]; w.v[9] = m[M_ZX];
+  w.v[10] = (m[M_QX] + m[M_QY]) + m[M_QZ];
+  wm64[i] = w;
+  NodeMoments<float> f;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) f.v[k] = (float)w.v[k];
+  wm32[i] = f;
+}
+
+__global__ __launch_bounds__(kT) void upward_kernel(int level, NodeLink* link,
+                                                   const double* __restrict__ x,
+                                                   const double* __restrict__ y,
+                                                   const double* __restrict__ z,
+                                                   const double* __restrict__ q,
+                                                   double* mom, NodeGeom* __restrict__ geom,
+                                                   NodeMoments<float>* __restrict__ wm32,
+                                                   NodeMoments<double>* __restrict__ wm64,
+                                                   const DevBox* __restrict__ box,
+                                                   const DevScalars* __restrict__ sc) {
+  if (tree_broken(sc)) return;
+  const int64_t b = sc->level_offset[level], e = sc->level_offset[level + 1];
+  const double b2 = box->b2[level];
+  for (int64_t i = b + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < e;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const NodeLink lk = link[i];
+    double m[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) m[k] = 0.0;
+    if (lk.nchild == 0) {
+      // multipole_from_particle, coulomb/module_interaction_specific.f90:67-79
+      const uint32_t p = lk.first;
+      m[M_CX] = x[p]; m[M_CY] = y[p]; m[M_CZ] = z[p];
+      m[M_Q] = q[p]; m[M_AQ] = fabs(q[p]);
+    } else {
+      const uint32_t c0 = lk.first, nc = lk.nchild;
+      double charge = 0.0, absq = 0.0;
+      uint32_t count = 0;
+      for (uint32_t j = 0; j < nc; ++j) {
+        charge = charge + mom[(int64_t)(c0 + j) * 16 + M_Q];
+        count += link[c0 + j].count;
+      }
+      for (uint32_t j = 0; j < nc; ++j) absq = absq + mom[(int64_t)(c0 + j) * 16 + M_AQ];
+      double coc[3] = {0.0, 0.0, 0.0};
+      if (absq != 0.0) {
+        for (uint32_t j = 0; j < nc; ++j) {
+          const double* c = mom + (int64_t)(c0 + j) * 16;
+          for (int d = 0; d < 3; ++d) coc[d] = coc[d] + (c[M_CX + d] * c[M_AQ]);
+        }
+        for (int d = 0; d < 3; ++d) coc[d] = coc[d] / absq;
+      } else {
+        for (uint32_t j = 0; j < nc; ++j) {
+          const double* c = mom + (int64_t)(c0 + j) * 16;
+          for (int d = 0; d < 3; ++d) coc[d] = coc[d] + c[M_CX + d];
+        }
+        for (int d = 0; d < 3; ++d) coc[d] = coc[d] / (double)nc;
+      }
+      double dip[3] = {0.0, 0.0, 0.0}, quad[3] = {0.0, 0.0, 0.0}, xy = 0.0, yz = 0.0, zx = 0.0;
+      double bmax = 0.0;
+      for (uint32_t j = 0; j < nc; ++j) {
+        const double* c = mom + (int64_t)(c0 + j) * 16;
+        double s[3];
+        for (int d = 0; d < 3; ++d) s[d] = coc[d] - c[M_CX + d];
+        for (int d = 0; d < 3; ++d) dip[d] = dip[d] + c[M_PX + d] + c[M_Q] * s[d];
+        for (int d = 0; d < 3; ++d)
+          quad[d] = quad[d] + c[M_QX + d] + c[M_PX + d] * s[d] + 0.5 * c[M_Q] * (s[d] * s[d]);
+        xy = xy + c[M_XY] + c[M_PX] * s[1] + c[M_PY] * s[0] + c[M_Q] * s[0] * s[1];
+        yz = yz + c[M_YZ] + c[M_PY] * s[2] + c[M_PZ] * s[1] + c[M_Q] * s[1] * s[2];
+        zx = zx + c[M_ZX] + c[M_PZ] * s[0] + c[M_PX] * s[2] + c[M_Q] * s[2] * s[0];
+        const double v = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + c[M_BMAX];
+        if (j == 0 || v > bmax) bmax = v;
+      }
+      m[M_CX] = coc[0]; m[M_CY] = coc[1]; m[M_CZ] = coc[2];
+      m[M_Q] = charge; m[M_AQ] = absq;
+      for (int d = 0; d < 3; ++d) { m[M_PX + d] = dip[d]; m[M_QX + d] = quad[d]; }
+      m[M_XY] = xy; m[M_YZ] = yz; m[M_ZX] = zx; m[M_BMAX] = bmax;
+      link[i].count = count;
+    }
+    double* out = mom + i * 16;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) out[k] = m[k];
+    write_walk_forms(i, m, b2, geom, wm32, wm64);
+  }
+}
+
+// ------------------------------------------------------------ groups
+__global__ void gmark_clear_kernel(int64_t n, uint32_t* __restrict__ gmark) {
+  for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < n;
+       a += (int64_t)gridDim.x * blockDim.x)
+    gmark[a] = 0;
+}
+
+__global__ void group_mark_kernel(int group_size, const NodeLink* __restrict__ link,
+                                  const uint32_t* __restrict__ nfirst, uint32_t* __restrict__ gmark,
+                                  const DevScalars* __restrict__ sc) {
+  if (tree_broken(sc)) return;
+  const int64_t m = sc->nnodes;
+  const uint32_t G = (uint32_t)group_size;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && m > 0 && link[0].count <= G) gmark[0] = link[0].count;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const NodeLink lk = link[i];
+    if (lk.nchild == 0 || lk.count <= G) continue;
+    for (uint32_t j = 0; j < lk.nchild; ++j) {
+      const uint32_t c = lk.first + j;
+      const uint32_t cc = link[c].count;
+      if (cc <= G) gmark[nfirst[c]] = cc;
+    }
+  }
+}
+
+__global__ void group_flag_kernel(int64_t n, const uint32_t* __restrict__ gmark, uint32_t* __restrict__ f) {
+  for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < n;
+       a += (int64_t)gridDim.x * blockDim.x)
+    f[a] = gmark[a] ? 1u : 0u;
+}
+
+__global__ void group_emit_kernel(int64_t n, const uint32_t* __restrict__ gmark,
+                                  const uint32_t* __restrict__ rank, uint32_t* __restrict__ gfirst,
+                                  uint32_t* __restrict__ gcount) {
+  for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < n;
+       a += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t c = gmark[a];
+    if (c) {
+      const uint32_t g = rank[a];
+      gfirst[g] = (uint32_t)a;
+      gcount[g] = c;
+    }
+  }
+}
+
+}  // namespace
+
+static unsigned grid_for(int64_t n) {
+  int64_t b = (n + kT - 1) / kT;
+  if (b < 1) b = 1;
+  if (b > 8192) b = 8192;
+  return (unsigned)b;
+}
+
+void launch_dup_detect(hipStream_t s, int64_t n, const unsigned long long* keys, DevScalars* sc) {
+  dup_detect_kernel<<<grid_for(n), kT, 0, s>>>(n, keys, sc);
+}
+
+void launch_dupfix(hipStream_t s, int64_t n, unsigned long long* keys, double* x, double* y,
+                   double* z, const DevBox* box, long long* kd, unsigned long long* oldkeys,
+                   uint32_t* dflag, uint32_t* dlist, uint32_t* bsum, DevScalars* sc) {
+  dup_prepare_kernel<<<grid_for(n), kT, 0, s>>>(n, keys, kd, oldkeys, dflag, sc);
+  launch_scan(s, dflag, dflag, n, bsum, nullptr);
+  dup_compact_kernel<<<grid_for(n), kT, 0, s>>>(n, kd, dflag, dlist, sc);
+  dup_serial_kernel<<<1, 1, 0, s>>>(n, keys, kd, dlist, sc);
+  dup_recoord_kernel<<<grid_for(n), kT, 0, s>>>(n, keys, oldkeys, x, y, z, box, sc);
+}
+
+void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size) {
+  const int64_t n = t.n, cap = t.cap;
+  levels_kernel<<<grid_for(n), kT, 0, s>>>(n, t.keys, t.lo, t.hi, t.cnt, t.sc);
+  launch_scan(s, t.cnt, t.cnt, n, t.bsum, &t.sc->nnodes);
+  dfs_emit_kernel<<<grid_for(n), kT, 0, s>>>(n, cap, t.lo, t.hi, t.cnt, t.dkey[0], t.dval[0], t.dpart, t.sc);
+  // one stable pass keyed on the level; entries beyond nnodes are never read
+  launch_radix_pass_dev(s, t.dkey[0], t.dval[0], t.dkey[1], t.dval[1], cap, &t.sc->nnodes, 0, 5,
+                        t.hist, t.bsum);
+  level_offsets_kernel<<<1, 64, 0, s>>>(t.hist, (int)sort_blocks(cap), t.sc);
+  inv_kernel<<<grid_for(cap), kT, 0, s>>>(t.dval[1], t.inv, t.sc);
+  node_fill_kernel<<<grid_for(cap), kT, 0, s>>>(cap, t.dkey[1], t.dval[1], t.dpart, t.inv, t.hi,
+                                                t.link, t.nfirst, t.flag, t.sc);
+  launch_scan(s, t.flag, t.flag, cap, t.bsum, &t.sc->ntwig);
+  twig_rank_kernel<<<grid_for(cap), kT, 0, s>>>(t.link, t.flag, t.fc_rank, t.node_rank, t.sc);
+  nchild_kernel<<<grid_for(cap), kT, 0, s>>>(t.link, t.fc_rank, t.node_rank, t.sc);
+  for (int l = kMaxLevel; l >= 0; --l)
+    upward_kernel<<<grid_for(cap / 4), kT, 0, s>>>(l, t.link, t.x, t.y, t.z, t.q, t.mom64,
+                                                   t.geom, t.wm32, t.wm64, t.box, t.sc);
+  gmark_clear_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark);
+  group_mark_kernel<<<grid_for(cap), kT, 0, s>>>(group_size, t.link, t.nfirst, t.gmark, t.sc);
+  group_flag_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark, t.cnt);
+  launch_scan(s, t.cnt, t.cnt, n, t.bsum, &t.sc->ngroups);
+  group_emit_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark, t.cnt, t.gfirst, t.gcount);
+}
+
+}  // namespace pepc

@@ -1,0 +1,416 @@
+// walk.hip -- persistent group walk with P2P/M2P force and potential kernels.
+//
+// Semantics (module_tree_walk_simple.f90:256-384): a group of target
+// particles (a walk tile) descends the tree from the root; a leaf interacts
+// with every target (P2P, self excluded); a twig is accepted for M2P only if
+// every target passes the Barnes-Hut test theta2*d^2 > b^2(level)
+// (coulomb/module_interaction_specific.f90:256-275), otherwise it is opened.
+// Interaction counts per target and MAC-evaluation counts per group equal the
+// reference's at the same group rule.
+//
+// Mapping to CDNA4.  One 64-lane wavefront owns one group at a time and pulls
+// the next group from a device queue (persistent kernel).  A group of cnt
+// targets occupies T = pow2 >= cnt lanes; the S = 64/T copies ("slices")
+// split every interaction list between them, so small groups still fill the
+// wave.  Traversal is breadth-parallel: each step pops up to 64 nodes from an
+// LDS stack, one node per lane, and decides it against the group's bounding
+// box; only nodes whose decision the box cannot settle run the exact
+// per-target FP64 test (all lanes, one ballot).  Accepted nodes and leaves are
+// appended to M2P / P2P lists staged in LDS and evaluated in batches: every
+// lane reads the same entry (LDS broadcast) and evaluates it for its target.
+//
+// Precision.  FP32 path: coordinates relative to the group's first particle
+// (differences taken in FP64 once per node, then rounded), FP32 kernels with
+// fused multiply-add, per-target compensated (Kahan) accumulation of each
+// batch's partial sums.  FP64 path: absolute coordinates, FP64 kernels,
+// plain FP64 accumulation, as the reference.
+//
+// The M2P evaluation regroups the reference's expression
+// (module_coulomb_kernels.f90:60-145) around the quadratic form
+// S = d.A.d of the quadrupole tensor; it is the same polynomial in d and the
+// moments (phi = q/r - d.p/r^3 + 3 S/r^5 - tr(A)/r^3 and E = -grad phi with
+// r^2 = d^2 + eps^2), evaluated with fewer operations.
+#include "kernels.h"
+
+namespace pepc {
+
+namespace {
+
+constexpr int kStackCap = 512;      // entries the pop policy keeps below
+constexpr int kStackSlack = 192;    // >= 7 * 21 + 8: depth-first overrun bound
+constexpr int kListP = 128;
+
+template <typename R> struct Prec;
+template <> struct Prec<float> {
+  static constexpr int kListM = 128;
+  using V4 = float4;
+  static __device__ __forceinline__ float rsq(float x) { return rsqrtf(x); }
+  static __device__ __forceinline__ float fm(float a, float b, float c) { return fmaf(a, b, c); }
+};
+template <> struct Prec<double> {
+  static constexpr int kListM = 64;
+  using V4 = double4;
+  static __device__ __forceinline__ double rsq(double x) { return rsqrt(x); }
+  static __device__ __forceinline__ double fm(double a, double b, double c) { return fma(a, b, c); }
+};
+
+template <typename R>
+struct Acc {
+  R s[4];  // running sums   ex, ey, ez, pot
+  R c[4];  // compensation (FP32 only)
+  R p[4];  // partial sums of the current batch
+};
+
+// M2P: entry = {dx,dy,dz,q | px,py,pz,2qx | 2qy,2qz,xy,yz | zx,tr,-,-}; the
+// separation d is target minus centre of charge.
+template <typename R>
+__device__ __forceinline__ void m2p(R dx, R dy, R dz, const typename Prec<R>::V4& b,
+                                    const typename Prec<R>::V4& c, const typename Prec<R>::V4& d,
+                                    R q, R eps2, R* p) {
+  using P = Prec<R>;
+  const R r2 = P::fm(dz, dz, P::fm(dy, dy, dx * dx)) + eps2;
+  const R rd = P::rsq(r2);
+  const R rd2 = rd * rd;
+  const R rd3 = rd * rd2;
+  const R rd5 = rd3 * rd2;
+  const R rd7 = rd5 * rd2;
+  const R px = b.x, py = b.y, pz = b.z, qx2 = b.w;
+  const R qy2 = c.x, qz2 = c.y, xy = c.z, yz = c.w;
+  const R zx = d.x, tr = d.y;
+  const R dp = P::fm(dz, pz, P::fm(dy, py, dx * px));
+  const R gx = P::fm(qx2, dx, P::fm(xy, dy, zx * dz));
+  const R gy = P::fm(qy2, dy, P::fm(xy, dx, yz * dz));
+  const R gz = P::fm(qz2, dz, P::fm(yz, dy, zx * dx));
+  const R s2 = P::fm(dz, gz, P::fm(dy, gy, dx * gx));   // 2 d.A.d
+  const R dpt = dp + tr;
+  const R m3rd5 = R(-3) * rd5;
+  p[3] += P::fm(q, rd, P::fm(-rd3, dpt, R(1.5) * rd5 * s2));
+  const R cc = P::fm(q, rd3, P::fm(m3rd5, dpt, R(7.5) * rd7 * s2));
+  p[0] += P::fm(cc, dx, P::fm(rd3, px, m3rd5 * gx));
+  p[1] += P::fm(cc, dy, P::fm(rd3, py, m3rd5 * gy));
+  p[2] += P::fm(cc, dz, P::fm(rd3, pz, m3rd5 * gz));
+}
+
+// P2P (module_coulomb_kernels.f90:242-257): phi = q/r, E = q d / r^3.
+template <typename R>
+__device__ __forceinline__ void p2p(R dx, R dy, R dz, R q, R eps2, bool self, R* p) {
+  using P = Prec<R>;
+  R r2 = P::fm(dz, dz, P::fm(dy, dy, dx * dx)) + eps2;
+  r2 = self ? R(1) : r2;
+  const R qe = self ? R(0) : q;
+  const R rd = P::rsq(r2);
+  const R qrd = qe * rd;
+  const R qrd3 = qrd * rd * rd;
+  p[3] += qrd;
+  p[0] = P::fm(qrd3, dx, p[0]);
+  p[1] = P::fm(qrd3, dy, p[1]);
+  p[2] = P::fm(qrd3, dz, p[2]);
+}
+
+template <typename R>
+__device__ __forceinline__ void fold(Acc<R>& a) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if constexpr (sizeof(R) == 4) {   // Kahan step on the batch's partial sum
+      const R y = a.p[k] - a.c[k];
+      const R t = a.s[k] + y;
+      a.c[k] = (t - a.s[k]) - y;
+      a.s[k] = t;
+    } else {
+      a.s[k] += a.p[k];
+    }
+    a.p[k] = R(0);
+  }
+}
+
+// Exact reference MAC for one target, FP64 without contraction.
+__device__ __forceinline__ bool mac_fails(double xt, double yt, double zt, double cx, double cy,
+                                          double cz, double b2, double theta2) {
+#pragma clang fp contract(off)
+  const double dx = xt - cx, dy = yt - cy, dz = zt - cz;
+  const double d2 = dx * dx + dy * dy + dz * dz;
+  return !(theta2 * d2 > b2);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
+  using P = Prec<R>;
+  using V4 = typename P::V4;
+  constexpr int LM = P::kListM;
+  __shared__ uint32_t stack[kStackCap + kStackSlack];
+  __shared__ V4 lm[LM][4];
+  __shared__ V4 lp[kListP];
+  __shared__ uint32_t lpi[kListP];
+  __shared__ uint32_t gsh;
+
+  const int lane = threadIdx.x;
+  const unsigned long long lt = lanemask_lt();
+  const R eps2 = (R)a.eps2;
+  const double theta2 = a.theta2;
+  const uint32_t ngroups = a.sc->ngroups;
+
+  for (;;) {
+    if (lane == 0) gsh = atomicAdd(&a.sc->group_counter, 1u);
+    __syncthreads();
+    const uint32_t g = gsh;
+    __syncthreads();
+    if (g >= ngroups) break;
+
+    const uint32_t a0 = a.gfirst[g];
+    const uint32_t cnt = a.gcount[g];
+    uint32_t T = 1;
+    while (T < cnt) T <<= 1;
+    const uint32_t S = kWave / T;
+    const uint32_t t = lane & (T - 1);
+    const uint32_t s = lane / T;
+    const bool valid = t < cnt;
+    const uint32_t tid = a0 + (valid ? t : cnt - 1);
+    const double xt = a.x[tid], yt = a.y[tid], zt = a.z[tid];
+    const double ox = a.x[a0], oy = a.y[a0], oz = a.z[a0];
+    const double lox = wave_min(xt), loy = wave_min(yt), loz = wave_min(zt);
+    const double hix = wave_max(xt), hiy = wave_max(yt), hiz = wave_max(zt);
+    // target position in the kernel's frame
+    R tx, ty, tz;
+    if constexpr (sizeof(R) == 4) {
+      tx = (R)(xt - ox); ty = (R)(yt - oy); tz = (R)(zt - oz);
+    } else {
+      tx = xt; ty = yt; tz = zt;
+    }
+
+    Acc<R> acc;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { acc.s[k] = R(0); acc.c[k] = R(0); acc.p[k] = R(0); }
+
+    uint32_t sp = 1, nm = 0, np = 0, cnt_p2p = 0, cnt_m2p = 0;
+    unsigned long long nmac = 0;
+    bool overflow = false;
+    if (lane == 0) stack[0] = 0;
+    __syncthreads();
+
+    auto flush_m2p = [&]() {
+      for (uint32_t k = s; k < nm; k += S) {
+        const V4 e0 = lm[k][0], e1 = lm[k][1], e2 = lm[k][2], e3 = lm[k][3];
+        m2p<R>(tx - (R)e0.x, ty - (R)e0.y, tz - (R)e0.z, e1, e2, e3, (R)e0.w, eps2, acc.p);
+      }
+      fold(acc);
+    };
+    auto flush_p2p = [&]() {
+      for (uint32_t k = s; k < np; k += S) {
+        const V4 e = lp[k];
+        const bool self = lpi[k] == tid;
+        p2p<R>(tx - (R)e.x, ty - (R)e.y, tz - (R)e.z, (R)e.w, eps2, self, acc.p);
+      }
+      fold(acc);
+    };
+
+    while (sp > 0) {
+      int room = ((int)kStackCap - (int)sp) / 7;
+      room = room < 1 ? 1 : room;
+      uint32_t m = sp < (uint32_t)room ? sp : (uint32_t)room;
+      m = m > (uint32_t)kWave ? (uint32_t)kWave : m;
+      if (nm + m > (uint32_t)LM) { __syncthreads(); flush_m2p(); nm = 0; }
+      if (np + m > (uint32_t)kListP) { __syncthreads(); flush_p2p(); np = 0; }
+      const bool has = (uint32_t)lane < m;
+      const uint32_t node = has ? stack[sp - 1 - lane] : 0u;
+      __syncthreads();
+      sp -= m;
+
+      NodeGeom gm = {0.0, 0.0, 0.0, 0.0};
+      NodeLink lk = {0u, 0u, 0u, 0u};
+      if (has) { gm = a.geom[node]; lk = a.link[node]; }
+      const bool leaf = has && lk.nchild == 0u;
+      const bool twig = has && lk.nchild != 0u;
+
+      // Decide from the group's bounding box when possible (margins cover
+      // every rounding of the exact per-target test below).
+      bool acc_ok = false, rej = false;
+      uint32_t macc = 0;
+      if (twig) {
+        const double ex = fmax(fmax(lox - gm.cx, gm.cx - hix), 0.0);
+        const double ey = fmax(fmax(loy - gm.cy, gm.cy - hiy), 0.0);
+        const double ez = fmax(fmax(loz - gm.cz, gm.cz - hiz), 0.0);
+        const double fx = fmax(fabs(gm.cx - lox), fabs(gm.cx - hix));
+        const double fy = fmax(fabs(gm.cy - loy), fabs(gm.cy - hiy));
+        const double fz = fmax(fabs(gm.cz - loz), fabs(gm.cz - hiz));
+        const double dmin2 = ex * ex + ey * ey + ez * ez;
+        const double dmax2 = fx * fx + fy * fy + fz * fz;
+        acc_ok = (theta2 * dmin2) * (1.0 - 1e-12) > gm.b2;
+        rej = !acc_ok && (theta2 * dmax2) * (1.0 + 1e-12) <= gm.b2;
+        macc = acc_ok ? cnt : (rej ? 1u : 0u);
+      }
+      unsigned long long amb = __ballot(twig && !acc_ok && !rej);
+      while (amb) {
+        const int j = __ffsll((long long)amb) - 1;
+        amb &= amb - 1ull;
+        const double cx = __shfl(gm.cx, j), cy = __shfl(gm.cy, j), cz = __shfl(gm.cz, j);
+        const double b2 = __shfl(gm.b2, j);
+        const bool fail = mac_fails(xt, yt, zt, cx, cy, cz, b2, theta2);
+        const unsigned long long fm = __ballot(fail && (uint32_t)lane < cnt);
+        if (lane == j) {
+          if (fm == 0ull) { acc_ok = true; macc = cnt; }
+          else { rej = true; macc = (uint32_t)__ffsll((long long)fm); }
+        }
+      }
+      nmac += (unsigned long long)wave_sum(twig ? macc : 0u);
+
+      const bool take_m = twig && acc_ok;
+      const unsigned long long am = __ballot(take_m);
+      const unsigned long long pm = __ballot(leaf);
+      if (take_m) {
+        const uint32_t pos = nm + (uint32_t)popc64(am & lt);
+        const V4* src;
+        V4 e0;
+        if constexpr (sizeof(R) == 4) {
+          src = reinterpret_cast<const V4*>(&a.wm32[node]);
+          e0 = make_float4((float)(gm.cx - ox), (float)(gm.cy - oy), (float)(gm.cz - oz), 0.0f);
+        } else {
+          src = reinterpret_cast<const V4*>(&a.wm64[node]);
+          e0 = make_double4(gm.cx, gm.cy, gm.cz, 0.0);
+        }
+        const V4 w0 = src[0], w1 = src[1], w2 = src[2];
+        // w0 = {q, px, py, pz}, w1 = {2qx, 2qy, 2qz, xy}, w2 = {yz, zx, tr, -}
+        e0.w = w0.x;
+        V4 e1, e2, e3;
+        e1.x = w0.y; e1.y = w0.z; e1.z = w0.w; e1.w = w1.x;
+        e2.x = w1.y; e2.y = w1.z; e2.z = w1.w; e2.w = w2.x;
+        e3.x = w2.y; e3.y = w2.z; e3.z = 0; e3.w = 0;
+        lm[pos][0] = e0; lm[pos][1] = e1; lm[pos][2] = e2; lm[pos][3] = e3;
+      }
+      if (leaf) {
+        const uint32_t pos = np + (uint32_t)popc64(pm & lt);
+        V4 e;
+        if constexpr (sizeof(R) == 4) {
+          e = make_float4((float)(gm.cx - ox), (float)(gm.cy - oy), (float)(gm.cz - oz),
+                          a.wm32[node].v[0]);
+        } else {
+          e = make_double4(gm.cx, gm.cy, gm.cz, a.wm64[node].v[0]);
+        }
+        lp[pos] = e;
+        lpi[pos] = lk.first;
+      }
+      const uint32_t nma = (uint32_t)popc64(am), npa = (uint32_t)popc64(pm);
+      nm += nma; np += npa; cnt_m2p += nma; cnt_p2p += npa;
+
+      const uint32_t nc = (twig && !acc_ok) ? lk.nchild : 0u;
+      const uint32_t incl = wave_inclusive_scan(nc);
+      const uint32_t tot = __shfl(incl, kWave - 1);
+      if (sp + tot > (uint32_t)(kStackCap + kStackSlack)) { overflow = true; break; }
+      for (uint32_t k = 0; k < nc; ++k) stack[sp + incl - nc + k] = lk.first + k;
+      sp += tot;
+      __syncthreads();
+    }
+    __syncthreads();
+    if (nm) flush_m2p();
+    if (np) flush_p2p();
+
+    // combine slices (target t lives in lanes t, t+T, t+2T, ...) in FP64
+    double out[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double v = (double)acc.s[k] - (double)acc.c[k];
+      for (uint32_t o = T; o < (uint32_t)kWave; o <<= 1) v += __shfl_xor(v, (int)o);
+      out[k] = v * a.sign;
+    }
+    if (overflow) {
+      if (lane == 0) atomicOr(&a.sc->flags, kFlagStackOverflow);
+    }
+    if (s == 0 && valid) {
+      a.ex[tid] = out[0];
+      a.ey[tid] = out[1];
+      a.ez[tid] = out[2];
+      a.pot[tid] = out[3];
+      a.work[tid] = 1.0 + (double)(cnt_p2p + cnt_m2p);
+      a.np2p[tid] = cnt_p2p;
+      a.nm2p[tid] = cnt_m2p;
+    }
+    if (lane == 0) {
+      a.gmac[g] = (uint32_t)nmac;
+      atomicAdd(&a.sc->interactions, (unsigned long long)cnt * (cnt_p2p + cnt_m2p));
+      atomicAdd(&a.sc->mac_evals, nmac);
+      atomicAdd(&a.sc->p2p, (unsigned long long)cnt * cnt_p2p);
+      atomicAdd(&a.sc->m2p, (unsigned long long)cnt * cnt_m2p);
+    }
+    __syncthreads();
+  }
+}
+
+// FP64 direct sum (src/utils/module_directsum.f90:34-172): all sources
+// against a list of targets, self excluded by index, reference P2P form.
+__global__ __launch_bounds__(256) void direct_kernel(int64_t n, const double* __restrict__ x,
+                                                    const double* __restrict__ y,
+                                                    const double* __restrict__ z,
+                                                    const double* __restrict__ q, int64_t ntest,
+                                                    const int64_t* __restrict__ testidx, double eps2,
+                                                    double sign, double* __restrict__ ex,
+                                                    double* __restrict__ ey, double* __restrict__ ez,
+                                                    double* __restrict__ pot) {
+#pragma clang fp contract(off)
+  __shared__ double sx[256], sy[256], sz[256], sq[256];
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t tj = j < ntest ? testidx[j] : -1;
+  const double xt = tj >= 0 ? x[tj] : 0.0, yt = tj >= 0 ? y[tj] : 0.0, zt = tj >= 0 ? z[tj] : 0.0;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t base = 0; base < n; base += 256) {
+    const int64_t i = base + threadIdx.x;
+    __syncthreads();
+    sx[threadIdx.x] = i < n ? x[i] : 0.0;
+    sy[threadIdx.x] = i < n ? y[i] : 0.0;
+    sz[threadIdx.x] = i < n ? z[i] : 0.0;
+    sq[threadIdx.x] = i < n ? q[i] : 0.0;
+    __syncthreads();
+    const int lim = (n - base) < 256 ? (int)(n - base) : 256;
+    double part[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < lim; ++k) {
+      if (base + k == tj) continue;
+      const double dx = xt - sx[k], dy = yt - sy[k], dz = zt - sz[k];
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      const double r = sqrt(d2 + eps2);
+      const double rd = 1.0 / r;
+      const double rd3q = sq[k] * rd * rd * rd;
+      part[3] += sq[k] * rd;
+      part[0] += rd3q * dx;
+      part[1] += rd3q * dy;
+      part[2] += rd3q * dz;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] += part[c];
+  }
+  if (j < ntest) {
+    ex[j] = sign * acc[0];
+    ey[j] = sign * acc[1];
+    ez[j] = sign * acc[2];
+    pot[j] = sign * acc[3];
+  }
+}
+
+}  // namespace
+
+int walk_default_blocks() {
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
+  }
+  int per = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<float>, kWave, 0) != hipSuccess ||
+      per < 1)
+    per = 8;
+  return cus * per;
+}
+
+void launch_walk(hipStream_t s, const WalkArgs& a, int precision, int blocks) {
+  if (precision == kWalkFP64)
+    walk_kernel<double><<<blocks, kWave, 0, s>>>(a);
+  else
+    walk_kernel<float><<<blocks, kWave, 0, s>>>(a);
+}
+
+void launch_direct(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
+                   const double* q, int64_t ntest, const int64_t* testidx, double eps2, double sign,
+                   double* ex, double* ey, double* ez, double* pot) {
+  if (ntest <= 0) return;
+  direct_kernel<<<(unsigned)((ntest + 255) / 256), 256, 0, s>>>(n, x, y, z, q, ntest, testidx, eps2,
+                                                                sign, ex, ey, ez, pot);
+}
+
+}  // namespace pepc

@@ -1,0 +1,175 @@
+"""Python front end of the GPU tree solve (thin layer over the C-ABI).
+
+``Solver`` owns one ``pepc_gpu`` context.  Device memory for inputs and
+outputs comes from torch (plumbing only); all computation happens in
+``libpepc_gpu.so`` on torch's current stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+@dataclass
+class TraverseResult:
+    e: "object"        # (n, 3) float64 tensor, sorted order
+    pot: "object"
+    work: "object"
+
+
+class Solver:
+    """One GPU context: ``grow`` then ``traverse``, as pepc_grow_tree and
+    pepc_traverse_tree (module_pepc.f90:435-468)."""
+
+    def __init__(self, theta2=0.36, eps2=0.0, group_size=16, precision="fp32",
+                 sign=1.0, node_factor=2.5, force_cubic_domain=False, walk_blocks=0,
+                 device=0):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("pepc_amd.Solver needs a GPU (no HIP device visible)")
+        self.torch = torch
+        self.lib = _lib.load()
+        prec = {"fp32": _lib.FP32_KAHAN, "fp64": _lib.FP64}[precision] \
+            if isinstance(precision, str) else int(precision)
+        self.cfg = _lib.default_config(theta2=float(theta2), eps2=float(eps2),
+                                       group_size=int(group_size), precision=prec,
+                                       sign=float(sign), node_factor=float(node_factor),
+                                       force_cubic_domain=int(bool(force_cubic_domain)),
+                                       walk_blocks=int(walk_blocks), device=int(device))
+        self.device = torch.device("cuda", device)
+        h = C.c_void_p()
+        rc = self.lib.pepc_gpu_create(C.byref(h), C.byref(self.cfg))
+        if rc:
+            raise _lib.PepcGpuError(rc, "pepc_gpu_create failed")
+        self.h = h
+        self.n = 0
+        self._bind_stream()
+
+    def _bind_stream(self):
+        s = self.torch.cuda.current_stream(self.device)
+        self._check(self.lib.pepc_gpu_set_stream(self.h, C.c_void_p(s.cuda_stream)))
+
+    def _check(self, rc):
+        if rc:
+            buf = C.create_string_buffer(512)
+            self.lib.pepc_gpu_last_error(self.h, buf, 512)
+            raise _lib.PepcGpuError(rc, buf.value.decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pepc_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ core
+    def _dev(self, a):
+        t = self.torch
+        if isinstance(a, np.ndarray):
+            return t.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(self.device)
+        return a.to(device=self.device, dtype=t.float64).contiguous()
+
+    def grow(self, x, y, z, q, sync=True):
+        """Builds the tree.  Inputs: numpy arrays or device tensors (float64)."""
+        self._bind_stream()
+        xs = [self._dev(v) for v in (x, y, z, q)]
+        self._inputs = xs      # keep alive until the stream has consumed them
+        n = xs[0].numel()
+        self._check(self.lib.pepc_gpu_grow_device(self.h, n, *[_ptr(v) for v in xs]))
+        self.n = n
+        if sync:
+            return self.sync()
+        return None
+
+    def traverse(self, out=None, sync=True):
+        """Fields, potentials and work for the grown particles (sorted order)."""
+        t = self.torch
+        n = self.n
+        if out is None:
+            out = TraverseResult(t.empty((3, n), dtype=t.float64, device=self.device),
+                                 t.empty(n, dtype=t.float64, device=self.device),
+                                 t.empty(n, dtype=t.float64, device=self.device))
+        e = out.e
+        self._check(self.lib.pepc_gpu_traverse_device(self.h, _ptr(e[0]), _ptr(e[1]), _ptr(e[2]),
+                                                      _ptr(out.pot), _ptr(out.work)))
+        if sync:
+            self.sync()
+        return out
+
+    def sync(self) -> dict:
+        st = _lib.Stats()
+        self._check(self.lib.pepc_gpu_sync(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def timings(self):
+        a, b = C.c_double(), C.c_double()
+        self._check(self.lib.pepc_gpu_timings(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    # ------------------------------------------------------------- read-back
+    def sorted_host(self):
+        """(keys uint64, perm int64, x, y, z, q) of the last grow, on the host."""
+        n = self.n
+        keys = np.empty(n, np.uint64)
+        perm = np.empty(n, np.int64)
+        arrs = [np.empty(n) for _ in range(4)]
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)
+        self._check(self.lib.pepc_gpu_read_sorted(self.h, vp(keys), vp(perm), *[vp(a) for a in arrs]))
+        return (keys, perm, *arrs)
+
+    def read_tree(self):
+        st = self.sync()
+        m = st["nnodes"]
+        link = np.empty((m, 4), np.uint32)
+        first = np.empty(m, np.uint32)
+        mom = np.empty((m, 16))
+        nn = C.c_int64()
+        self._check(self.lib.pepc_gpu_read_tree(self.h, m, link.ctypes.data_as(C.c_void_p),
+                                                first.ctypes.data_as(C.c_void_p),
+                                                mom.ctypes.data_as(C.c_void_p), C.byref(nn)))
+        return {"link": link, "first": first, "mom": mom, "stats": st}
+
+    def read_walk(self):
+        st = self.sync()
+        g = st["ngroups"]
+        gf = np.empty(g, np.uint32)
+        gc = np.empty(g, np.uint32)
+        gm = np.empty(g, np.uint32)
+        p2p = np.empty(self.n, np.uint32)
+        m2p = np.empty(self.n, np.uint32)
+        ng = C.c_int64()
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)
+        self._check(self.lib.pepc_gpu_read_walk(self.h, g, vp(gf), vp(gc), vp(gm), vp(p2p), vp(m2p),
+                                                C.byref(ng)))
+        return {"gfirst": gf, "gcount": gc, "gmac": gm, "p2p": p2p, "m2p": m2p, "stats": st}
+
+    def direct(self, x, y, z, q, testidx, eps2=None, sign=None):
+        """FP64 direct sum at the given targets (device computation)."""
+        t = self.torch
+        xs = [self._dev(v) for v in (x, y, z, q)]
+        idx = np.asarray(testidx, dtype=np.int64)
+        if idx.size and (idx.min() < 0 or idx.max() >= xs[0].numel()):
+            raise ValueError("direct: target index out of range")
+        ti = t.as_tensor(idx).to(self.device)
+        m = ti.numel()
+        out = [t.empty(m, dtype=t.float64, device=self.device) for _ in range(4)]
+        self._bind_stream()
+        self._check(self.lib.pepc_gpu_direct_device(
+            self.h, xs[0].numel(), *[_ptr(v) for v in xs], m, _ptr(ti),
+            float(self.cfg.eps2 if eps2 is None else eps2),
+            float(self.cfg.sign if sign is None else sign), *[_ptr(o) for o in out]))
+        t.cuda.synchronize(self.device)
+        return t.stack(out[:3], 1), out[3]
+

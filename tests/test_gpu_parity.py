@@ -1,0 +1,172 @@
+"""GPU parity: the HIP path (through the C-ABI) against the oracle on the
+same seeded inputs.  Integer work (keys, permutation, tree topology, groups,
+interaction and MAC counts) must be bit-exact; FP64 moments are bit-exact
+(same operation order, no contraction); forces agree to the stated
+tolerances."""
+import numpy as np
+import pytest
+
+from pepc_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from pepc_amd.solver import Solver
+    return Solver
+
+
+SYSTEMS = {
+    "mini": lambda: workloads.mini(),                      # 32768, +-1 charges
+    "explosion64k": lambda: workloads.explosion(n=65536, seed=21),
+    "plummer64k": lambda: workloads.plummer(n=65536, seed=22),
+}
+
+_cache = {}
+
+
+def _oracle(orc, name, G):
+    key = (name, G)
+    if key not in _cache:
+        s = SYSTEMS[name]()
+        d = orc.domain_decompose(s.x, s.y, s.z, s.q)
+        t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
+        w = orc.walk_group(d.x, d.y, d.z, t, s.theta2, s.eps2, tile_size=G, sign=s.sign,
+                           keys=d.keys)
+        _cache[key] = (s, d, t, w)
+    return _cache[key]
+
+
+def _run(gpu, s, G, precision):
+    sol = gpu(theta2=s.theta2, eps2=s.eps2, group_size=G, precision=precision, sign=s.sign)
+    st = sol.grow(s.x, s.y, s.z, s.q)
+    res = sol.traverse()
+    return sol, st, res
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_keys_permutation_tree_bitexact(gpu, orc, name):
+    s, d, t, _ = _oracle(orc, name, 16)
+    sol, st, _ = _run(gpu, s, 16, "fp64")
+    keys, perm, x, y, z, q = sol.sorted_host()
+    print(f"[{name}] n={s.n} nodes={st['nnodes']} twigs={st['ntwig']}", flush=True)
+    assert np.array_equal(keys, d.keys)
+    assert np.array_equal(perm, d.perm)
+    assert np.array_equal(x, d.x) and np.array_equal(y, d.y) and np.array_equal(z, d.z)
+    assert st["nnodes"] == t.nnodes and st["ntwig"] == t.ntwig and st["nleaf"] == s.n
+    tr = sol.read_tree()
+    link, first, mom = tr["link"], tr["first"], tr["mom"]
+    # match nodes by (level, first particle): unique per node
+    og = np.lexsort((t.first, t.level))
+    gg = np.lexsort((first, link[:, 3]))
+    assert np.array_equal(t.level[og], link[gg, 3].astype(np.int32))
+    assert np.array_equal(t.first[og], first[gg].astype(np.int64))
+    assert np.array_equal(t.leaves[og], link[gg, 2].astype(np.int64))
+    assert np.array_equal((t.first_child[og] < 0), link[gg, 1] == 0)
+    assert np.array_equal(t.nchild[og], link[gg, 1].astype(np.int32))
+    # FP64 moments, bit for bit (15 used slots)
+    assert np.array_equal(t.mom[og][:, :15], mom[gg][:, :15])
+    sol.close()
+
+
+@pytest.mark.parametrize("G", [1, 16, 64])
+@pytest.mark.parametrize("name", ["mini", "plummer64k"])
+def test_groups_and_counts_bitexact(gpu, orc, name, G):
+    s, d, t, w = _oracle(orc, name, G)
+    sol, st, res = _run(gpu, s, G, "fp64")
+    wk = sol.read_walk()
+    print(f"[{name} G={G}] groups={st['ngroups']} interactions={wk['stats']['interactions']}",
+          flush=True)
+    assert np.array_equal(wk["gfirst"].astype(np.int64), w.tile_first)
+    assert np.array_equal(wk["gcount"].astype(np.int64), w.tile_count)
+    assert np.array_equal(wk["p2p"].astype(np.int32), w.p2p)
+    assert np.array_equal(wk["m2p"].astype(np.int32), w.m2p)
+    assert np.array_equal(wk["gmac"].astype(np.int64), w.tile_mac)
+    assert wk["stats"]["interactions"] == w.num_interactions
+    assert wk["stats"]["mac_evaluations"] == w.num_mac
+    assert np.array_equal(res.work.cpu().numpy(), w.work)
+    sol.close()
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_fp64_forces_match_oracle(gpu, orc, name):
+    s, d, t, w = _oracle(orc, name, 16)
+    sol, st, res = _run(gpu, s, 16, "fp64")
+    e = res.e.cpu().numpy().T
+    pot = res.pot.cpu().numpy()
+    scale = np.abs(w.e).max()
+    # same terms, different summation order and rsqrt rounding: 1e-12 relative
+    assert np.max(np.abs(e - w.e)) <= 1e-12 * scale
+    assert np.max(np.abs(pot - w.pot)) <= 1e-12 * np.abs(w.pot).max()
+    sol.close()
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_fp32_kahan_forces_close_to_fp64_tree(gpu, orc, name):
+    s, d, t, w = _oracle(orc, name, 16)
+    sol, st, res = _run(gpu, s, 16, "fp32")
+    e = res.e.cpu().numpy().T
+    rel = np.linalg.norm(e - w.e, axis=1) / np.linalg.norm(w.e, axis=1)
+    prel = np.abs(res.pot.cpu().numpy() - w.pot) / np.abs(w.pot).max()
+    print(f"[{name}] fp32 vs fp64 tree: median {np.median(rel):.2e} max {rel.max():.2e}", flush=True)
+    # FP32 arithmetic on identical interaction lists: rounding level only
+    assert np.median(rel) < 2e-6
+    assert np.quantile(rel, 0.999) < 1e-4
+    assert prel.max() < 1e-5
+    wk = sol.read_walk()
+    assert np.array_equal(wk["p2p"].astype(np.int32), w.p2p)
+    sol.close()
+
+
+def test_direct_sum_matches_oracle(gpu, orc):
+    s = workloads.mini(n=4096, seed=31)
+    sol = gpu(theta2=0.36, eps2=s.eps2)
+    idx = np.array([0, 5, 77, 4095])
+    e, pot = sol.direct(s.x, s.y, s.z, s.q, idx)
+    oe, op = orc.direct(s.x, s.y, s.z, s.q, idx, s.eps2)
+    assert np.allclose(e.cpu().numpy(), oe, rtol=1e-12, atol=1e-12 * np.abs(oe).max())
+    assert np.allclose(pot.cpu().numpy(), op, rtol=1e-12)
+    sol.close()
+
+
+def test_coincident_particles_dupfix(gpu, orc):
+    # pairs of particles at identical positions exercise the duplicate-key
+    # fix-up (module_domains.f90:248-307); equal charges, so the result does
+    # not depend on which particle of a tied pair is moved
+    r = np.random.default_rng(5)
+    p = r.random((3000, 3))
+    p[1000:1010] = p[0:10]
+    q = np.ones(3000)
+    d = orc.domain_decompose(p[:, 0], p[:, 1], p[:, 2], q)
+    assert d.nfixed > 0
+    sol = gpu(theta2=0.36, eps2=1e-4)
+    st = sol.grow(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), q)
+    keys, perm, x, y, z, _ = sol.sorted_host()
+    assert st["dup_fixed"] == 10
+    assert np.array_equal(keys, d.keys)
+    got = np.stack([x, y, z], 1)
+    want = np.stack([d.x, d.y, d.z], 1)
+    assert np.array_equal(got[np.lexsort(got.T)], want[np.lexsort(want.T)])
+    t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
+    assert st["nnodes"] == t.nnodes
+    sol.close()
+
+
+def test_error_paths(gpu):
+    from pepc_amd import _lib
+    sol = gpu()
+    x = np.array([0.0, np.nan, 1.0])
+    with pytest.raises(_lib.PepcGpuError) as ei:
+        sol.grow(x, x.copy(), x.copy(), np.ones(3))
+    assert ei.value.code == _lib.EBOX
+    sol.close()
+    sol = gpu(node_factor=1.0)   # too small a node pool for a real tree
+    s = workloads.mini(n=20000)
+    with pytest.raises(_lib.PepcGpuError) as ei:
+        sol.grow(s.x, s.y, s.z, s.q)
+    assert ei.value.code == _lib.EOVERFLOW
+    sol.close()
