@@ -113,12 +113,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=4194304)
     ap.add_argument("--theta", type=float, default=0.6)
-    ap.add_argument("--group", type=int, default=32)
+    ap.add_argument("--group", type=int, default=64)
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--err-samples", type=int, default=65536)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=1)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run one plain device copy of MIB MiB (reference kernel for "
+                         "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding "
@@ -130,6 +133,13 @@ def main():
     from pepc_amd import workloads
     s = workloads.explosion(n=args.n, theta=args.theta)
     dev = torch.device("cuda", 0)
+    if args.calibrate_copy:
+        a = torch.ones(args.calibrate_copy * (1 << 20) // 4, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {a.numel() * 4} bytes read, {a.numel() * 4} bytes written")
+        del a, b
     dev_inputs = [torch.from_numpy(a).to(dev) for a in (s.x, s.y, s.z, s.q)]
     log(f"workload explosion n={s.n} theta={args.theta} G={args.group} {args.precision}")
 

@@ -72,6 +72,8 @@ typedef struct {
     uint64_t interactions;      /* sum over targets of P2P + M2P (self included) */
     uint64_t mac_evaluations;   /* per-tile MAC tests, first failure ends a node */
     uint64_t p2p, m2p;
+    uint64_t amb_nodes;         /* twigs the group bounding box left undecided */
+    uint64_t walk_steps;        /* traversal steps of all groups */
     uint32_t flags;
     int32_t maxlevel;
     uint32_t level_offset[22];  /* first node of each level in the device layout */

@@ -10,7 +10,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpepc_gpu.so")
+# PEPC_GPU_LIB selects another build of the same C-ABI (e.g. a tuning variant)
+LIB_PATH = os.environ.get("PEPC_GPU_LIB") or os.path.join(HERE, "libpepc_gpu.so")
 
 FP32_KAHAN = 0
 FP64 = 1
@@ -38,7 +39,8 @@ class Stats(C.Structure):
     _fields_ = [("n", C.c_int64), ("nnodes", C.c_int64), ("nleaf", C.c_int64),
                 ("ntwig", C.c_int64), ("ngroups", C.c_int64), ("dup_fixed", C.c_int64),
                 ("interactions", C.c_uint64), ("mac_evaluations", C.c_uint64),
-                ("p2p", C.c_uint64), ("m2p", C.c_uint64), ("flags", C.c_uint32),
+                ("p2p", C.c_uint64), ("m2p", C.c_uint64), ("amb_nodes", C.c_uint64),
+                ("walk_steps", C.c_uint64), ("flags", C.c_uint32),
                 ("maxlevel", C.c_int32), ("level_offset", C.c_uint32 * 22)]
 
     def as_dict(self):

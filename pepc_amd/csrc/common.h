@@ -39,19 +39,22 @@ struct DevScalars {
   uint32_t nnodes;
   uint32_t ntwig;
   uint32_t ngroups;
-  uint32_t group_counter;       // work queue head of the persistent walk
+  uint32_t reserved;
   uint32_t maxlevel_used;
   uint32_t level_offset[kMaxLevel + 2];
   unsigned long long interactions;
   unsigned long long mac_evals;
   unsigned long long p2p;
   unsigned long long m2p;
+  unsigned long long amb_nodes;   // twigs the group box could not decide
+  unsigned long long walk_steps;  // traversal steps (batches of up to 64 nodes)
   uint32_t dup_count;
   uint32_t pad;
 };
 
 // Node record read by the walk for the acceptance test: centre of charge
-// and the squared box length of the node's level.
+// and the squared box length of the node's level (for a leaf, whose centre
+// is its particle, the last slot holds the particle's charge instead).
 struct __align__(32) NodeGeom {
   double cx, cy, cz, b2;
 };

@@ -29,6 +29,8 @@ void launch_gather4(hipStream_t s, int64_t n, const uint32_t* perm, const double
                     const double* z, const double* q, double* xs, double* ys, double* zs,
                     double* qs);
 
+enum WalkPrecision { kWalkFP32Kahan = 0, kWalkFP64 = 1 };
+
 // tree.hip
 struct TreeBuffers {
   int64_t n;
@@ -58,6 +60,7 @@ struct TreeBuffers {
   uint32_t* bsum;
   DevBox* box;
   DevScalars* sc;
+  int prec;                          // walk record form to write (WalkPrecision)
 };
 
 void launch_dup_detect(hipStream_t s, int64_t n, const unsigned long long* keys, DevScalars* sc);
@@ -65,6 +68,7 @@ void launch_dupfix(hipStream_t s, int64_t n, unsigned long long* keys, double* x
                    double* z, const DevBox* box, long long* kd, unsigned long long* oldkeys,
                    uint32_t* dflag, uint32_t* dlist, uint32_t* bsum, DevScalars* sc);
 void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size);
+void launch_leaf_moments(hipStream_t s, const TreeBuffers& t);
 
 // walk.hip
 struct WalkArgs {
@@ -85,10 +89,12 @@ struct WalkArgs {
   double* work;
   uint32_t* np2p;
   uint32_t* nm2p;
-  uint32_t* gmac;
+  uint32_t* gstat;                   // 8 words per group: mac, p2p, m2p, amb, steps, count
+  uint32_t* queue;                   // work-queue head (own cache line)
 };
-enum WalkPrecision { kWalkFP32Kahan = 0, kWalkFP64 = 1 };
+void launch_walk_reset(hipStream_t s, const WalkArgs& a);
 void launch_walk(hipStream_t s, const WalkArgs& a, int precision, int blocks);
+void launch_walk_stats(hipStream_t s, const WalkArgs& a);
 int walk_default_blocks();
 
 // direct.hip

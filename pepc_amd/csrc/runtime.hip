@@ -34,7 +34,7 @@ struct Ctx {
   uint8_t *lo = nullptr, *hi = nullptr;
   uint32_t *cnt = nullptr, *gmark = nullptr, *dflag = nullptr, *dlist = nullptr;
   long long* kd = nullptr;
-  uint32_t *gfirst = nullptr, *gcount = nullptr, *gmac = nullptr;
+  uint32_t *gfirst = nullptr, *gcount = nullptr, *gstat = nullptr, *queue = nullptr;
   uint32_t *np2p = nullptr, *nm2p = nullptr;
   double *res[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // host-path outputs
   // nodes
@@ -55,6 +55,7 @@ struct Ctx {
   DevScalars* sc = nullptr;
   DevScalars* sc_host = nullptr;    // pinned
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  TreeBuffers tb;                   // buffers of the last grow
 };
 
 int fail(Ctx* c, int code, const std::string& msg) {
@@ -98,7 +99,7 @@ int dalloc(Ctx* c, T** p, int64_t count) {
 void free_all(Ctx* c) {
   void* ptrs[] = {c->in[0], c->in[1], c->in[2], c->in[3], c->k0, c->k1, c->oldkeys, c->v0, c->v1,
                   c->xs, c->ys, c->zs, c->qs, c->lo, c->hi, c->cnt, c->gmark, c->dflag, c->dlist,
-                  c->kd, c->gfirst, c->gcount, c->gmac, c->np2p, c->nm2p, c->res[0], c->res[1],
+                  c->kd, c->gfirst, c->gcount, c->gstat, c->queue, c->np2p, c->nm2p, c->res[0], c->res[1],
                   c->res[2], c->res[3], c->res[4], c->dkey[0], c->dkey[1], c->dval[0], c->dval[1],
                   c->dpart, c->inv, c->flag, c->fc_rank, c->node_rank, c->nfirst, c->link, c->geom,
                   c->mom64, c->wm32, c->wm64, c->hist, c->bsum, c->partial, c->box, c->sc};
@@ -115,7 +116,7 @@ int ensure_particles(Ctx* c, int64_t n) {
   ALLOC(c, c->lo, nc); ALLOC(c, c->hi, nc);
   ALLOC(c, c->cnt, nc); ALLOC(c, c->gmark, nc); ALLOC(c, c->dflag, nc); ALLOC(c, c->dlist, nc);
   ALLOC(c, c->kd, nc);
-  ALLOC(c, c->gfirst, nc); ALLOC(c, c->gcount, nc); ALLOC(c, c->gmac, nc);
+  ALLOC(c, c->gfirst, nc); ALLOC(c, c->gcount, nc); ALLOC(c, c->gstat, 8 * nc);
   ALLOC(c, c->np2p, nc); ALLOC(c, c->nm2p, nc);
   const int64_t mc = (int64_t)(c->cfg.node_factor * (double)nc) + 64;
   ALLOC(c, c->dkey[0], mc); ALLOC(c, c->dkey[1], mc);
@@ -148,16 +149,6 @@ __global__ void reset_scalars_kernel(DevScalars* sc) {
     DevScalars z;
     memset(&z, 0, sizeof(z));
     *sc = z;
-  }
-}
-
-__global__ void reset_walk_kernel(DevScalars* sc) {
-  if (threadIdx.x == 0) {
-    sc->group_counter = 0;
-    sc->interactions = 0;
-    sc->mac_evals = 0;
-    sc->p2p = 0;
-    sc->m2p = 0;
   }
 }
 
@@ -202,6 +193,8 @@ int grow_device_impl(Ctx* c, int64_t n, const double* x, const double* y, const 
   t.mom64 = c->mom64; t.wm32 = c->wm32; t.wm64 = c->wm64;
   t.gmark = c->gmark; t.gfirst = c->gfirst; t.gcount = c->gcount;
   t.hist = c->hist; t.bsum = c->bsum; t.box = c->box; t.sc = c->sc;
+  t.prec = c->cfg.precision == PEPC_GPU_FP64 ? kWalkFP64 : kWalkFP32Kahan;
+  c->tb = t;
   launch_tree_build(s, t, c->cfg.group_size);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipEventRecord(c->ev[1], s));
@@ -215,8 +208,6 @@ int traverse_device_impl(Ctx* c, double* ex, double* ey, double* ez, double* pot
   if (!c->grown) return fail(c, PEPC_GPU_EINVAL, "traverse: no tree (call grow first)");
   if (!ex || !ey || !ez || !pot || !work) return fail(c, PEPC_GPU_EINVAL, "traverse: null output");
   hipStream_t s = c->stream;
-  reset_walk_kernel<<<1, 64, 0, s>>>(c->sc);
-  HIPCHECK(c, hipEventRecord(c->ev[2], s));
   WalkArgs a;
   a.n = c->n;
   a.x = c->xs; a.y = c->ys; a.z = c->zs; a.q = c->qs;
@@ -224,10 +215,13 @@ int traverse_device_impl(Ctx* c, double* ex, double* ey, double* ez, double* pot
   a.gfirst = c->gfirst; a.gcount = c->gcount; a.sc = c->sc;
   a.theta2 = c->cfg.theta2; a.eps2 = c->cfg.eps2; a.sign = c->cfg.sign;
   a.ex = ex; a.ey = ey; a.ez = ez; a.pot = pot; a.work = work;
-  a.np2p = c->np2p; a.nm2p = c->nm2p; a.gmac = c->gmac;
+  a.np2p = c->np2p; a.nm2p = c->nm2p; a.gstat = c->gstat; a.queue = c->queue;
+  launch_walk_reset(s, a);
+  HIPCHECK(c, hipEventRecord(c->ev[2], s));
   launch_walk(s, a, c->cfg.precision, c->walk_blocks);
-  HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipEventRecord(c->ev[3], s));
+  launch_walk_stats(s, a);
+  HIPCHECK(c, hipGetLastError());
   c->walked = true;
   return 0;
 }
@@ -248,6 +242,8 @@ int sync_impl(Ctx* c, pepc_gpu_stats* st) {
     st->mac_evaluations = h.mac_evals;
     st->p2p = h.p2p;
     st->m2p = h.m2p;
+    st->amb_nodes = h.amb_nodes;
+    st->walk_steps = h.walk_steps;
     st->flags = h.flags;
     int ml = 0;
     for (int l = 0; l <= kMaxLevel; ++l) {
@@ -303,6 +299,7 @@ int pepc_gpu_create(void** out, const pepc_gpu_config* cfg) {
       return PEPC_GPU_EHIP;
     }
   if (dalloc(c, &c->box, 1) || dalloc(c, &c->sc, 1) || dalloc(c, &c->partial, 1024 * 6) ||
+      dalloc(c, &c->queue, 64) ||
       hipHostMalloc((void**)&c->sc_host, sizeof(DevScalars), hipHostMallocDefault) != hipSuccess) {
     free_all(c);
     delete c;
@@ -473,6 +470,11 @@ int pepc_gpu_read_tree(void* p, int64_t cap, uint32_t* link, uint32_t* first, do
   const int64_t m = c->sc_host->nnodes;
   if (nnodes) *nnodes = m;
   if (m > cap) return fail(c, PEPC_GPU_EINVAL, "read_tree: capacity too small");
+  if (mom) {   // leaves keep no record during a solve; materialise them for the copy
+    launch_leaf_moments(c->stream, c->tb);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+  }
   if (link) HIPCHECK(c, hipMemcpy(link, c->link, sizeof(NodeLink) * m, hipMemcpyDeviceToHost));
   if (first) HIPCHECK(c, hipMemcpy(first, c->nfirst, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
   if (mom) HIPCHECK(c, hipMemcpy(mom, c->mom64, sizeof(double) * 16 * m, hipMemcpyDeviceToHost));
@@ -491,7 +493,14 @@ int pepc_gpu_read_walk(void* p, int64_t gcap, uint32_t* gfirst, uint32_t* gcount
   if (gfirst) HIPCHECK(c, hipMemcpy(gfirst, c->gfirst, sizeof(uint32_t) * g, hipMemcpyDeviceToHost));
   if (gcount) HIPCHECK(c, hipMemcpy(gcount, c->gcount, sizeof(uint32_t) * g, hipMemcpyDeviceToHost));
   if (c->walked) {
-    if (gmac) HIPCHECK(c, hipMemcpy(gmac, c->gmac, sizeof(uint32_t) * g, hipMemcpyDeviceToHost));
+    if (gmac && g > 0) {
+      uint32_t* tmp = new uint32_t[8 * g];
+      hipError_t e = hipMemcpy(tmp, c->gstat, sizeof(uint32_t) * 8 * g, hipMemcpyDeviceToHost);
+      if (e == hipSuccess)
+        for (int64_t i = 0; i < g; ++i) gmac[i] = tmp[8 * i];
+      delete[] tmp;
+      if (e != hipSuccess) return hip_fail(c, e, "copy group records");
+    }
     if (p2p) HIPCHECK(c, hipMemcpy(p2p, c->np2p, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost));
     if (m2p) HIPCHECK(c, hipMemcpy(m2p, c->nm2p, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost));
   }

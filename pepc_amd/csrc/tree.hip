@@ -212,9 +212,16 @@ __global__ void inv_kernel(const uint32_t* __restrict__ dval, uint32_t* __restri
     inv[dval[i]] = (uint32_t)i;
 }
 
+// Leaves carry no moment record: a leaf's multipole is its particle
+// (multipole_from_particle, coulomb/module_interaction_specific.f90:67-79), so
+// parents and the walk read the sorted particle arrays directly.  Only the
+// leaf's geometry record (centre = particle position) is written here.
 __global__ void node_fill_kernel(int64_t cap, const unsigned long long* __restrict__ dkey,
                                  const uint32_t* __restrict__ dval, const uint32_t* __restrict__ dpart,
                                  const uint32_t* __restrict__ inv, const uint8_t* __restrict__ hi,
+                                 const double* __restrict__ x, const double* __restrict__ y,
+                                 const double* __restrict__ z, const double* __restrict__ q,
+                                 NodeGeom* __restrict__ geom,
                                  NodeLink* __restrict__ link, uint32_t* __restrict__ nfirst,
                                  uint32_t* __restrict__ flag, const DevScalars* __restrict__ sc) {
   const int64_t m = tree_broken(sc) ? 0 : (int64_t)sc->nnodes;
@@ -230,6 +237,9 @@ __global__ void node_fill_kernel(int64_t cap, const unsigned long long* __restri
     if (l == hi[a]) {          // leaf
       lk.first = a; lk.nchild = 0; lk.count = 1;
       flag[i] = 0;
+      NodeGeom g;   // a leaf has no acceptance test: its b2 slot holds the charge
+      g.cx = x[a]; g.cy = y[a]; g.cz = z[a]; g.b2 = q[a];
+      geom[i] = g;
     } else {                   // twig: first child is this particle's next node
       lk.first = inv[d + 1]; lk.nchild = 0xFFFFFFFFu; lk.count = 0;
       flag[i] = 1;
@@ -266,27 +276,37 @@ __global__ void nchild_kernel(NodeLink* __restrict__ link, const uint32_t* __res
 // Moment record slots, reference layout (t_tree_node_interaction_data order).
 enum { M_CX = 0, M_CY, M_CZ, M_Q, M_AQ, M_PX, M_PY, M_PZ, M_QX, M_QY, M_QZ, M_XY, M_YZ, M_ZX, M_BMAX };
 
-__device__ void write_walk_forms(int64_t i, const double* m, double b2, NodeGeom* geom,
-                                 NodeMoments<float>* wm32, NodeMoments<double>* wm64) {
+__device__ __forceinline__ void write_walk_forms(int64_t i, const double* m, double b2, int prec,
+                                                 NodeGeom* geom, NodeMoments<float>* wm32,
+                                                 NodeMoments<double>* wm64) {
   NodeGeom g;
   g.cx = m[M_CX]; g.cy = m[M_CY]; g.cz = m[M_CZ]; g.b2 = b2;
   geom[i] = g;
-  NodeMoments<double> w;
-#pragma unroll
-  for (int k = 0; k < 16; ++k) w.v[k] = 0.0;
-  w.v[0] = m[M_Q];
-  w.v[1] = m[M_PX]; w.v[2] = m[M_PY]; w.v[3] = m[M_PZ];
-  w.v[4] = 2.0 * m[M_QX]; w.v[5] = 2.0 * m[M_QY]; w.v[6] = 2.0 * m[M_QZ];
-  w.v[7] = m[M_XY]; w.v[8] = m[M_YZ]; w.v[9] = m[M_ZX];
-  w.v[10] = (m[M_QX] + m[M_QY]) + m[M_QZ];
-  wm64[i] = w;
-  NodeMoments<float> f;
-#pragma unroll
-  for (int k = 0; k < 16; ++k) f.v[k] = (float)w.v[k];
-  wm32[i] = f;
+  double w[12];
+  w[0] = m[M_Q];
+  w[1] = m[M_PX]; w[2] = m[M_PY]; w[3] = m[M_PZ];
+  w[4] = 2.0 * m[M_QX]; w[5] = 2.0 * m[M_QY]; w[6] = 2.0 * m[M_QZ];
+  w[7] = m[M_XY]; w[8] = m[M_YZ]; w[9] = m[M_ZX];
+  w[10] = (m[M_QX] + m[M_QY]) + m[M_QZ];
+  w[11] = 0.0;
+  if (prec == kWalkFP64) {
+    double4* o = reinterpret_cast<double4*>(&wm64[i]);
+    o[0] = make_double4(w[0], w[1], w[2], w[3]);
+    o[1] = make_double4(w[4], w[5], w[6], w[7]);
+    o[2] = make_double4(w[8], w[9], w[10], w[11]);
+  } else {
+    float4* o = reinterpret_cast<float4*>(&wm32[i]);
+    o[0] = make_float4((float)w[0], (float)w[1], (float)w[2], (float)w[3]);
+    o[1] = make_float4((float)w[4], (float)w[5], (float)w[6], (float)w[7]);
+    o[2] = make_float4((float)w[8], (float)w[9], (float)w[10], 0.0f);
+  }
 }
 
-__global__ __launch_bounds__(kT) void upward_kernel(int level, NodeLink* link,
+// One launch per level, deepest first; a thread per twig.  Children are
+// contiguous; a leaf child's moments are its particle's (coc = x, charge = q,
+// |charge| = |q|, higher moments and bmax 0), a twig child's come from its
+// record of the level below.
+__global__ __launch_bounds__(kT) void upward_kernel(int level, int prec, NodeLink* link,
                                                    const double* __restrict__ x,
                                                    const double* __restrict__ y,
                                                    const double* __restrict__ z,
@@ -302,62 +322,92 @@ __global__ __launch_bounds__(kT) void upward_kernel(int level, NodeLink* link,
   for (int64_t i = b + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < e;
        i += (int64_t)gridDim.x * blockDim.x) {
     const NodeLink lk = link[i];
-    double m[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) m[k] = 0.0;
-    if (lk.nchild == 0) {
-      // multipole_from_particle, coulomb/module_interaction_specific.f90:67-79
-      const uint32_t p = lk.first;
-      m[M_CX] = x[p]; m[M_CY] = y[p]; m[M_CZ] = z[p];
-      m[M_Q] = q[p]; m[M_AQ] = fabs(q[p]);
-    } else {
-      const uint32_t c0 = lk.first, nc = lk.nchild;
-      double charge = 0.0, absq = 0.0;
-      uint32_t count = 0;
-      for (uint32_t j = 0; j < nc; ++j) {
-        charge = charge + mom[(int64_t)(c0 + j) * 16 + M_Q];
-        count += link[c0 + j].count;
-      }
-      for (uint32_t j = 0; j < nc; ++j) absq = absq + mom[(int64_t)(c0 + j) * 16 + M_AQ];
-      double coc[3] = {0.0, 0.0, 0.0};
-      if (absq != 0.0) {
-        for (uint32_t j = 0; j < nc; ++j) {
-          const double* c = mom + (int64_t)(c0 + j) * 16;
-          for (int d = 0; d < 3; ++d) coc[d] = coc[d] + (c[M_CX + d] * c[M_AQ]);
-        }
-        for (int d = 0; d < 3; ++d) coc[d] = coc[d] / absq;
-      } else {
-        for (uint32_t j = 0; j < nc; ++j) {
-          const double* c = mom + (int64_t)(c0 + j) * 16;
-          for (int d = 0; d < 3; ++d) coc[d] = coc[d] + c[M_CX + d];
-        }
-        for (int d = 0; d < 3; ++d) coc[d] = coc[d] / (double)nc;
-      }
-      double dip[3] = {0.0, 0.0, 0.0}, quad[3] = {0.0, 0.0, 0.0}, xy = 0.0, yz = 0.0, zx = 0.0;
-      double bmax = 0.0;
-      for (uint32_t j = 0; j < nc; ++j) {
-        const double* c = mom + (int64_t)(c0 + j) * 16;
-        double s[3];
-        for (int d = 0; d < 3; ++d) s[d] = coc[d] - c[M_CX + d];
-        for (int d = 0; d < 3; ++d) dip[d] = dip[d] + c[M_PX + d] + c[M_Q] * s[d];
-        for (int d = 0; d < 3; ++d)
-          quad[d] = quad[d] + c[M_QX + d] + c[M_PX + d] * s[d] + 0.5 * c[M_Q] * (s[d] * s[d]);
-        xy = xy + c[M_XY] + c[M_PX] * s[1] + c[M_PY] * s[0] + c[M_Q] * s[0] * s[1];
-        yz = yz + c[M_YZ] + c[M_PY] * s[2] + c[M_PZ] * s[1] + c[M_Q] * s[1] * s[2];
-        zx = zx + c[M_ZX] + c[M_PZ] * s[0] + c[M_PX] * s[2] + c[M_Q] * s[2] * s[0];
-        const double v = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + c[M_BMAX];
-        if (j == 0 || v > bmax) bmax = v;
-      }
-      m[M_CX] = coc[0]; m[M_CY] = coc[1]; m[M_CZ] = coc[2];
-      m[M_Q] = charge; m[M_AQ] = absq;
-      for (int d = 0; d < 3; ++d) { m[M_PX + d] = dip[d]; m[M_QX + d] = quad[d]; }
-      m[M_XY] = xy; m[M_YZ] = yz; m[M_ZX] = zx; m[M_BMAX] = bmax;
-      link[i].count = count;
+    if (lk.nchild == 0) continue;
+    const uint32_t c0 = lk.first, nc = lk.nchild;
+    // child j as (is_leaf, particle or node index)
+    uint32_t cidx[8];
+    uint32_t leafmask = 0, count = 0;
+    for (uint32_t j = 0; j < nc; ++j) {
+      const NodeLink cl = link[c0 + j];
+      if (cl.nchild == 0) { leafmask |= 1u << j; cidx[j] = cl.first; }
+      else cidx[j] = c0 + j;
+      count += cl.count;
     }
-    double* out = mom + i * 16;
+    auto val = [&](uint32_t j, int slot) -> double {
+      if (leafmask & (1u << j)) {
+        const uint32_t p = cidx[j];
+        switch (slot) {
+          case M_CX: return x[p];
+          case M_CY: return y[p];
+          case M_CZ: return z[p];
+          case M_Q: return q[p];
+          case M_AQ: return fabs(q[p]);
+          default: return 0.0;
+        }
+      }
+      return mom[(int64_t)cidx[j] * 16 + slot];
+    };
+    double charge = 0.0, absq = 0.0;
+    for (uint32_t j = 0; j < nc; ++j) charge = charge + val(j, M_Q);
+    for (uint32_t j = 0; j < nc; ++j) absq = absq + val(j, M_AQ);
+    double coc[3] = {0.0, 0.0, 0.0};
+    if (absq != 0.0) {
+      for (uint32_t j = 0; j < nc; ++j) {
+        const double aq = val(j, M_AQ);
+        for (int d = 0; d < 3; ++d) coc[d] = coc[d] + (val(j, M_CX + d) * aq);
+      }
+      for (int d = 0; d < 3; ++d) coc[d] = coc[d] / absq;
+    } else {
+      for (uint32_t j = 0; j < nc; ++j)
+        for (int d = 0; d < 3; ++d) coc[d] = coc[d] + val(j, M_CX + d);
+      for (int d = 0; d < 3; ++d) coc[d] = coc[d] / (double)nc;
+    }
+    double dip[3] = {0.0, 0.0, 0.0}, quad[3] = {0.0, 0.0, 0.0}, xy = 0.0, yz = 0.0, zx = 0.0;
+    double bmax = 0.0;
+    for (uint32_t j = 0; j < nc; ++j) {
+      double c[15];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) out[k] = m[k];
-    write_walk_forms(i, m, b2, geom, wm32, wm64);
+      for (int k = 0; k < 15; ++k) c[k] = val(j, k);
+      double s[3];
+      for (int d = 0; d < 3; ++d) s[d] = coc[d] - c[M_CX + d];
+      for (int d = 0; d < 3; ++d) dip[d] = dip[d] + c[M_PX + d] + c[M_Q] * s[d];
+      for (int d = 0; d < 3; ++d)
+        quad[d] = quad[d] + c[M_QX + d] + c[M_PX + d] * s[d] + 0.5 * c[M_Q] * (s[d] * s[d]);
+      xy = xy + c[M_XY] + c[M_PX] * s[1] + c[M_PY] * s[0] + c[M_Q] * s[0] * s[1];
+      yz = yz + c[M_YZ] + c[M_PY] * s[2] + c[M_PZ] * s[1] + c[M_Q] * s[1] * s[2];
+      zx = zx + c[M_ZX] + c[M_PZ] * s[0] + c[M_PX] * s[2] + c[M_Q] * s[2] * s[0];
+      const double v = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + c[M_BMAX];
+      if (j == 0 || v > bmax) bmax = v;
+    }
+    double m[16];
+    m[M_CX] = coc[0]; m[M_CY] = coc[1]; m[M_CZ] = coc[2];
+    m[M_Q] = charge; m[M_AQ] = absq;
+    for (int d = 0; d < 3; ++d) { m[M_PX + d] = dip[d]; m[M_QX + d] = quad[d]; }
+    m[M_XY] = xy; m[M_YZ] = yz; m[M_ZX] = zx; m[M_BMAX] = bmax; m[15] = 0.0;
+    link[i].count = count;
+    double2* out = reinterpret_cast<double2*>(mom + i * 16);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) out[k] = make_double2(m[2 * k], m[2 * k + 1]);
+    write_walk_forms(i, m, b2, prec, geom, wm32, wm64);
+  }
+}
+
+// Leaf records in the reference layout, for read-back only (not part of a
+// solve): coc = x, charge = q, abs_charge = |q|, the rest 0.
+__global__ void leaf_mom_kernel(const NodeLink* __restrict__ link, const double* __restrict__ x,
+                                const double* __restrict__ y, const double* __restrict__ z,
+                                const double* __restrict__ q, double* __restrict__ mom,
+                                const DevScalars* __restrict__ sc) {
+  if (tree_broken(sc)) return;
+  const int64_t m = sc->nnodes;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const NodeLink lk = link[i];
+    if (lk.nchild != 0) continue;
+    double* o = mom + i * 16;
+    for (int k = 0; k < 16; ++k) o[k] = 0.0;
+    o[M_CX] = x[lk.first]; o[M_CY] = y[lk.first]; o[M_CZ] = z[lk.first];
+    o[M_Q] = q[lk.first]; o[M_AQ] = fabs(q[lk.first]);
   }
 }
 
@@ -441,18 +491,23 @@ void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size) {
   level_offsets_kernel<<<1, 64, 0, s>>>(t.hist, (int)sort_blocks(cap), t.sc);
   inv_kernel<<<grid_for(cap), kT, 0, s>>>(t.dval[1], t.inv, t.sc);
   node_fill_kernel<<<grid_for(cap), kT, 0, s>>>(cap, t.dkey[1], t.dval[1], t.dpart, t.inv, t.hi,
-                                                t.link, t.nfirst, t.flag, t.sc);
+                                                t.x, t.y, t.z, t.q, t.geom, t.link, t.nfirst,
+                                                t.flag, t.sc);
   launch_scan(s, t.flag, t.flag, cap, t.bsum, &t.sc->ntwig);
   twig_rank_kernel<<<grid_for(cap), kT, 0, s>>>(t.link, t.flag, t.fc_rank, t.node_rank, t.sc);
   nchild_kernel<<<grid_for(cap), kT, 0, s>>>(t.link, t.fc_rank, t.node_rank, t.sc);
   for (int l = kMaxLevel; l >= 0; --l)
-    upward_kernel<<<grid_for(cap / 4), kT, 0, s>>>(l, t.link, t.x, t.y, t.z, t.q, t.mom64,
+    upward_kernel<<<grid_for(cap / 4), kT, 0, s>>>(l, t.prec, t.link, t.x, t.y, t.z, t.q, t.mom64,
                                                    t.geom, t.wm32, t.wm64, t.box, t.sc);
   gmark_clear_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark);
   group_mark_kernel<<<grid_for(cap), kT, 0, s>>>(group_size, t.link, t.nfirst, t.gmark, t.sc);
   group_flag_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark, t.cnt);
   launch_scan(s, t.cnt, t.cnt, n, t.bsum, &t.sc->ngroups);
   group_emit_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark, t.cnt, t.gfirst, t.gcount);
+}
+
+void launch_leaf_moments(hipStream_t s, const TreeBuffers& t) {
+  leaf_mom_kernel<<<grid_for(t.cap), kT, 0, s>>>(t.link, t.x, t.y, t.z, t.q, t.mom64, t.sc);
 }
 
 }  // namespace pepc

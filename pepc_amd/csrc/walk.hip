@@ -44,7 +44,10 @@ template <typename R> struct Prec;
 template <> struct Prec<float> {
   static constexpr int kListM = 128;
   using V4 = float4;
-  static __device__ __forceinline__ float rsq(float x) { return rsqrtf(x); }
+  // v_rsq_f32 (1 ulp).  Its argument is r^2 = d^2 + eps^2 of an interaction
+  // that is not a self term; the input guard of rsqrtf() only matters for
+  // denormal arguments, which would need |d| and eps below 1e-19.
+  static __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
   static __device__ __forceinline__ float fm(float a, float b, float c) { return fmaf(a, b, c); }
 };
 template <> struct Prec<double> {
@@ -61,46 +64,48 @@ struct Acc {
   R p[4];  // partial sums of the current batch
 };
 
-// M2P: entry = {dx,dy,dz,q | px,py,pz,2qx | 2qy,2qz,xy,yz | zx,tr,-,-}; the
-// separation d is target minus centre of charge.
+// M2P: entry = {cx,cy,cz,q | px,py,pz,2qx | 2qy,2qz,xy,yz | zx,tr,-,-}; the
+// separation d is target minus centre of charge.  With w = 1/r^2, the matrix
+// A2 = [[2qx,xy,zx],[xy,2qy,yz],[zx,yz,2qz]], g = A2 d, S2 = d.g, a = d.p + tr:
+//   phi = (1/r) (q - w (a - 1.5 w S2))
+//   E   = (w/r) (c d + p - 3 w g),   c = q - 3 w (a - 2.5 w S2)
+// which is phi and -grad phi of module_coulomb_kernels.f90:60-145.
 template <typename R>
 __device__ __forceinline__ void m2p(R dx, R dy, R dz, const typename Prec<R>::V4& b,
                                     const typename Prec<R>::V4& c, const typename Prec<R>::V4& d,
                                     R q, R eps2, R* p) {
   using P = Prec<R>;
-  const R r2 = P::fm(dz, dz, P::fm(dy, dy, dx * dx)) + eps2;
+  const R r2 = P::fm(dz, dz, P::fm(dy, dy, P::fm(dx, dx, eps2)));
   const R rd = P::rsq(r2);
-  const R rd2 = rd * rd;
-  const R rd3 = rd * rd2;
-  const R rd5 = rd3 * rd2;
-  const R rd7 = rd5 * rd2;
+  const R w = rd * rd;
+  const R rd3 = rd * w;
   const R px = b.x, py = b.y, pz = b.z, qx2 = b.w;
   const R qy2 = c.x, qz2 = c.y, xy = c.z, yz = c.w;
   const R zx = d.x, tr = d.y;
-  const R dp = P::fm(dz, pz, P::fm(dy, py, dx * px));
   const R gx = P::fm(qx2, dx, P::fm(xy, dy, zx * dz));
   const R gy = P::fm(qy2, dy, P::fm(xy, dx, yz * dz));
   const R gz = P::fm(qz2, dz, P::fm(yz, dy, zx * dx));
-  const R s2 = P::fm(dz, gz, P::fm(dy, gy, dx * gx));   // 2 d.A.d
-  const R dpt = dp + tr;
-  const R m3rd5 = R(-3) * rd5;
-  p[3] += P::fm(q, rd, P::fm(-rd3, dpt, R(1.5) * rd5 * s2));
-  const R cc = P::fm(q, rd3, P::fm(m3rd5, dpt, R(7.5) * rd7 * s2));
-  p[0] += P::fm(cc, dx, P::fm(rd3, px, m3rd5 * gx));
-  p[1] += P::fm(cc, dy, P::fm(rd3, py, m3rd5 * gy));
-  p[2] += P::fm(cc, dz, P::fm(rd3, pz, m3rd5 * gz));
+  const R s2 = P::fm(dz, gz, P::fm(dy, gy, dx * gx));
+  const R av = P::fm(dz, pz, P::fm(dy, py, P::fm(dx, px, tr)));
+  const R bv = w * s2;
+  p[3] = P::fm(rd, P::fm(-w, P::fm(R(-1.5), bv, av), q), p[3]);
+  const R w3 = R(3) * w;
+  const R cc = P::fm(-w3, P::fm(R(-2.5), bv, av), q);
+  p[0] = P::fm(rd3, P::fm(cc, dx, P::fm(-w3, gx, px)), p[0]);
+  p[1] = P::fm(rd3, P::fm(cc, dy, P::fm(-w3, gy, py)), p[1]);
+  p[2] = P::fm(rd3, P::fm(cc, dz, P::fm(-w3, gz, pz)), p[2]);
 }
 
 // P2P (module_coulomb_kernels.f90:242-257): phi = q/r, E = q d / r^3.
 template <typename R>
 __device__ __forceinline__ void p2p(R dx, R dy, R dz, R q, R eps2, bool self, R* p) {
   using P = Prec<R>;
-  R r2 = P::fm(dz, dz, P::fm(dy, dy, dx * dx)) + eps2;
+  R r2 = P::fm(dz, dz, P::fm(dy, dy, P::fm(dx, dx, eps2)));
   r2 = self ? R(1) : r2;
   const R qe = self ? R(0) : q;
   const R rd = P::rsq(r2);
   const R qrd = qe * rd;
-  const R qrd3 = qrd * rd * rd;
+  const R qrd3 = qrd * (rd * rd);
   p[3] += qrd;
   p[0] = P::fm(qrd3, dx, p[0]);
   p[1] = P::fm(qrd3, dy, p[1]);
@@ -149,22 +154,28 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
   const double theta2 = a.theta2;
   const uint32_t ngroups = a.sc->ngroups;
 
+  // Work queue: the first group of each wave is its block index, later ones
+  // come from a counter on a cache line of its own.
+  bool first = true;
   for (;;) {
-    if (lane == 0) gsh = atomicAdd(&a.sc->group_counter, 1u);
+    if (lane == 0) gsh = first ? blockIdx.x : gridDim.x + atomicAdd(a.queue, 1u);
+    first = false;
     __syncthreads();
     const uint32_t g = gsh;
     __syncthreads();
     if (g >= ngroups) break;
 
+    // Lanes s*cnt .. s*cnt+cnt-1 form slice s (s < S = 64/cnt) and hold the
+    // group's targets in order; the 64 - S*cnt remaining lanes hold copies
+    // of targets for the box and take no part in the sums.
     const uint32_t a0 = a.gfirst[g];
     const uint32_t cnt = a.gcount[g];
-    uint32_t T = 1;
-    while (T < cnt) T <<= 1;
+    const uint32_t T = cnt;
     const uint32_t S = kWave / T;
-    const uint32_t t = lane & (T - 1);
-    const uint32_t s = lane / T;
-    const bool valid = t < cnt;
-    const uint32_t tid = a0 + (valid ? t : cnt - 1);
+    const uint32_t s = (uint32_t)lane / T;
+    const uint32_t t = (uint32_t)lane - s * T;
+    const bool valid = s < S;
+    const uint32_t tid = a0 + t;
     const double xt = a.x[tid], yt = a.y[tid], zt = a.z[tid];
     const double ox = a.x[a0], oy = a.y[a0], oz = a.z[a0];
     const double lox = wave_min(xt), loy = wave_min(yt), loz = wave_min(zt);
@@ -181,24 +192,46 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) { acc.s[k] = R(0); acc.c[k] = R(0); acc.p[k] = R(0); }
 
-    uint32_t sp = 1, nm = 0, np = 0, cnt_p2p = 0, cnt_m2p = 0;
+    uint32_t sp = 1, nm = 0, np = 0, cnt_p2p = 0, cnt_m2p = 0, namb = 0, nsteps = 0;
     unsigned long long nmac = 0;
     bool overflow = false;
     if (lane == 0) stack[0] = 0;
     __syncthreads();
 
+    // Batches: entry k goes to slice k mod S; the loop count is uniform.
+    // FP32: two entries per pass (their LDS reads issued together), then the
+    // tail; FP64 keeps one entry per pass (register pressure).
     auto flush_m2p = [&]() {
-      for (uint32_t k = s; k < nm; k += S) {
-        const V4 e0 = lm[k][0], e1 = lm[k][1], e2 = lm[k][2], e3 = lm[k][3];
-        m2p<R>(tx - (R)e0.x, ty - (R)e0.y, tz - (R)e0.z, e1, e2, e3, (R)e0.w, eps2, acc.p);
+      uint32_t kk = 0;
+      constexpr bool kPair = sizeof(R) == 4;
+      const uint32_t n2 = (kPair && nm >= 2 * S) ? nm - 2 * S + 1 : 0;
+      if (kPair && valid) {
+        for (; kk < n2; kk += 2 * S) {
+          const uint32_t k = kk + s, k2 = k + S;
+          const V4 e0 = lm[k][0], e1 = lm[k][1], e2 = lm[k][2], e3 = lm[k][3];
+          const V4 f0 = lm[k2][0], f1 = lm[k2][1], f2 = lm[k2][2], f3 = lm[k2][3];
+          m2p<R>(tx - (R)e0.x, ty - (R)e0.y, tz - (R)e0.z, e1, e2, e3, (R)e0.w, eps2, acc.p);
+          m2p<R>(tx - (R)f0.x, ty - (R)f0.y, tz - (R)f0.z, f1, f2, f3, (R)f0.w, eps2, acc.p);
+        }
+      }
+      kk = n2 ? ((n2 - 1) / (2 * S) + 1) * 2 * S : 0;
+      for (; kk < nm; kk += S) {
+        const uint32_t k = kk + s;
+        if (valid && k < nm) {
+          const V4 e0 = lm[k][0], e1 = lm[k][1], e2 = lm[k][2], e3 = lm[k][3];
+          m2p<R>(tx - (R)e0.x, ty - (R)e0.y, tz - (R)e0.z, e1, e2, e3, (R)e0.w, eps2, acc.p);
+        }
       }
       fold(acc);
     };
     auto flush_p2p = [&]() {
-      for (uint32_t k = s; k < np; k += S) {
-        const V4 e = lp[k];
-        const bool self = lpi[k] == tid;
-        p2p<R>(tx - (R)e.x, ty - (R)e.y, tz - (R)e.z, (R)e.w, eps2, self, acc.p);
+      for (uint32_t kk = 0; kk < np; kk += S) {
+        const uint32_t k = kk + s;
+        if (valid && k < np) {
+          const V4 e = lp[k];
+          const bool self = lpi[k] == tid;
+          p2p<R>(tx - (R)e.x, ty - (R)e.y, tz - (R)e.z, (R)e.w, eps2, self, acc.p);
+        }
       }
       fold(acc);
     };
@@ -214,10 +247,22 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
       const uint32_t node = has ? stack[sp - 1 - lane] : 0u;
       __syncthreads();
       sp -= m;
+      ++nsteps;
 
+      // All of a node's data is requested at once, before any decision, so a
+      // traversal step waits for one memory round trip (leaf moment slots are
+      // never written and their contents are not used).
       NodeGeom gm = {0.0, 0.0, 0.0, 0.0};
       NodeLink lk = {0u, 0u, 0u, 0u};
-      if (has) { gm = a.geom[node]; lk = a.link[node]; }
+      V4 w0, w1, w2;
+      if (has) {
+        gm = a.geom[node];
+        lk = a.link[node];
+        const V4* src;
+        if constexpr (sizeof(R) == 4) src = reinterpret_cast<const V4*>(&a.wm32[node]);
+        else src = reinterpret_cast<const V4*>(&a.wm64[node]);
+        w0 = src[0]; w1 = src[1]; w2 = src[2];
+      }
       const bool leaf = has && lk.nchild == 0u;
       const bool twig = has && lk.nchild != 0u;
 
@@ -238,18 +283,31 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
         rej = !acc_ok && (theta2 * dmax2) * (1.0 + 1e-12) <= gm.b2;
         macc = acc_ok ? cnt : (rej ? 1u : 0u);
       }
-      unsigned long long amb = __ballot(twig && !acc_ok && !rej);
+      // Undecided nodes: exact per-target test.  Every slice holds all the
+      // targets, so slice s tests the s-th undecided node of the batch and S
+      // nodes are settled per pass.
+      const bool amb_me = twig && !acc_ok && !rej;
+      unsigned long long amb = __ballot(amb_me);
+      namb += (uint32_t)popc64(amb);
+      const uint32_t my_rank = (uint32_t)popc64(amb & lt);
+      uint32_t base = 0;
       while (amb) {
-        const int j = __ffsll((long long)amb) - 1;
-        amb &= amb - 1ull;
+        unsigned long long mk = amb;
+        for (uint32_t k = 0; k < s && mk; ++k) mk &= mk - 1ull;
+        const bool active = valid && mk != 0ull;
+        const int j = active ? __ffsll((long long)mk) - 1 : lane;
         const double cx = __shfl(gm.cx, j), cy = __shfl(gm.cy, j), cz = __shfl(gm.cz, j);
         const double b2 = __shfl(gm.b2, j);
-        const bool fail = mac_fails(xt, yt, zt, cx, cy, cz, b2, theta2);
-        const unsigned long long fm = __ballot(fail && (uint32_t)lane < cnt);
-        if (lane == j) {
+        const bool fail = active && mac_fails(xt, yt, zt, cx, cy, cz, b2, theta2);
+        const unsigned long long ball = __ballot(fail && valid);
+        if (amb_me && my_rank >= base && my_rank < base + S) {
+          const uint32_t r = my_rank - base;
+          const unsigned long long fm = (ball >> (r * T)) & (T == 64 ? ~0ull : ((1ull << T) - 1ull));
           if (fm == 0ull) { acc_ok = true; macc = cnt; }
           else { rej = true; macc = (uint32_t)__ffsll((long long)fm); }
         }
+        for (uint32_t k = 0; k < S && amb; ++k) amb &= amb - 1ull;
+        base += S;
       }
       nmac += (unsigned long long)wave_sum(twig ? macc : 0u);
 
@@ -258,16 +316,11 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
       const unsigned long long pm = __ballot(leaf);
       if (take_m) {
         const uint32_t pos = nm + (uint32_t)popc64(am & lt);
-        const V4* src;
         V4 e0;
-        if constexpr (sizeof(R) == 4) {
-          src = reinterpret_cast<const V4*>(&a.wm32[node]);
+        if constexpr (sizeof(R) == 4)
           e0 = make_float4((float)(gm.cx - ox), (float)(gm.cy - oy), (float)(gm.cz - oz), 0.0f);
-        } else {
-          src = reinterpret_cast<const V4*>(&a.wm64[node]);
+        else
           e0 = make_double4(gm.cx, gm.cy, gm.cz, 0.0);
-        }
-        const V4 w0 = src[0], w1 = src[1], w2 = src[2];
         // w0 = {q, px, py, pz}, w1 = {2qx, 2qy, 2qz, xy}, w2 = {yz, zx, tr, -}
         e0.w = w0.x;
         V4 e1, e2, e3;
@@ -281,9 +334,9 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
         V4 e;
         if constexpr (sizeof(R) == 4) {
           e = make_float4((float)(gm.cx - ox), (float)(gm.cy - oy), (float)(gm.cz - oz),
-                          a.wm32[node].v[0]);
+                          (float)gm.b2);
         } else {
-          e = make_double4(gm.cx, gm.cy, gm.cz, a.wm64[node].v[0]);
+          e = make_double4(gm.cx, gm.cy, gm.cz, gm.b2);
         }
         lp[pos] = e;
         lpi[pos] = lk.first;
@@ -307,8 +360,9 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
     double out[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      double v = (double)acc.s[k] - (double)acc.c[k];
-      for (uint32_t o = T; o < (uint32_t)kWave; o <<= 1) v += __shfl_xor(v, (int)o);
+      const double v0 = (double)acc.s[k] - (double)acc.c[k];
+      double v = v0;
+      for (uint32_t o = 1; o < S; ++o) v += __shfl(v0, (int)(t + o * T));
       out[k] = v * a.sign;
     }
     if (overflow) {
@@ -323,14 +377,61 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
       a.np2p[tid] = cnt_p2p;
       a.nm2p[tid] = cnt_m2p;
     }
-    if (lane == 0) {
-      a.gmac[g] = (uint32_t)nmac;
-      atomicAdd(&a.sc->interactions, (unsigned long long)cnt * (cnt_p2p + cnt_m2p));
-      atomicAdd(&a.sc->mac_evals, nmac);
-      atomicAdd(&a.sc->p2p, (unsigned long long)cnt * cnt_p2p);
-      atomicAdd(&a.sc->m2p, (unsigned long long)cnt * cnt_m2p);
+    if (lane == 0) {   // per-group record, summed by walk_stats_kernel
+      uint4* gs = reinterpret_cast<uint4*>(a.gstat + 8 * (size_t)g);
+      gs[0] = make_uint4((uint32_t)nmac, cnt_p2p, cnt_m2p, namb);
+      gs[1] = make_uint4(nsteps, cnt, 0u, 0u);
     }
     __syncthreads();
+  }
+}
+
+// Sums the per-group records into the device scalars (walk statistics of
+// tree_walk_statistics, module_tree_walk_simple.f90:63-115).
+__global__ __launch_bounds__(256) void walk_stats_kernel(const uint32_t* __restrict__ gstat,
+                                                        DevScalars* __restrict__ sc) {
+  const int64_t ng = sc->ngroups;
+  unsigned long long v[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ng;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const uint4 r0 = reinterpret_cast<const uint4*>(gstat + 8 * g)[0];
+    const uint4 r1 = reinterpret_cast<const uint4*>(gstat + 8 * g)[1];
+    const unsigned long long c = r1.y;
+    v[0] += c * (r0.y + r0.z);
+    v[1] += r0.x;
+    v[2] += c * r0.y;
+    v[3] += c * r0.z;
+    v[4] += r0.w;
+    v[5] += r1.x;
+  }
+  __shared__ unsigned long long red[6][4];
+  const int w = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const unsigned long long s = wave_sum(v[k]);
+    if (lane_id() == 0) red[k][w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    const unsigned long long s = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+    if (s) {
+      unsigned long long* dst[6] = {&sc->interactions, &sc->mac_evals, &sc->p2p, &sc->m2p,
+                                    &sc->amb_nodes, &sc->walk_steps};
+      atomicAdd(dst[k], s);
+    }
+  }
+}
+
+__global__ void walk_reset_kernel(DevScalars* sc, uint32_t* queue) {
+  if (threadIdx.x == 0) {
+    *queue = 0;
+    sc->interactions = 0;
+    sc->mac_evals = 0;
+    sc->p2p = 0;
+    sc->m2p = 0;
+    sc->amb_nodes = 0;
+    sc->walk_steps = 0;
   }
 }
 
@@ -398,11 +499,19 @@ int walk_default_blocks() {
   return cus * per;
 }
 
+void launch_walk_reset(hipStream_t s, const WalkArgs& a) {
+  walk_reset_kernel<<<1, 64, 0, s>>>(a.sc, a.queue);
+}
+
 void launch_walk(hipStream_t s, const WalkArgs& a, int precision, int blocks) {
   if (precision == kWalkFP64)
     walk_kernel<double><<<blocks, kWave, 0, s>>>(a);
   else
     walk_kernel<float><<<blocks, kWave, 0, s>>>(a);
+}
+
+void launch_walk_stats(hipStream_t s, const WalkArgs& a) {
+  walk_stats_kernel<<<256, 256, 0, s>>>(a.gstat, a.sc);
 }
 
 void launch_direct(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
