@@ -39,12 +39,12 @@ def walk_flops(st):
     return FLOPS_P2P * st["p2p"] + FLOPS_M2P * st["m2p"] + FLOPS_MAC * st["mac_evaluations"]
 
 
-def run_gpu(sys_, precision, group, steps, warmup, dev_inputs):
+def run_gpu(sys_, precision, group, steps, warmup, dev_inputs, pack=0):
     import torch
 
     from pepc_amd.solver import Solver
     sol = Solver(theta2=sys_.theta2, eps2=sys_.eps2, group_size=group, precision=precision,
-                 sign=sys_.sign)
+                 sign=sys_.sign, group_pack=pack)
     x, y, z, q = dev_inputs
     out = None
     for _ in range(warmup):
@@ -84,13 +84,13 @@ def force_error(sol, out, sys_, nsamp):
     return float(np.median(rel)), float(np.max(rel)), float(np.median(prel))
 
 
-def cpu_baseline(sys_, group, stride, nthreads):
+def cpu_baseline(sys_, group, stride, nthreads, pack=0):
     from oracle import oracle as orc
     t0 = time.perf_counter()
     d = orc.domain_decompose(sys_.x, sys_.y, sys_.z, sys_.q)
     t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
     t1 = time.perf_counter()
-    tiles = orc.walk_tiles(d.keys, group)
+    tiles = orc.walk_tiles(d.keys, group, pack)
     t2 = time.perf_counter()
     w = orc.walk_group(d.x, d.y, d.z, t, sys_.theta2, sys_.eps2, tile_size=group, sign=sys_.sign,
                        tile_stride=stride, nthreads=nthreads, tiles=tiles)
@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--n", type=int, default=4194304)
     ap.add_argument("--theta", type=float, default=0.6)
     ap.add_argument("--group", type=int, default=64)
+    ap.add_argument("--pack", type=int, default=0,
+                    help="merge runs of sibling walk tiles up to this many targets (0 = off)")
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--err-samples", type=int, default=65536)
     ap.add_argument("--no-fp64", action="store_true")
@@ -144,7 +146,7 @@ def main():
     log(f"workload explosion n={s.n} theta={args.theta} G={args.group} {args.precision}")
 
     sol, out, st, ms, t_grow_ms, t_walk_ms = run_gpu(s, args.precision, args.group, args.steps,
-                                                     args.warmup, dev_inputs)
+                                                     args.warmup, dev_inputs, args.pack)
     log(f"{args.precision}: {ms:.3f} ms/step (grow {t_grow_ms:.3f} ms, walk {t_walk_ms:.3f} ms) "
         f"stats {json.dumps({k: v for k, v in st.items() if k != 'level_offset'})}")
     med, mx, pmed = force_error(sol, out, s, args.err_samples)
@@ -163,7 +165,7 @@ def main():
     fp64 = None
     if not args.no_fp64 and args.precision != "fp64":
         sol64, out64, st64, ms64, g64, w64 = run_gpu(s, "fp64", args.group, max(2, args.steps // 2),
-                                                      1, dev_inputs)
+                                                      1, dev_inputs, args.pack)
         med64, mx64, _ = force_error(sol64, out64, s, min(args.err_samples, 16384))
         fp64 = {"ms_per_step": ms64, "t_walk_ms": w64, "t_grow_ms": g64,
                 "median_rel_force_err": med64, "speedup_fp32_over_fp64": ms64 / ms}
@@ -174,7 +176,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         nthr = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cpu = cpu_baseline(s, args.group, args.cpu_stride, nthr)
+        cpu = cpu_baseline(s, args.group, args.cpu_stride, nthr, args.pack)
         log(f"cpu baseline: {cpu['value']:.3e} updates/s ({cpu['sample']})")
 
     rec = {
@@ -192,7 +194,8 @@ def main():
         "data": "synthetic (seeded Coulomb explosion, uniform unit ball)",
         "config": {"workload": f"coulomb_explosion_n{s.n}_theta{args.theta}", "n": s.n,
                    "theta": args.theta, "eps": 1e-3, "expansion": "quadrupole",
-                   "group_size": args.group, "precision": args.precision,
+                   "group_size": args.group, "group_pack": args.pack,
+                   "precision": args.precision,
                    "t_grow_ms": t_grow_ms, "t_walk_ms": t_walk_ms,
                    "interactions_per_s": st["interactions"] / (t_walk_ms * 1e-3),
                    "median_rel_force_err": med, "max_rel_force_err": mx,

@@ -61,6 +61,10 @@ typedef struct {
     int32_t force_cubic_domain;  /* module_box.f90:29 */
     int32_t walk_blocks;    /* persistent walk grid; 0 = fill the device */
     int32_t device;         /* HIP device ordinal */
+    int32_t group_pack;     /* 0 = reference tiles; 1..64 = merge runs of sibling tiles
+                               up to this many particles per group (no reference
+                               counterpart; fuller wavefronts) */
+    int32_t reserved;
 } pepc_gpu_config;
 
 /* Counters of the last grow/traverse (tree_stats, module_tree.f90:622-699;

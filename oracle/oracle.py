@@ -68,7 +68,7 @@ def lib():
                                     _u64, _i32, _i64, _i64, _i64, _i64, _i32, _f64, _i64, _i64]
         L.or_tree_build.restype = C.c_int64
         L.or_boxlength2.argtypes = [C.POINTER(Box), _f64]
-        L.or_walk_tiles.argtypes = [C.c_int64, _u64, C.c_int64, _i64, _i64]
+        L.or_walk_tiles.argtypes = [C.c_int64, _u64, C.c_int64, C.c_int64, _i64, _i64]
         L.or_walk_tiles.restype = C.c_int64
         L.or_walk_group.argtypes = [C.c_int64, _f64, _f64, _f64, C.c_int64, _i32, _i64, _i64, _i64,
                                     _f64, _f64, C.c_double, C.c_double, C.c_double,
@@ -212,11 +212,13 @@ def tree_build(keys, x, y, z, q, b: Box, cap=None) -> Tree:
                 node_leaf=node_leaf, b2=boxlength2(b), **out)
 
 
-def walk_tiles(keys, tile_size):
+def walk_tiles(keys, tile_size, pack=0):
+    """Walk tiles: maximal key cells with <= tile_size particles (reference
+    rule); pack > 0 merges runs of sibling tiles up to pack particles."""
     n = len(keys)
     tf = np.zeros(n, np.int64)
     tc = np.zeros(n, np.int64)
-    nt = lib().or_walk_tiles(n, _c(keys, np.uint64), int(tile_size), tf, tc)
+    nt = lib().or_walk_tiles(n, _c(keys, np.uint64), int(tile_size), int(pack), tf, tc)
     if nt < 0:
         raise ValueError("insufficient key resolution in tile formation")
     return tf[:nt].copy(), tc[:nt].copy()
@@ -238,10 +240,10 @@ class WalkResult:
 
 
 def walk_group(x, y, z, tree: Tree, theta2, eps2, tile_size=16, sign=1.0,
-               tile_stride=1, nthreads=0, keys=None, tiles=None) -> WalkResult:
+               tile_stride=1, nthreads=0, keys=None, tiles=None, pack=0) -> WalkResult:
     n = len(x)
     if tiles is None:
-        tiles = walk_tiles(keys, tile_size)
+        tiles = walk_tiles(keys, tile_size, pack)
     tf, tc = tiles
     ex, ey, ez, pot, work = (np.zeros(n) for _ in range(5))
     p2p = np.zeros(n, np.int32)
@@ -284,7 +286,8 @@ def direct(x, y, z, q, testidx, eps2, sign=1.0, nthreads=0):
     return np.stack([ex, ey, ez], 1), pot
 
 
-def solve(x, y, z, q, theta2, eps2, tile_size=16, sign=1.0, nthreads=0, force_cubic=False):
+def solve(x, y, z, q, theta2, eps2, tile_size=16, sign=1.0, nthreads=0, force_cubic=False,
+          pack=0):
     """Full single-rank grow + traverse as the reference frontends call it.
 
     Returns (decomposition, tree, walk result); results are in sorted order.
@@ -292,7 +295,7 @@ def solve(x, y, z, q, theta2, eps2, tile_size=16, sign=1.0, nthreads=0, force_cu
     d = domain_decompose(x, y, z, q, force_cubic)
     t = tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
     w = walk_group(d.x, d.y, d.z, t, theta2, eps2, tile_size=tile_size, sign=sign,
-                   nthreads=nthreads, keys=d.keys)
+                   nthreads=nthreads, keys=d.keys, pack=pack)
     return d, t, w
 
 

@@ -490,29 +490,46 @@ void or_m2p(const double *t, const double d[3], double dist2, double exyz[3], do
 /* Walk tiles: module_tree_walk_simple.f90:135-212 (maximal key cells holding */
 /* at most TILE_SIZE particles; TILE_SIZE=16 at :40 is the reference rule).   */
 /* ------------------------------------------------------------------------- */
+/* pack > 0 (this project's packed rule, no reference counterpart): runs of
+ * consecutive sibling tiles of one parent are merged while the merged count
+ * stays <= pack; a sibling that is split further ends the run.  pack = 0 is
+ * the reference rule. */
 static void tiles_aux(const uint64_t *keys, int64_t lo, int64_t hi, uint64_t k, int l,
-                      int64_t G, int64_t *tf, int64_t *tc, int64_t *nt, int *err)
+                      int64_t G, int64_t pack, int64_t *tf, int64_t *tc, int64_t *nt, int *err)
 {
     int64_t np = hi - lo;
     if (np == 0) return;
     if (np <= G) { tf[*nt] = lo; tc[*nt] = np; (*nt)++; return; }
     if (l >= OR_MAXLEVEL) { *err = 1; return; }
     int sh = 3 * (OR_MAXLEVEL - (l + 1));
-    int64_t ps = lo;
+    int64_t ps = lo, run_first = -1, run_cnt = 0;
     for (int ic = 0; ic < 8; ++ic) {
         uint64_t ck = (k << 3) + (uint64_t)ic;
         int64_t pe = ps;
         while (pe < hi && (keys[pe] >> sh) == ck) ++pe;
-        tiles_aux(keys, ps, pe, ck, l + 1, G, tf, tc, nt, err);
+        int64_t cc = pe - ps;
+        if (cc > 0 && cc <= G) {
+            if (pack > 0 && run_first >= 0 && run_cnt + cc <= pack) {
+                run_cnt += cc;
+            } else {
+                if (run_first >= 0) { tf[*nt] = run_first; tc[*nt] = run_cnt; (*nt)++; }
+                run_first = ps; run_cnt = cc;
+            }
+        } else if (cc > G) {
+            if (run_first >= 0) { tf[*nt] = run_first; tc[*nt] = run_cnt; (*nt)++; }
+            run_first = -1; run_cnt = 0;
+            tiles_aux(keys, ps, pe, ck, l + 1, G, pack, tf, tc, nt, err);
+        }
         ps = pe;
     }
+    if (run_first >= 0) { tf[*nt] = run_first; tc[*nt] = run_cnt; (*nt)++; }
 }
 
-int64_t or_walk_tiles(int64_t n, const uint64_t *keys, int64_t tile_size,
+int64_t or_walk_tiles(int64_t n, const uint64_t *keys, int64_t tile_size, int64_t pack,
                       int64_t *tile_first, int64_t *tile_count)
 {
     int64_t nt = 0; int err = 0;
-    tiles_aux(keys, 0, n, 1, 0, tile_size, tile_first, tile_count, &nt, &err);
+    tiles_aux(keys, 0, n, 1, 0, tile_size, pack, tile_first, tile_count, &nt, &err);
     return err ? -1 : nt;
 }
 

@@ -87,7 +87,7 @@ int64_t or_tree_build(int64_t n, const uint64_t *keys, const double *x, const do
 void or_boxlength2(const or_box *b, double b2[OR_MAXLEVEL + 1]);
 
 /* ---- tiles and walks ------------------------------------------------------- */
-int64_t or_walk_tiles(int64_t n, const uint64_t *keys, int64_t tile_size,
+int64_t or_walk_tiles(int64_t n, const uint64_t *keys, int64_t tile_size, int64_t pack,
                       int64_t *tile_first, int64_t *tile_count);
 int or_walk_group(int64_t n, const double *x, const double *y, const double *z,
                   int64_t nnodes, const int32_t *nlevel, const int64_t *nleaves,

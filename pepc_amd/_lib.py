@@ -32,7 +32,7 @@ class Config(C.Structure):
                 ("node_factor", C.c_double), ("mac_select", C.c_int32),
                 ("group_size", C.c_int32), ("precision", C.c_int32),
                 ("force_cubic_domain", C.c_int32), ("walk_blocks", C.c_int32),
-                ("device", C.c_int32)]
+                ("device", C.c_int32), ("group_pack", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Stats(C.Structure):

@@ -67,7 +67,7 @@ void launch_dup_detect(hipStream_t s, int64_t n, const unsigned long long* keys,
 void launch_dupfix(hipStream_t s, int64_t n, unsigned long long* keys, double* x, double* y,
                    double* z, const DevBox* box, long long* kd, unsigned long long* oldkeys,
                    uint32_t* dflag, uint32_t* dlist, uint32_t* bsum, DevScalars* sc);
-void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size);
+void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size, int group_pack);
 void launch_leaf_moments(hipStream_t s, const TreeBuffers& t);
 
 // walk.hip

@@ -195,7 +195,7 @@ int grow_device_impl(Ctx* c, int64_t n, const double* x, const double* y, const 
   t.hist = c->hist; t.bsum = c->bsum; t.box = c->box; t.sc = c->sc;
   t.prec = c->cfg.precision == PEPC_GPU_FP64 ? kWalkFP64 : kWalkFP32Kahan;
   c->tb = t;
-  launch_tree_build(s, t, c->cfg.group_size);
+  launch_tree_build(s, t, c->cfg.group_size, c->cfg.group_pack);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipEventRecord(c->ev[1], s));
   c->n = n;
@@ -278,6 +278,7 @@ int pepc_gpu_create(void** out, const pepc_gpu_config* cfg) {
   if (!out || !cfg) return PEPC_GPU_EINVAL;
   *out = nullptr;
   if (cfg->mac_select != 0 || cfg->group_size < 1 || cfg->group_size > 64 ||
+      cfg->group_pack < 0 || cfg->group_pack > 64 ||
       !(cfg->theta2 >= 0.0) || !(cfg->eps2 >= 0.0) || !(cfg->node_factor >= 1.0) ||
       (cfg->precision != PEPC_GPU_FP32_KAHAN && cfg->precision != PEPC_GPU_FP64) ||
       !(cfg->sign == 1.0 || cfg->sign == -1.0))

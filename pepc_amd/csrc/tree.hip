@@ -418,22 +418,40 @@ __global__ void gmark_clear_kernel(int64_t n, uint32_t* __restrict__ gmark) {
     gmark[a] = 0;
 }
 
-__global__ void group_mark_kernel(int group_size, const NodeLink* __restrict__ link,
+// A twig holding more than G particles marks each child with at most G as a
+// group start (the child's first particle gets the group size).  With
+// pack > 0, runs of consecutive such children are merged while the merged
+// size stays <= pack; a child that is split further ends a run.
+__global__ void group_mark_kernel(int group_size, int pack, const NodeLink* __restrict__ link,
                                   const uint32_t* __restrict__ nfirst, uint32_t* __restrict__ gmark,
                                   const DevScalars* __restrict__ sc) {
   if (tree_broken(sc)) return;
   const int64_t m = sc->nnodes;
-  const uint32_t G = (uint32_t)group_size;
+  const uint32_t G = (uint32_t)group_size, P = (uint32_t)pack;
   if (blockIdx.x == 0 && threadIdx.x == 0 && m > 0 && link[0].count <= G) gmark[0] = link[0].count;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
        i += (int64_t)gridDim.x * blockDim.x) {
     const NodeLink lk = link[i];
     if (lk.nchild == 0 || lk.count <= G) continue;
+    uint32_t run_first = 0xFFFFFFFFu, run_cnt = 0;
     for (uint32_t j = 0; j < lk.nchild; ++j) {
       const uint32_t c = lk.first + j;
       const uint32_t cc = link[c].count;
-      if (cc <= G) gmark[nfirst[c]] = cc;
+      if (cc <= G) {
+        if (P > 0 && run_first != 0xFFFFFFFFu && run_cnt + cc <= P) {
+          run_cnt += cc;
+        } else {
+          if (run_first != 0xFFFFFFFFu) gmark[run_first] = run_cnt;
+          run_first = nfirst[c];
+          run_cnt = cc;
+        }
+      } else {
+        if (run_first != 0xFFFFFFFFu) gmark[run_first] = run_cnt;
+        run_first = 0xFFFFFFFFu;
+        run_cnt = 0;
+      }
     }
+    if (run_first != 0xFFFFFFFFu) gmark[run_first] = run_cnt;
   }
 }
 
@@ -480,7 +498,7 @@ void launch_dupfix(hipStream_t s, int64_t n, unsigned long long* keys, double* x
   dup_recoord_kernel<<<grid_for(n), kT, 0, s>>>(n, keys, oldkeys, x, y, z, box, sc);
 }
 
-void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size) {
+void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size, int group_pack) {
   const int64_t n = t.n, cap = t.cap;
   levels_kernel<<<grid_for(n), kT, 0, s>>>(n, t.keys, t.lo, t.hi, t.cnt, t.sc);
   launch_scan(s, t.cnt, t.cnt, n, t.bsum, &t.sc->nnodes);
@@ -500,7 +518,8 @@ void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size) {
     upward_kernel<<<grid_for(cap / 4), kT, 0, s>>>(l, t.prec, t.link, t.x, t.y, t.z, t.q, t.mom64,
                                                    t.geom, t.wm32, t.wm64, t.box, t.sc);
   gmark_clear_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark);
-  group_mark_kernel<<<grid_for(cap), kT, 0, s>>>(group_size, t.link, t.nfirst, t.gmark, t.sc);
+  group_mark_kernel<<<grid_for(cap), kT, 0, s>>>(group_size, group_pack, t.link, t.nfirst, t.gmark,
+                                                 t.sc);
   group_flag_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark, t.cnt);
   launch_scan(s, t.cnt, t.cnt, n, t.bsum, &t.sc->ngroups);
   group_emit_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark, t.cnt, t.gfirst, t.gcount);

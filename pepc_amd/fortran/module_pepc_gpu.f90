@@ -1,0 +1,401 @@
+! module_pepc_gpu.f90 -- Fortran host shim of the MI355X tree solve.
+!
+! Keeps PEPC's frontend-facing Fortran API (module_pepc, module_pepc.f90:32-56)
+! and its Coulomb XYZQ particle type (module_pepc_types.f90:50-58,
+! coulomb/module_interaction_specific_types_XYZQ.f90:30-42) so a pepc-mini or
+! pepc-essential style frontend compiles against it unchanged; the treecode
+! underneath is the HIP library behind include/pepc_gpu.h, bound here with
+! iso_c_binding.  The shim copies the array-of-structures particles into
+! structure-of-arrays buffers, calls the C-ABI, and reproduces the
+! reference's side effects: grow reorders the particle array by key and
+! resets %work to 1 (module_domains.f90:204-234, :318), traverse adds the
+! fields and potentials to %results and counts interactions in %work.
+! A nonzero status aborts with the library's message, as DEBUG_ERROR does
+! (pepc_debug.h:28-35).
+
+module module_pepc_kinds
+   use iso_c_binding, only: c_double, c_int64_t, c_int32_t
+   implicit none
+   ! module_pepc_kinds.f90: 8-byte reals and integers for keys and counts
+   integer, parameter :: kind_physics = c_double
+   integer, parameter :: kind_particle = c_int64_t
+   integer, parameter :: kind_key = c_int64_t
+   integer, parameter :: kind_node = c_int64_t
+   integer, parameter :: kind_default = c_int32_t
+   integer, parameter :: kind_dim = 4
+end module module_pepc_kinds
+
+module module_interaction_specific_types
+   use module_pepc_kinds
+   implicit none
+   type t_particle_data
+      real(kind_physics) :: q            !< electric charge
+   end type t_particle_data
+
+   type t_particle_results
+      real(kind_physics), dimension(3) :: e
+      real(kind_physics) :: pot
+   end type t_particle_results
+
+   type(t_particle_results), parameter :: EMPTY_PARTICLE_RESULTS = &
+      t_particle_results([0._kind_physics, 0._kind_physics, 0._kind_physics], 0._kind_physics)
+end module module_interaction_specific_types
+
+module module_pepc_types
+   use module_pepc_kinds
+   use module_interaction_specific_types
+   implicit none
+   type t_particle
+      real(kind_physics) :: x(1:3)        !< coordinates
+      real*8 :: work                      !< work load (1 + interactions after traverse)
+      integer(kind_key) :: key            !< space-filling key on the finest level
+      integer(kind_node) :: node_leaf     !< not tracked by the GPU tree (0)
+      integer(kind_particle) :: label     !< frontend label
+      type(t_particle_data) :: data
+      type(t_particle_results) :: results
+   end type t_particle
+end module module_pepc_types
+
+!> iso_c_binding interface of include/pepc_gpu.h
+module pepc_gpu_c
+   use iso_c_binding
+   implicit none
+
+   type, bind(C) :: pepc_gpu_config
+      real(c_double) :: theta2, eps2, sign, node_factor
+      integer(c_int32_t) :: mac_select, group_size, precision, force_cubic_domain
+      integer(c_int32_t) :: walk_blocks, device, group_pack, reserved
+   end type pepc_gpu_config
+
+   type, bind(C) :: pepc_gpu_stats
+      integer(c_int64_t) :: n, nnodes, nleaf, ntwig, ngroups, dup_fixed
+      integer(c_int64_t) :: interactions, mac_evaluations, p2p, m2p, amb_nodes, walk_steps
+      integer(c_int32_t) :: flags, maxlevel
+      integer(c_int32_t) :: level_offset(22)
+   end type pepc_gpu_stats
+
+   interface
+      subroutine pepc_gpu_config_default(cfg) bind(C, name="pepc_gpu_config_default")
+         import :: pepc_gpu_config
+         type(pepc_gpu_config), intent(out) :: cfg
+      end subroutine
+
+      integer(c_int) function pepc_gpu_create(ctx, cfg) bind(C, name="pepc_gpu_create")
+         import :: c_int, c_ptr, pepc_gpu_config
+         type(c_ptr), intent(out) :: ctx
+         type(pepc_gpu_config), intent(in) :: cfg
+      end function
+
+      integer(c_int) function pepc_gpu_destroy(ctx) bind(C, name="pepc_gpu_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
+      integer(c_int) function pepc_gpu_last_error(ctx, buf, len) bind(C, name="pepc_gpu_last_error")
+         import :: c_int, c_ptr, c_char, c_size_t
+         type(c_ptr), value :: ctx
+         character(kind=c_char), intent(out) :: buf(*)
+         integer(c_size_t), value :: len
+      end function
+
+      integer(c_int) function pepc_gpu_grow(ctx, n, x, y, z, q, keys, perm, xs, ys, zs) &
+         bind(C, name="pepc_gpu_grow")
+         import :: c_int, c_ptr, c_int64_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: n
+         real(c_double), intent(in) :: x(*), y(*), z(*), q(*)
+         integer(c_int64_t), intent(out) :: keys(*), perm(*)
+         real(c_double), intent(out) :: xs(*), ys(*), zs(*)
+      end function
+
+      integer(c_int) function pepc_gpu_traverse(ctx, ex, ey, ez, pot, work) &
+         bind(C, name="pepc_gpu_traverse")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: ex(*), ey(*), ez(*), pot(*), work(*)
+      end function
+
+      integer(c_int) function pepc_gpu_sync(ctx, st) bind(C, name="pepc_gpu_sync")
+         import :: c_int, c_ptr, pepc_gpu_stats
+         type(c_ptr), value :: ctx
+         type(pepc_gpu_stats), intent(out) :: st
+      end function
+
+      integer(c_int) function pepc_gpu_timber(ctx) bind(C, name="pepc_gpu_timber")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
+      integer(c_int) function pepc_gpu_timings(ctx, tg, tw) bind(C, name="pepc_gpu_timings")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: tg, tw
+      end function
+   end interface
+end module pepc_gpu_c
+
+!> The frontend API of module_pepc.f90:32-56 on top of the GPU library.
+module module_pepc
+   use iso_c_binding
+   use module_pepc_kinds
+   use module_pepc_types
+   use pepc_gpu_c
+   implicit none
+   private
+
+   public :: pepc_initialize, pepc_finalize, pepc_prepare
+   public :: pepc_read_parameters, pepc_read_parameters_from_file_name
+   public :: pepc_read_parameters_from_first_argument
+   public :: pepc_particleresults_clear, pepc_grow_tree, pepc_traverse_tree
+   public :: pepc_timber_tree, pepc_restore_particles, pepc_grow_and_traverse
+   public :: pepc_statistics, pepc_last_stats, pepc_last_timings
+
+   ! &calc_force_coulomb (coulomb/module_interaction_specific.f90:35-43)
+   integer, public :: force_law = 3
+   integer, public :: mac_select = 0
+   real(kind_physics), public :: theta2 = 0.36_kind_physics
+   real(kind_physics), public :: eps2 = 0.0_kind_physics
+   ! &libpepc entries that apply (module_libpepc_main.f90:44, module_box.f90:29)
+   logical, public :: force_cubic_domain = .false.
+   real, public :: np_mult = 2.5
+   ! &walk_para_gpu: the GPU walk's own parameters
+   integer, public :: group_size = 16         ! walk tile size (module_tree_walk_simple TILE_SIZE)
+   integer, public :: group_pack = 0
+   character(len=8), public :: precision = 'fp32'
+   real(kind_physics), public :: interaction_sign = 1.0_kind_physics
+   logical, public :: print_stats = .false.   ! the reference's DBG_STATS
+
+   namelist /calc_force_coulomb/ force_law, mac_select, theta2, eps2
+   namelist /libpepc/ force_cubic_domain, np_mult, print_stats
+   namelist /walk_para_gpu/ group_size, group_pack, precision, interaction_sign
+
+   type(c_ptr) :: ctx = c_null_ptr
+   integer(c_int64_t), allocatable :: last_perm(:)
+   integer :: my_rank_ = 0
+
+contains
+
+   subroutine check(rc, where)
+      integer(c_int), intent(in) :: rc
+      character(*), intent(in) :: where
+      character(kind=c_char) :: buf(512)
+      character(len=512) :: msg
+      integer :: i, ierr
+      if (rc == 0) return
+      ierr = pepc_gpu_last_error(ctx, buf, int(512, c_size_t))
+      msg = ' '
+      do i = 1, 512
+         if (buf(i) == c_null_char) exit
+         msg(i:i) = buf(i)
+      end do
+      write (*, '(a,a,a,i0,a,a)') 'PEPC-GPU error in ', where, ' (status ', rc, '): ', trim(msg)
+      error stop 1
+   end subroutine check
+
+   !> module_pepc.f90:70-175 (no MPI on one GPU: rank 0 of 1)
+   subroutine pepc_initialize(frontendname, my_rank, n_cpu, init_mpi, db_level_in, comm)
+      character(*), intent(in) :: frontendname
+      integer, intent(out) :: my_rank, n_cpu
+      logical, intent(in) :: init_mpi
+      integer, intent(in), optional :: db_level_in
+      integer, intent(inout), optional :: comm
+      my_rank = 0
+      n_cpu = 1
+      my_rank_ = 0
+      write (*, '(a,a)') ' PEPC-GPU frontend: ', trim(frontendname)
+      if (present(db_level_in) .or. present(comm) .or. init_mpi) continue
+   end subroutine pepc_initialize
+
+   subroutine pepc_read_parameters(filehandle)
+      integer, intent(in) :: filehandle
+      integer :: ios
+      rewind (filehandle)
+      read (filehandle, NML=libpepc, iostat=ios)
+      rewind (filehandle)
+      read (filehandle, NML=calc_force_coulomb, iostat=ios)
+      rewind (filehandle)
+      read (filehandle, NML=walk_para_gpu, iostat=ios)
+   end subroutine pepc_read_parameters
+
+   subroutine pepc_read_parameters_from_file_name(filename)
+      character(*), intent(in) :: filename
+      integer, parameter :: u = 47
+      open (u, file=filename, status='old', action='read')
+      call pepc_read_parameters(u)
+      close (u)
+   end subroutine pepc_read_parameters_from_file_name
+
+   subroutine pepc_read_parameters_from_first_argument(read_para_file, para_file)
+      logical, optional, intent(out) :: read_para_file
+      character(len=255), optional, intent(out) :: para_file
+      character(len=255) :: f
+      if (command_argument_count() >= 1) then
+         call get_command_argument(1, f)
+         call pepc_read_parameters_from_file_name(trim(f))
+         if (present(read_para_file)) read_para_file = .true.
+         if (present(para_file)) para_file = f
+      else
+         if (present(read_para_file)) read_para_file = .false.
+      end if
+   end subroutine pepc_read_parameters_from_first_argument
+
+   !> module_pepc.f90:269-288: creates the GPU context with the parameters read
+   subroutine pepc_prepare(idim)
+      integer(kind_dim), optional, intent(in) :: idim
+      type(pepc_gpu_config) :: cfg
+      if (present(idim)) then
+         if (idim /= 3) then
+            write (*, *) 'PEPC-GPU: only idim = 3 is supported'
+            error stop 1
+         end if
+      end if
+      if (force_law /= 3 .or. mac_select /= 0) then
+         write (*, *) 'PEPC-GPU: only force_law = 3 (3-D Coulomb) and mac_select = 0 (BH)'
+         error stop 1
+      end if
+      if (c_associated(ctx)) call check(pepc_gpu_destroy(ctx), 'pepc_prepare')
+      call pepc_gpu_config_default(cfg)
+      cfg%theta2 = theta2
+      cfg%eps2 = eps2
+      cfg%sign = interaction_sign
+      cfg%node_factor = max(1.0_c_double, real(abs(np_mult), c_double))
+      cfg%group_size = group_size
+      cfg%group_pack = group_pack
+      cfg%force_cubic_domain = merge(1, 0, force_cubic_domain)
+      if (trim(precision) == 'fp64') then
+         cfg%precision = 1
+      else
+         cfg%precision = 0
+      end if
+      call check(pepc_gpu_create(ctx, cfg), 'pepc_prepare')
+   end subroutine pepc_prepare
+
+   subroutine pepc_finalize(comm)
+      integer, intent(in), optional :: comm
+      if (c_associated(ctx)) call check(pepc_gpu_destroy(ctx), 'pepc_finalize')
+      ctx = c_null_ptr
+      if (allocated(last_perm)) deallocate (last_perm)
+      if (present(comm)) continue
+   end subroutine pepc_finalize
+
+   !> coulomb/module_interaction_specific.f90:282-288
+   subroutine pepc_particleresults_clear(particles)
+      type(t_particle), intent(inout) :: particles(:)
+      particles(:)%results = EMPTY_PARTICLE_RESULTS
+   end subroutine pepc_particleresults_clear
+
+   !> module_pepc.f90:435-446: builds the tree and reorders particles by key
+   subroutine pepc_grow_tree(particles)
+      type(t_particle), allocatable, intent(inout) :: particles(:)
+      real(c_double), allocatable :: x(:), y(:), z(:), q(:), xs(:), ys(:), zs(:)
+      integer(c_int64_t), allocatable :: keys(:)
+      type(t_particle), allocatable :: tmp(:)
+      integer(c_int64_t) :: n, i
+      if (.not. c_associated(ctx)) call pepc_prepare()
+      n = size(particles, kind=c_int64_t)
+      allocate (x(n), y(n), z(n), q(n), xs(n), ys(n), zs(n), keys(n))
+      if (allocated(last_perm)) deallocate (last_perm)
+      allocate (last_perm(n))
+      do i = 1, n
+         x(i) = particles(i)%x(1)
+         y(i) = particles(i)%x(2)
+         z(i) = particles(i)%x(3)
+         q(i) = particles(i)%data%q
+      end do
+      call check(pepc_gpu_grow(ctx, n, x, y, z, q, keys, last_perm, xs, ys, zs), 'pepc_grow_tree')
+      call move_alloc(particles, tmp)
+      allocate (particles(n))
+      do i = 1, n
+         particles(i) = tmp(last_perm(i) + 1)
+         particles(i)%key = keys(i)
+         particles(i)%x = [xs(i), ys(i), zs(i)]   ! duplicate-key fix-up may move a particle
+         particles(i)%work = 1.0_8                ! module_domains.f90:318
+         particles(i)%node_leaf = 0
+      end do
+      deallocate (tmp, x, y, z, q, xs, ys, zs, keys)
+   end subroutine pepc_grow_tree
+
+   !> module_pepc.f90:459-468 for the grown particles (in their grown order)
+   subroutine pepc_traverse_tree(particles)
+      type(t_particle), target, intent(inout) :: particles(:)
+      real(c_double), allocatable :: ex(:), ey(:), ez(:), pot(:), work(:)
+      integer(c_int64_t) :: n, i
+      n = size(particles, kind=c_int64_t)
+      if (.not. allocated(last_perm)) then
+         write (*, *) 'PEPC-GPU: pepc_traverse_tree before pepc_grow_tree'
+         error stop 1
+      end if
+      if (n /= size(last_perm, kind=c_int64_t)) then
+         write (*, *) 'PEPC-GPU: traverse supports the grown particle set only'
+         error stop 1
+      end if
+      allocate (ex(n), ey(n), ez(n), pot(n), work(n))
+      call check(pepc_gpu_traverse(ctx, ex, ey, ez, pot, work), 'pepc_traverse_tree')
+      do i = 1, n
+         particles(i)%results%e = particles(i)%results%e + [ex(i), ey(i), ez(i)]
+         particles(i)%results%pot = particles(i)%results%pot + pot(i)
+         particles(i)%work = work(i)
+      end do
+      deallocate (ex, ey, ez, pot, work)
+   end subroutine pepc_traverse_tree
+
+   !> module_pepc.f90:538
+   subroutine pepc_timber_tree()
+      if (c_associated(ctx)) call check(pepc_gpu_timber(ctx), 'pepc_timber_tree')
+   end subroutine pepc_timber_tree
+
+   !> module_pepc.f90:525 -> domain_restore, module_domains.f90:372-408
+   subroutine pepc_restore_particles(particles)
+      type(t_particle), allocatable, intent(inout) :: particles(:)
+      type(t_particle), allocatable :: tmp(:)
+      integer(c_int64_t) :: i
+      if (.not. allocated(last_perm)) return
+      call move_alloc(particles, tmp)
+      allocate (particles(size(tmp)))
+      do i = 1, size(tmp, kind=c_int64_t)
+         particles(last_perm(i) + 1) = tmp(i)
+      end do
+      deallocate (tmp)
+   end subroutine pepc_restore_particles
+
+   !> module_pepc.f90:374-429
+   subroutine pepc_grow_and_traverse(particles, itime, no_dealloc, no_restore)
+      type(t_particle), allocatable, intent(inout) :: particles(:)
+      integer, intent(in) :: itime
+      logical, optional, intent(in) :: no_dealloc, no_restore
+      logical :: restore, dealloc
+      restore = .true.
+      dealloc = .true.
+      if (present(no_dealloc)) dealloc = .not. no_dealloc
+      if (present(no_restore)) restore = .not. no_restore
+      call pepc_grow_tree(particles)
+      call pepc_traverse_tree(particles)
+      if (print_stats) call pepc_statistics(itime)
+      if (restore) call pepc_restore_particles(particles)
+      if (dealloc) call pepc_timber_tree()
+   end subroutine pepc_grow_and_traverse
+
+   function pepc_last_stats() result(st)
+      type(pepc_gpu_stats) :: st
+      call check(pepc_gpu_sync(ctx, st), 'pepc_last_stats')
+   end function pepc_last_stats
+
+   subroutine pepc_last_timings(t_grow_ms, t_walk_ms)
+      real(c_double), intent(out) :: t_grow_ms, t_walk_ms
+      call check(pepc_gpu_timings(ctx, t_grow_ms, t_walk_ms), 'pepc_last_timings')
+   end subroutine pepc_last_timings
+
+   !> module_pepc.f90:478-500: tree and walk statistics (tree_stats,
+   !> tree_walk_statistics) to stdout
+   subroutine pepc_statistics(itime)
+      integer, intent(in) :: itime
+      type(pepc_gpu_stats) :: st
+      st = pepc_last_stats()
+      write (*, '(a,i8)') ' ######## STATISTICS step ', itime
+      write (*, '(a,3i14)') ' nodes / leaves / twigs: ', st%nnodes, st%nleaf, st%ntwig
+      write (*, '(a,i14)') ' walk groups:             ', st%ngroups
+      write (*, '(a,2i16)') ' interactions / mac evals:', st%interactions, st%mac_evaluations
+      write (*, '(a,2i16)') ' p2p / m2p:               ', st%p2p, st%m2p
+   end subroutine pepc_statistics
+
+end module module_pepc

@@ -1,0 +1,141 @@
+! pepc_mini_gpu.f90 -- a pepc-mini style frontend on the GPU treecode.
+!
+! Follows the timestep loop of src/frontends/pepc-mini/pepc.f90:55-100:
+! clear results, grow, traverse, push (kick then drift,
+! pepc-mini/module_helper.f90:205-222), timber.  It is written only against
+! module_pepc, so the same source builds against the reference library.
+!
+! Usage:
+!   pepc_mini_gpu params                      ! timestep run, &pepcmini in params
+!   pepc_mini_gpu params in.bin out.bin       ! one grow+traverse of the particles
+!                                             ! in in.bin, results to out.bin
+! in.bin : int64 n, then float64 x(n), y(n), z(n), q(n)
+! out.bin: int64 n, int64 label(n) (0-based input index), float64 x,y,z(n),
+!          ex,ey,ez,pot,work(n), all in the grown (key) order.
+program pepc_mini_gpu
+   use module_pepc_kinds
+   use module_pepc_types
+   use module_pepc
+   implicit none
+
+   type(t_particle), allocatable :: particles(:)
+   integer :: my_rank, n_ranks, step, nt, ip
+   integer(kind_particle) :: tnp, i, n
+   real(kind_physics) :: dt, epot, ekin
+   real(kind_physics), allocatable :: v(:, :), a(:)
+   character(len=255) :: fin, fout
+   integer(8) :: t0, t1, rate
+   real(8) :: tg, tw
+   namelist /pepcmini/ tnp, nt, dt
+
+   call pepc_initialize("pepc-mini-gpu", my_rank, n_ranks, .true.)
+   call pepc_read_parameters_from_first_argument()
+   call pepc_prepare(3_kind_dim)
+
+   if (command_argument_count() >= 3) then
+      call get_command_argument(2, fin)
+      call get_command_argument(3, fout)
+      open (11, file=trim(fin), access='stream', form='unformatted', status='old')
+      read (11) n
+      allocate (particles(n), a(n))
+      read (11) a; particles(:)%x(1) = a
+      read (11) a; particles(:)%x(2) = a
+      read (11) a; particles(:)%x(3) = a
+      read (11) a; particles(:)%data%q = a
+      close (11)
+      do i = 1, n
+         particles(i)%label = i - 1
+         particles(i)%work = 1.0_8
+      end do
+      call pepc_particleresults_clear(particles)
+      call pepc_grow_tree(particles)
+      call pepc_traverse_tree(particles)
+      open (12, file=trim(fout), access='stream', form='unformatted', status='replace')
+      write (12) n
+      write (12) particles(:)%label
+      write (12) particles(:)%x(1)
+      write (12) particles(:)%x(2)
+      write (12) particles(:)%x(3)
+      write (12) particles(:)%results%e(1)
+      write (12) particles(:)%results%e(2)
+      write (12) particles(:)%results%e(3)
+      write (12) particles(:)%results%pot
+      write (12) particles(:)%work
+      close (12)
+      call pepc_statistics(0)
+      call pepc_timber_tree()
+      call pepc_finalize()
+      stop
+   end if
+
+   ! timestep run: uniform cube, alternating charges, unit masses, at rest
+   tnp = 32768
+   nt = 10
+   dt = 0.01_kind_physics
+   if (command_argument_count() >= 1) then
+      call get_command_argument(1, fin)
+      open (13, file=trim(fin), status='old', action='read')
+      read (13, NML=pepcmini, iostat=ip)
+      close (13)
+   end if
+   allocate (particles(tnp), v(3, tnp))
+   call init_particles(particles)
+   v = 0.0_kind_physics
+   call system_clock(count_rate=rate)
+   do step = 0, nt
+      call pepc_particleresults_clear(particles)
+      call system_clock(t0)
+      call pepc_grow_tree(particles)
+      call pepc_traverse_tree(particles)
+      call system_clock(t1)
+      call pepc_last_timings(tg, tw)
+      epot = 0.5_kind_physics*sum(particles(:)%data%q*particles(:)%results%pot)
+      ! velocities follow the particles through the reordering by label
+      call push(particles, v, dt)
+      ekin = 0.5_kind_physics*sum(v*v)
+      write (*, '(a,i5,a,es12.4,a,es12.4,a,f9.3,a,f9.3,a,f9.3)') ' step', step, '  Epot', epot, &
+         '  Ekin', ekin, '  grow+walk [ms]', 1e3*real(t1 - t0, 8)/real(rate, 8), '  dev grow', tg, &
+         '  dev walk', tw
+      call pepc_timber_tree()
+   end do
+   call pepc_finalize()
+
+contains
+
+   subroutine init_particles(p)
+      type(t_particle), intent(inout) :: p(:)
+      integer(kind_particle) :: j
+      integer(8) :: s
+      s = 88172645463325252_8
+      do j = 1, size(p, kind=kind_particle)
+         p(j)%label = j
+         p(j)%data%q = -1.0_8 + 2.0_8*mod(j, 2_kind_particle)   ! module_helper.f90:185
+         p(j)%x(1) = rnd(s)
+         p(j)%x(2) = rnd(s)
+         p(j)%x(3) = rnd(s)
+         p(j)%work = 1.0_8
+      end do
+   end subroutine init_particles
+
+   real(8) function rnd(s)
+      integer(8), intent(inout) :: s
+      ! xorshift64 mapped to [0,1)
+      s = ieor(s, ishft(s, 13))
+      s = ieor(s, ishft(s, -7))
+      s = ieor(s, ishft(s, 17))
+      rnd = real(ishft(s, -11), 8)*(1.0_8/9007199254740992.0_8)
+   end function rnd
+
+   subroutine push(p, vel, h)
+      type(t_particle), intent(inout) :: p(:)
+      real(kind_physics), intent(inout) :: vel(:, :)
+      real(kind_physics), intent(in) :: h
+      integer(kind_particle) :: j, l
+      do j = 1, size(p, kind=kind_particle)
+         l = p(j)%label
+         vel(:, l) = vel(:, l) + h*p(j)%data%q*p(j)%results%e
+         p(j)%x = p(j)%x + h*vel(:, l)
+      end do
+   end subroutine push
+
+end program pepc_mini_gpu

@@ -29,20 +29,21 @@ SYSTEMS = {
 _cache = {}
 
 
-def _oracle(orc, name, G):
-    key = (name, G)
+def _oracle(orc, name, G, pack=0):
+    key = (name, G, pack)
     if key not in _cache:
         s = SYSTEMS[name]()
         d = orc.domain_decompose(s.x, s.y, s.z, s.q)
         t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
         w = orc.walk_group(d.x, d.y, d.z, t, s.theta2, s.eps2, tile_size=G, sign=s.sign,
-                           keys=d.keys)
+                           keys=d.keys, pack=pack)
         _cache[key] = (s, d, t, w)
     return _cache[key]
 
 
-def _run(gpu, s, G, precision):
-    sol = gpu(theta2=s.theta2, eps2=s.eps2, group_size=G, precision=precision, sign=s.sign)
+def _run(gpu, s, G, precision, pack=0):
+    sol = gpu(theta2=s.theta2, eps2=s.eps2, group_size=G, precision=precision, sign=s.sign,
+              group_pack=pack)
     st = sol.grow(s.x, s.y, s.z, s.q)
     res = sol.traverse()
     return sol, st, res
@@ -73,11 +74,11 @@ def test_keys_permutation_tree_bitexact(gpu, orc, name):
     sol.close()
 
 
-@pytest.mark.parametrize("G", [1, 16, 64])
+@pytest.mark.parametrize("G,pack", [(1, 0), (16, 0), (64, 0), (16, 64), (64, 64)])
 @pytest.mark.parametrize("name", ["mini", "plummer64k"])
-def test_groups_and_counts_bitexact(gpu, orc, name, G):
-    s, d, t, w = _oracle(orc, name, G)
-    sol, st, res = _run(gpu, s, G, "fp64")
+def test_groups_and_counts_bitexact(gpu, orc, name, G, pack):
+    s, d, t, w = _oracle(orc, name, G, pack)
+    sol, st, res = _run(gpu, s, G, "fp64", pack)
     wk = sol.read_walk()
     print(f"[{name} G={G}] groups={st['ngroups']} interactions={wk['stats']['interactions']}",
           flush=True)

@@ -143,6 +143,11 @@ def test_tiles_partition_and_maximal(orc):
         assert np.all(tc <= G) and np.all(tc >= 1)
         assert tf[0] == 0 and np.all(tf[1:] == tf[:-1] + tc[:-1]) and tf[-1] + tc[-1] == 8000
         # each tile is a whole key cell; its parent cell holds more than G
+        # the packed rule merges runs of whole sibling tiles: every packed group
+        # is a union of consecutive reference tiles and holds at most 64
+        pf, pc = orc.walk_tiles(d.keys, G, pack=64)
+        assert np.all(pc <= 64) and pf[0] == 0 and np.all(pf[1:] == pf[:-1] + pc[:-1])
+        assert set(pf.tolist()) <= set(tf.tolist()) and len(pf) <= len(tf)
         for a, c in zip(tf[::53], tc[::53]):
             k = d.keys[a:a + c]
             for l in range(20, -1, -1):
