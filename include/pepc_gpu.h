@@ -167,6 +167,55 @@ int pepc_gpu_direct_device(void *ctx, int64_t n, const double *x, const double *
  * on the context's stream, recorded by the *_device and host forms). */
 int pepc_gpu_timings(void *ctx, double *t_grow_ms, double *t_walk_ms);
 
+/* ---------------------------------------------------------------------------
+ * Batched systems (no reference counterpart: the reference solves one system
+ * per MPI communicator).  nsys independent systems in one grow/traverse:
+ * system s holds particles [offsets[s], offsets[s+1]) of the device arrays
+ * (host offsets, nsys + 1 entries, offsets[0] = 0, offsets[nsys] = n,
+ * 1 <= nsys <= 65536).  Each system gets its own box, keys, tree and walk,
+ * exactly as a separate pepc_gpu_grow_device of that range would; sorted
+ * results keep the ranges (system s sorted by key in [offsets[s],
+ * offsets[s+1])).  Traverse with pepc_gpu_traverse_device. */
+int pepc_gpu_grow_batch_device(void *ctx, int64_t nsys, const int64_t *offsets, const double *x,
+                               const double *y, const double *z, const double *q);
+
+/* ---------------------------------------------------------------------------
+ * On-device timestep around the solve (the frontends' particle push,
+ * pepc-mini module_helper.f90:205-222, kept in HBM).  One step is
+ * kick-drift-kick leapfrog: v += dt/2 a; x += dt v; grow + traverse;
+ * a = q E / m; v += dt/2 a.  The reference frontend pushes kick-drift
+ * (v += dt a; x += dt v) instead.  After every step the particles are in key
+ * order, as after pepc_grow_tree; label[i] is the input index of particle i.
+ *
+ * state_set loads n particles (host arrays, nsys systems as in
+ * grow_batch_device; offsets may be NULL when nsys = 1), runs the initial
+ * solve and records energies (record 0).  Energies are kept for up to
+ * max_records solves: record r, system s = {1/2 sum q phi, 1/2 sum m v^2}. */
+int pepc_gpu_state_set(void *ctx, int64_t n, int64_t nsys, const int64_t *offsets,
+                       const double *x, const double *y, const double *z, const double *vx,
+                       const double *vy, const double *vz, const double *q, const double *m,
+                       int64_t max_records);
+
+enum { PEPC_GPU_RUN_GRAPH = 1 };
+
+/* Runs nsteps steps of size dt, asynchronously on the context's stream.  With
+ * PEPC_GPU_RUN_GRAPH the step is captured once into a HIP graph (re-captured
+ * when dt changes) and replayed; results are identical to the eager form. */
+int pepc_gpu_run(void *ctx, int64_t nsteps, double dt, int flags);
+
+/* Host copies of the state (each may be NULL), current order. */
+int pepc_gpu_state_get(void *ctx, double *x, double *y, double *z, double *vx, double *vy,
+                       double *vz, double *q, double *m, int64_t *label);
+
+/* Device pointers to the state: arrays[11] = x y z vx vy vz q m ax ay az,
+ * *label = int64 labels.  Valid until the next state_set. */
+int pepc_gpu_state_device(void *ctx, double **arrays, const int64_t **label);
+
+/* Waits, then copies min(records, cap) energy records (cap x nsys x 2
+ * doubles) to out; *nrec = solves so far (records beyond max_records are
+ * not kept). */
+int pepc_gpu_read_energies(void *ctx, int64_t cap, double *out, int64_t *nrec);
+
 #ifdef __cplusplus
 }
 #endif

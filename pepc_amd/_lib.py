@@ -23,8 +23,12 @@ EXPORTS = [
     "pepc_gpu_config_default", "pepc_gpu_create", "pepc_gpu_destroy", "pepc_gpu_last_error",
     "pepc_gpu_set_stream", "pepc_gpu_grow", "pepc_gpu_traverse", "pepc_gpu_grow_device",
     "pepc_gpu_traverse_device", "pepc_gpu_sync", "pepc_gpu_timber", "pepc_gpu_sorted_device",
-    "pepc_gpu_read_sorted", "pepc_gpu_read_tree", "pepc_gpu_read_walk", "pepc_gpu_direct_device", "pepc_gpu_timings",
+    "pepc_gpu_read_sorted", "pepc_gpu_read_tree", "pepc_gpu_read_walk", "pepc_gpu_direct_device",
+    "pepc_gpu_timings", "pepc_gpu_grow_batch_device", "pepc_gpu_state_set", "pepc_gpu_run",
+    "pepc_gpu_state_get", "pepc_gpu_state_device", "pepc_gpu_read_energies",
 ]
+
+RUN_GRAPH = 1
 
 
 class Config(C.Structure):
@@ -89,6 +93,12 @@ def load(path: str = LIB_PATH):
     L.pepc_gpu_direct_device.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp,
                                          C.c_double, C.c_double, _vp, _vp, _vp, _vp]
     L.pepc_gpu_timings.argtypes = [_vp, _dp, _dp]
+    L.pepc_gpu_grow_batch_device.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp]
+    L.pepc_gpu_state_set.argtypes = [_vp, _i64, _i64, _vp] + [_vp] * 8 + [_i64]
+    L.pepc_gpu_run.argtypes = [_vp, _i64, C.c_double, C.c_int]
+    L.pepc_gpu_state_get.argtypes = [_vp] * 10
+    L.pepc_gpu_state_device.argtypes = [_vp, C.POINTER(_vp), C.POINTER(_vp)]
+    L.pepc_gpu_read_energies.argtypes = [_vp, _i64, _vp, C.POINTER(_i64)]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is not None and name != "pepc_gpu_config_default":

@@ -9,8 +9,14 @@ namespace pepc {
 int box_partial_blocks(int64_t n);
 void launch_box(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
                 int force_cubic, double* partial, DevBox* box, DevScalars* sc);
+// Batched systems: per-system boxes over the ranges [soff[s], soff[s+1]) and
+// the system index of every particle.
+void launch_box_batch(hipStream_t s, int nsys, const int64_t* soff, const double* x,
+                      const double* y, const double* z, int force_cubic, DevBox* boxes,
+                      uint32_t* sys, DevScalars* sc);
+// sys == nullptr: one system with boxes[0].
 void launch_keys(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
-                 const DevBox* box, unsigned long long* keys, uint32_t* idx);
+                 const DevBox* boxes, const uint32_t* sys, unsigned long long* keys, uint32_t* idx);
 
 // sort.hip
 int64_t scan_blocks(int64_t n);
@@ -22,9 +28,11 @@ void launch_radix_pass(hipStream_t s, const unsigned long long* kin, const uint3
                        uint32_t* hist, uint32_t* bsum);
 void launch_radix_pass_dev(hipStream_t s, const unsigned long long* kin, const uint32_t* vin,
                            unsigned long long* kout, uint32_t* vout, int64_t n, const uint32_t* ndev,
-                           int shift, int nbits, uint32_t* hist, uint32_t* bsum);
+                           int shift, int nbits, uint32_t* hist, uint32_t* bsum,
+                           const uint32_t* dsrc);
 void launch_sort_keys(hipStream_t s, unsigned long long* k0, uint32_t* v0, unsigned long long* k1,
-                      uint32_t* v1, int64_t n, uint32_t* hist, uint32_t* bsum);
+                      uint32_t* v1, int64_t n, uint32_t* hist, uint32_t* bsum,
+                      const uint32_t* sysid, int64_t nsys);
 void launch_gather4(hipStream_t s, int64_t n, const uint32_t* perm, const double* x, const double* y,
                     const double* z, const double* q, double* xs, double* ys, double* zs,
                     double* qs);
@@ -58,15 +66,22 @@ struct TreeBuffers {
   uint32_t* gcount;
   uint32_t* hist;
   uint32_t* bsum;
-  DevBox* box;
+  DevBox* box;                       // one box per system
   DevScalars* sc;
   int prec;                          // walk record form to write (WalkPrecision)
+  // batched systems (sys == nullptr for a single system)
+  const uint32_t* sys;               // system index per sorted particle
+  const int64_t* soff;               // system ranges, nsys + 1 entries
+  uint32_t* sysroot;                 // root node per system
+  int64_t nsys;
 };
 
-void launch_dup_detect(hipStream_t s, int64_t n, const unsigned long long* keys, DevScalars* sc);
+void launch_dup_detect(hipStream_t s, int64_t n, const unsigned long long* keys, const uint32_t* sys,
+                       DevScalars* sc);
 void launch_dupfix(hipStream_t s, int64_t n, unsigned long long* keys, double* x, double* y,
-                   double* z, const DevBox* box, long long* kd, unsigned long long* oldkeys,
-                   uint32_t* dflag, uint32_t* dlist, uint32_t* bsum, DevScalars* sc);
+                   double* z, const DevBox* boxes, const uint32_t* sys, const int64_t* soff,
+                   long long* kd, unsigned long long* oldkeys, uint32_t* dflag, uint32_t* dlist,
+                   uint32_t* bsum, DevScalars* sc);
 void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size, int group_pack);
 void launch_leaf_moments(hipStream_t s, const TreeBuffers& t);
 
@@ -91,13 +106,31 @@ struct WalkArgs {
   uint32_t* nm2p;
   uint32_t* gstat;                   // 8 words per group: mac, p2p, m2p, amb, steps, count
   uint32_t* queue;                   // work-queue head (own cache line)
+  const uint32_t* sys;               // batched systems: system per particle (else nullptr)
+  const uint32_t* sysroot;           // batched systems: root node per system
 };
 void launch_walk_reset(hipStream_t s, const WalkArgs& a);
 void launch_walk(hipStream_t s, const WalkArgs& a, int precision, int blocks);
 void launch_walk_stats(hipStream_t s, const WalkArgs& a);
 int walk_default_blocks();
 
-// direct.hip
+// step.hip: particle state of the on-device timestep (state order = key
+// order of the last grow) and the gather targets of the reorder.
+struct StepState {
+  double *x, *y, *z, *vx, *vy, *vz, *q, *m, *ax, *ay, *az;
+  long long* label;
+  double *tvx, *tvy, *tvz, *tm;
+  long long* tlabel;
+};
+void launch_kick_drift(hipStream_t s, int64_t n, double dt, const StepState& st);
+void launch_reorder_kick(hipStream_t s, int64_t n, double dt, const uint32_t* perm, const double* xs,
+                         const double* ys, const double* zs, const double* qs, const double* ex,
+                         const double* ey, const double* ez, const StepState& st);
+void launch_energy(hipStream_t s, int nsys, const int64_t* soff, int64_t n, const double* pot,
+                   const StepState& st, double* hist, int64_t cap, uint32_t* counter);
+void launch_counter_set(hipStream_t s, uint32_t* counter, uint32_t v);
+
+// walk.hip: FP64 direct sum
 void launch_direct(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
                    const double* q, int64_t ntest, const int64_t* testidx, double eps2, double sign,
                    double* ex, double* ey, double* ez, double* pot);

@@ -45,21 +45,10 @@ __global__ __launch_bounds__(kBoxThreads) void box_partial_kernel(
   }
 }
 
-__global__ __launch_bounds__(kWave) void box_final_kernel(int nblocks, const double* __restrict__ partial,
-                                                         int force_cubic, DevBox* __restrict__ box,
-                                                         DevScalars* __restrict__ sc) {
-  double mn[3] = {INFINITY, INFINITY, INFINITY};
-  double mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int b = threadIdx.x; b < nblocks; b += kWave) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      mn[d] = fmin(mn[d], partial[b * 6 + d]);
-      mx[d] = fmax(mx[d], partial[b * 6 + 3 + d]);
-    }
-  }
-#pragma unroll
-  for (int d = 0; d < 3; ++d) { mn[d] = wave_min(mn[d]); mx[d] = wave_max(mx[d]); }
-  if (threadIdx.x != 0) return;
+// module_box.f90:65-93 from the coordinate extrema: optional cube, margin
+// boxsize/10000 on both sides, refinement cell, boxlength2 table.
+__device__ void box_finalize(double mn[3], double mx[3], int force_cubic, DevBox* __restrict__ box,
+                             DevScalars* __restrict__ sc) {
   if (force_cubic) {  // module_box.f90:71-74
     double lo = fmin(fmin(mn[0], mn[1]), mn[2]);
     double hi = fmax(fmax(mx[0], mx[1]), mx[2]);
@@ -80,6 +69,68 @@ __global__ __launch_bounds__(kWave) void box_final_kernel(int nblocks, const dou
   box->b2[0] = m * m;
   for (int l = 1; l <= kMaxLevel; ++l) box->b2[l] = box->b2[l - 1] / 4.0;
   if (bad) atomicOr(&sc->flags, kFlagBadBox);
+}
+
+__global__ __launch_bounds__(kWave) void box_final_kernel(int nblocks, const double* __restrict__ partial,
+                                                         int force_cubic, DevBox* __restrict__ box,
+                                                         DevScalars* __restrict__ sc) {
+  double mn[3] = {INFINITY, INFINITY, INFINITY};
+  double mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int b = threadIdx.x; b < nblocks; b += kWave) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      mn[d] = fmin(mn[d], partial[b * 6 + d]);
+      mx[d] = fmax(mx[d], partial[b * 6 + 3 + d]);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) { mn[d] = wave_min(mn[d]); mx[d] = wave_max(mx[d]); }
+  if (threadIdx.x != 0) return;
+  box_finalize(mn, mx, force_cubic, box, sc);
+}
+
+// Batched systems: one block per system computes that system's box over its
+// particle range [soff[s], soff[s+1]) and tags the range with the system
+// index.  An empty system gets a unit box that no particle refers to.
+__global__ __launch_bounds__(kBoxThreads) void box_segment_kernel(
+    int nsys, const int64_t* __restrict__ soff, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, int force_cubic,
+    DevBox* __restrict__ boxes, uint32_t* __restrict__ sys, DevScalars* __restrict__ sc) {
+  const int sid = blockIdx.x;
+  const int64_t b = soff[sid], e = soff[sid + 1];
+  double mn[3] = {INFINITY, INFINITY, INFINITY};
+  double mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  bool nonfinite = false;
+  for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+    double v0 = x[i], v1 = y[i], v2 = z[i];
+    nonfinite |= !isfinite(v0) || !isfinite(v1) || !isfinite(v2);
+    mn[0] = fmin(mn[0], v0); mx[0] = fmax(mx[0], v0);
+    mn[1] = fmin(mn[1], v1); mx[1] = fmax(mx[1], v1);
+    mn[2] = fmin(mn[2], v2); mx[2] = fmax(mx[2], v2);
+    sys[i] = (uint32_t)sid;
+  }
+  if (__ballot(nonfinite) != 0ull && lane_id() == 0) atomicOr(&sc->flags, kFlagBadBox);
+  __shared__ double red[6][kBoxThreads / kWave];
+  const int w = threadIdx.x / kWave;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double lo = wave_min(mn[d]), hi = wave_max(mx[d]);
+    if (lane_id() == 0) { red[d][w] = lo; red[3 + d][w] = hi; }
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int d = 0; d < 3; ++d) {
+    mn[d] = red[d][0];
+    mx[d] = red[3 + d][0];
+    for (int k = 1; k < kBoxThreads / kWave; ++k) {
+      mn[d] = fmin(mn[d], red[d][k]);
+      mx[d] = fmax(mx[d], red[3 + d][k]);
+    }
+  }
+  if (e <= b) {
+    for (int d = 0; d < 3; ++d) { mn[d] = 0.0; mx[d] = 1.0; }
+  }
+  box_finalize(mn, mx, force_cubic, boxes + sid, sc);
 }
 
 // Hilbert tables packed 3 bits per entry (module_spacefilling.f90:389-390):
@@ -118,11 +169,13 @@ __device__ __forceinline__ unsigned long long hilbert3d(uint32_t c0, uint32_t c1
 __global__ __launch_bounds__(256) void keys_kernel(int64_t n, const double* __restrict__ x,
                                                    const double* __restrict__ y,
                                                    const double* __restrict__ z,
-                                                   const DevBox* __restrict__ box,
+                                                   const DevBox* __restrict__ boxes,
+                                                   const uint32_t* __restrict__ sys,
                                                    unsigned long long* __restrict__ keys,
                                                    uint32_t* __restrict__ idx) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const DevBox* __restrict__ box = boxes + (sys ? sys[i] : 0u);
   // int((x - boxmin) / s): the FP64 quotient is correctly rounded and
   // truncated toward zero, exactly as the Fortran int() conversion.
   const long long i0 = (long long)((x[i] - box->boxmin[0]) / box->cell[0]);
@@ -146,9 +199,15 @@ void launch_box(hipStream_t s, int64_t n, const double* x, const double* y, cons
   box_final_kernel<<<1, kWave, 0, s>>>(nb, partial, force_cubic, box, sc);
 }
 
+void launch_box_batch(hipStream_t s, int nsys, const int64_t* soff, const double* x,
+                      const double* y, const double* z, int force_cubic, DevBox* boxes,
+                      uint32_t* sys, DevScalars* sc) {
+  box_segment_kernel<<<nsys, kBoxThreads, 0, s>>>(nsys, soff, x, y, z, force_cubic, boxes, sys, sc);
+}
+
 void launch_keys(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
-                 const DevBox* box, unsigned long long* keys, uint32_t* idx) {
-  keys_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, x, y, z, box, keys, idx);
+                 const DevBox* boxes, const uint32_t* sys, unsigned long long* keys, uint32_t* idx) {
+  keys_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, x, y, z, boxes, sys, keys, idx);
 }
 
 }  // namespace pepc

@@ -6,9 +6,11 @@
 // device scalar block the kernels report through.  grow and traverse are
 // sequences of kernel launches on the context's stream with no host round
 // trip; the host reads the scalars once, in pepc_gpu_sync().
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/pepc_gpu.h"
 #include "kernels.h"
@@ -56,6 +58,24 @@ struct Ctx {
   DevScalars* sc_host = nullptr;    // pinned
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   TreeBuffers tb;                   // buffers of the last grow
+  // batched systems
+  int64_t nsys = 1, sys_cap = 0;
+  std::vector<int64_t> hoff;        // offsets on the device (soff), host copy
+  uint32_t* sys = nullptr;          // system per particle (input and sorted order)
+  int64_t* soff = nullptr;
+  uint32_t* sysroot = nullptr;
+  // on-device timestep
+  bool has_state = false, capturing = false;
+  double* st[11] = {};              // x y z vx vy vz q m ax ay az
+  double* tmp[4] = {};              // gathered vx vy vz m
+  long long *label = nullptr, *tlabel = nullptr;
+  double* ehist = nullptr;          // energies: record x system x {pot, kin}
+  int64_t ehist_cap = 0, nrec = 0;
+  uint32_t* ecount = nullptr;
+  hipStream_t cstream = nullptr;    // capture stream
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  double graph_dt = 0.0;
 };
 
 int fail(Ctx* c, int code, const std::string& msg) {
@@ -102,9 +122,25 @@ void free_all(Ctx* c) {
                   c->kd, c->gfirst, c->gcount, c->gstat, c->queue, c->np2p, c->nm2p, c->res[0], c->res[1],
                   c->res[2], c->res[3], c->res[4], c->dkey[0], c->dkey[1], c->dval[0], c->dval[1],
                   c->dpart, c->inv, c->flag, c->fc_rank, c->node_rank, c->nfirst, c->link, c->geom,
-                  c->mom64, c->wm32, c->wm64, c->hist, c->bsum, c->partial, c->box, c->sc};
+                  c->mom64, c->wm32, c->wm64, c->hist, c->bsum, c->partial, c->box, c->sc,
+                  c->sys, c->soff, c->sysroot, c->label, c->tlabel, c->ehist, c->ecount};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (double* p : c->st)
+    if (p) (void)hipFree(p);
+  for (double* p : c->tmp)
+    if (p) (void)hipFree(p);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+}
+
+void drop_graph(Ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
 }
 
 int ensure_particles(Ctx* c, int64_t n) {
@@ -117,6 +153,7 @@ int ensure_particles(Ctx* c, int64_t n) {
   ALLOC(c, c->cnt, nc); ALLOC(c, c->gmark, nc); ALLOC(c, c->dflag, nc); ALLOC(c, c->dlist, nc);
   ALLOC(c, c->kd, nc);
   ALLOC(c, c->gfirst, nc); ALLOC(c, c->gcount, nc); ALLOC(c, c->gstat, 8 * nc);
+  ALLOC(c, c->sys, nc);
   ALLOC(c, c->np2p, nc); ALLOC(c, c->nm2p, nc);
   const int64_t mc = (int64_t)(c->cfg.node_factor * (double)nc) + 64;
   ALLOC(c, c->dkey[0], mc); ALLOC(c, c->dkey[1], mc);
@@ -132,12 +169,42 @@ int ensure_particles(Ctx* c, int64_t n) {
   ALLOC(c, c->bsum, bc);
   c->ncap = nc;
   c->mcap = mc;
+  c->io_cap = 0;        // staging and state arrays follow the particle capacity
+  c->has_state = false;
+  drop_graph(c);
+  return 0;
+}
+
+int ensure_systems(Ctx* c, int64_t nsys) {
+  if (nsys <= c->sys_cap) return 0;
+  ALLOC(c, c->box, nsys);
+  ALLOC(c, c->soff, nsys + 1);
+  ALLOC(c, c->sysroot, nsys);
+  c->sys_cap = nsys;
+  c->hoff.clear();
+  drop_graph(c);
+  return 0;
+}
+
+// Uploads the system ranges when they differ from the ones on the device.
+int set_offsets(Ctx* c, int64_t nsys, const int64_t* off) {
+  int r = ensure_systems(c, nsys);
+  if (r) return r;
+  if ((int64_t)c->hoff.size() == nsys + 1 && std::equal(c->hoff.begin(), c->hoff.end(), off))
+    return 0;
+  if (c->capturing) return fail(c, PEPC_GPU_EINVAL, "system ranges changed during graph capture");
+  c->hoff.assign(off, off + nsys + 1);
+  HIPCHECK(c, hipMemcpyAsync(c->soff, c->hoff.data(), sizeof(int64_t) * (nsys + 1),
+                             hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  drop_graph(c);
   return 0;
 }
 
 int ensure_host_io(Ctx* c, int64_t n) {
-  // device staging buffers of the host-pointer entry points
-  if (n <= c->io_cap) return 0;
+  // device staging buffers of the host-pointer entry points (also the force
+  // outputs of the on-device timestep)
+  if (n <= c->io_cap && c->res[0]) return 0;
   for (int k = 0; k < 4; ++k) ALLOC(c, c->in[k], c->ncap);
   for (int k = 0; k < 5; ++k) ALLOC(c, c->res[k], c->ncap);
   c->io_cap = c->ncap;
@@ -164,22 +231,53 @@ int check_flags(Ctx* c, uint32_t f) {
 
 Ctx* as_ctx(void* p) { return reinterpret_cast<Ctx*>(p); }
 
+int record(Ctx* c, int k) {
+  if (!c->capturing) HIPCHECK(c, hipEventRecord(c->ev[k], c->stream));
+  return 0;
+}
+
+#define RECORD(c, k)              \
+  do {                            \
+    int _r = record((c), (k));    \
+    if (_r) return _r;            \
+  } while (0)
+
+// nsys == 1: one system (no offsets needed).  nsys > 1: particles of system s
+// occupy [off[s], off[s+1]) on input and, sorted by key, on output.
 int grow_device_impl(Ctx* c, int64_t n, const double* x, const double* y, const double* z,
-                     const double* q) {
+                     const double* q, int64_t nsys, const int64_t* off) {
   if (n < 1) return fail(c, PEPC_GPU_EINVAL, "grow: need at least one particle");
   if (n > (int64_t)0x7fffffff) return fail(c, PEPC_GPU_EINVAL, "grow: n exceeds 2^31-1");
+  if (nsys < 1 || nsys > 65536) return fail(c, PEPC_GPU_EINVAL, "grow: nsys must be in [1, 65536]");
+  if (nsys > 1) {
+    if (!off) return fail(c, PEPC_GPU_EINVAL, "grow: null system offsets");
+    if (off[0] != 0 || off[nsys] != n)
+      return fail(c, PEPC_GPU_EINVAL, "grow: offsets must start at 0 and end at n");
+    for (int64_t k = 0; k < nsys; ++k)
+      if (off[k + 1] < off[k]) return fail(c, PEPC_GPU_EINVAL, "grow: offsets must not decrease");
+  }
   int r = ensure_particles(c, n);
   if (r) return r;
+  r = ensure_systems(c, nsys);
+  if (r) return r;
+  if (nsys > 1) {
+    r = set_offsets(c, nsys, off);
+    if (r) return r;
+  }
+  const uint32_t* sys = nsys > 1 ? c->sys : nullptr;
   hipStream_t s = c->stream;
-  HIPCHECK(c, hipEventRecord(c->ev[0], s));
+  RECORD(c, 0);
   reset_scalars_kernel<<<1, 64, 0, s>>>(c->sc);
-  launch_box(s, n, x, y, z, c->cfg.force_cubic_domain, c->partial, c->box, c->sc);
-  launch_keys(s, n, x, y, z, c->box, c->k0, c->v0);
-  launch_sort_keys(s, c->k0, c->v0, c->k1, c->v1, n, c->hist, c->bsum);
+  if (nsys > 1)
+    launch_box_batch(s, (int)nsys, c->soff, x, y, z, c->cfg.force_cubic_domain, c->box, c->sys, c->sc);
+  else
+    launch_box(s, n, x, y, z, c->cfg.force_cubic_domain, c->partial, c->box, c->sc);
+  launch_keys(s, n, x, y, z, c->box, sys, c->k0, c->v0);
+  launch_sort_keys(s, c->k0, c->v0, c->k1, c->v1, n, c->hist, c->bsum, sys, nsys);
   launch_gather4(s, n, c->v0, x, y, z, q, c->xs, c->ys, c->zs, c->qs);
-  launch_dup_detect(s, n, c->k0, c->sc);
-  launch_dupfix(s, n, c->k0, c->xs, c->ys, c->zs, c->box, c->kd, c->oldkeys, c->dflag, c->dlist,
-                c->bsum, c->sc);
+  launch_dup_detect(s, n, c->k0, sys, c->sc);
+  launch_dupfix(s, n, c->k0, c->xs, c->ys, c->zs, c->box, sys, nsys > 1 ? c->soff : nullptr, c->kd,
+                c->oldkeys, c->dflag, c->dlist, c->bsum, c->sc);
   TreeBuffers t;
   t.n = n;
   t.cap = c->mcap;
@@ -194,11 +292,16 @@ int grow_device_impl(Ctx* c, int64_t n, const double* x, const double* y, const 
   t.gmark = c->gmark; t.gfirst = c->gfirst; t.gcount = c->gcount;
   t.hist = c->hist; t.bsum = c->bsum; t.box = c->box; t.sc = c->sc;
   t.prec = c->cfg.precision == PEPC_GPU_FP64 ? kWalkFP64 : kWalkFP32Kahan;
+  t.sys = sys;
+  t.soff = nsys > 1 ? c->soff : nullptr;
+  t.sysroot = c->sysroot;
+  t.nsys = nsys;
   c->tb = t;
   launch_tree_build(s, t, c->cfg.group_size, c->cfg.group_pack);
   HIPCHECK(c, hipGetLastError());
-  HIPCHECK(c, hipEventRecord(c->ev[1], s));
+  RECORD(c, 1);
   c->n = n;
+  c->nsys = nsys;
   c->grown = true;
   c->walked = false;
   return 0;
@@ -216,10 +319,12 @@ int traverse_device_impl(Ctx* c, double* ex, double* ey, double* ez, double* pot
   a.theta2 = c->cfg.theta2; a.eps2 = c->cfg.eps2; a.sign = c->cfg.sign;
   a.ex = ex; a.ey = ey; a.ez = ez; a.pot = pot; a.work = work;
   a.np2p = c->np2p; a.nm2p = c->nm2p; a.gstat = c->gstat; a.queue = c->queue;
+  a.sys = c->nsys > 1 ? c->sys : nullptr;
+  a.sysroot = c->sysroot;
   launch_walk_reset(s, a);
-  HIPCHECK(c, hipEventRecord(c->ev[2], s));
+  RECORD(c, 2);
   launch_walk(s, a, c->cfg.precision, c->walk_blocks);
-  HIPCHECK(c, hipEventRecord(c->ev[3], s));
+  RECORD(c, 3);
   launch_walk_stats(s, a);
   HIPCHECK(c, hipGetLastError());
   c->walked = true;
@@ -350,7 +455,16 @@ int pepc_gpu_grow_device(void* p, int64_t n, const double* x, const double* y, c
   Ctx* c = as_ctx(p);
   if (!c || !x || !y || !z || !q) return fail(c, PEPC_GPU_EINVAL, "grow: null argument");
   HIPCHECK(c, hipSetDevice(c->cfg.device));
-  return grow_device_impl(c, n, x, y, z, q);
+  return grow_device_impl(c, n, x, y, z, q, 1, nullptr);
+}
+
+int pepc_gpu_grow_batch_device(void* p, int64_t nsys, const int64_t* offsets, const double* x,
+                               const double* y, const double* z, const double* q) {
+  Ctx* c = as_ctx(p);
+  if (!c || !x || !y || !z || !q || !offsets) return fail(c, PEPC_GPU_EINVAL, "grow_batch: null argument");
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  if (nsys < 1 || nsys > 65536) return fail(c, PEPC_GPU_EINVAL, "grow_batch: nsys must be in [1, 65536]");
+  return grow_device_impl(c, offsets[nsys], x, y, z, q, nsys, offsets);
 }
 
 int pepc_gpu_traverse_device(void* p, double* ex, double* ey, double* ez, double* pot,
@@ -380,7 +494,7 @@ int pepc_gpu_grow(void* p, int64_t n, const double* x, const double* y, const do
   const double* src[4] = {x, y, z, q};
   for (int k = 0; k < 4; ++k)
     HIPCHECK(c, hipMemcpyAsync(c->in[k], src[k], sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-  r = grow_device_impl(c, n, c->in[0], c->in[1], c->in[2], c->in[3]);
+  r = grow_device_impl(c, n, c->in[0], c->in[1], c->in[2], c->in[3], 1, nullptr);
   if (r) return r;
   r = sync_impl(c, nullptr);
   if (r) return r;
@@ -532,6 +646,198 @@ int pepc_gpu_timings(void* p, double* t_grow_ms, double* t_walk_ms) {
   if (t_walk_ms) {
     *t_walk_ms = -1.0;
     if (c->walked && hipEventElapsedTime(&b, c->ev[2], c->ev[3]) == hipSuccess) *t_walk_ms = b;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------ timestep
+}  // extern "C"
+
+namespace {
+
+StepState step_state(Ctx* c) {
+  StepState st;
+  st.x = c->st[0]; st.y = c->st[1]; st.z = c->st[2];
+  st.vx = c->st[3]; st.vy = c->st[4]; st.vz = c->st[5];
+  st.q = c->st[6]; st.m = c->st[7];
+  st.ax = c->st[8]; st.ay = c->st[9]; st.az = c->st[10];
+  st.label = c->label;
+  st.tvx = c->tmp[0]; st.tvy = c->tmp[1]; st.tvz = c->tmp[2]; st.tm = c->tmp[3];
+  st.tlabel = c->tlabel;
+  return st;
+}
+
+// One kick-drift-kick step (dt = 0: the initial solve, which only computes
+// accelerations and energies).  Launches only; no host synchronisation.
+int step_impl(Ctx* c, double dt) {
+  const int64_t n = c->n;
+  const StepState st = step_state(c);
+  hipStream_t s = c->stream;
+  if (dt != 0.0) launch_kick_drift(s, n, dt, st);
+  int r = grow_device_impl(c, n, st.x, st.y, st.z, st.q, c->nsys,
+                           c->nsys > 1 ? c->hoff.data() : nullptr);
+  if (r) return r;
+  r = traverse_device_impl(c, c->res[0], c->res[1], c->res[2], c->res[3], c->res[4]);
+  if (r) return r;
+  s = c->stream;
+  launch_reorder_kick(s, n, dt, c->v0, c->xs, c->ys, c->zs, c->qs, c->res[0], c->res[1], c->res[2],
+                      st);
+  launch_energy(s, (int)c->nsys, c->nsys > 1 ? c->soff : nullptr, n, c->res[3], st, c->ehist,
+                c->ehist_cap, c->ecount);
+  HIPCHECK(c, hipGetLastError());
+  c->nrec += 1;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pepc_gpu_state_set(void* p, int64_t n, int64_t nsys, const int64_t* offsets, const double* x,
+                       const double* y, const double* z, const double* vx, const double* vy,
+                       const double* vz, const double* q, const double* m, int64_t max_records) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PEPC_GPU_EINVAL;
+  const double* src[8] = {x, y, z, vx, vy, vz, q, m};
+  for (const double* a : src)
+    if (!a) return fail(c, PEPC_GPU_EINVAL, "state_set: null array");
+  if (n < 1 || n > (int64_t)0x7fffffff) return fail(c, PEPC_GPU_EINVAL, "state_set: bad n");
+  if (nsys < 1 || nsys > 65536) return fail(c, PEPC_GPU_EINVAL, "state_set: nsys must be in [1, 65536]");
+  if (nsys > 1) {
+    if (!offsets) return fail(c, PEPC_GPU_EINVAL, "state_set: null offsets");
+    if (offsets[0] != 0 || offsets[nsys] != n)
+      return fail(c, PEPC_GPU_EINVAL, "state_set: offsets must start at 0 and end at n");
+    for (int64_t k = 0; k < nsys; ++k)
+      if (offsets[k + 1] < offsets[k])
+        return fail(c, PEPC_GPU_EINVAL, "state_set: offsets must not decrease");
+  }
+  if (max_records < 1) return fail(c, PEPC_GPU_EINVAL, "state_set: max_records must be >= 1");
+  for (int64_t k = 0; k < n; ++k)
+    if (!(m[k] > 0.0)) return fail(c, PEPC_GPU_EINVAL, "state_set: masses must be positive");
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  int r = ensure_particles(c, n);
+  if (r) return r;
+  r = ensure_host_io(c, n);
+  if (r) return r;
+  r = ensure_systems(c, nsys);
+  if (r) return r;
+  if (nsys > 1) {
+    r = set_offsets(c, nsys, offsets);
+    if (r) return r;
+  }
+  for (int k = 0; k < 11; ++k) ALLOC(c, c->st[k], c->ncap);
+  for (int k = 0; k < 4; ++k) ALLOC(c, c->tmp[k], c->ncap);
+  ALLOC(c, c->label, c->ncap);
+  ALLOC(c, c->tlabel, c->ncap);
+  ALLOC(c, c->ehist, max_records * nsys * 2);
+  ALLOC(c, c->ecount, 64);
+  c->ehist_cap = max_records;
+  drop_graph(c);
+  for (int k = 0; k < 8; ++k)
+    HIPCHECK(c, hipMemcpyAsync(c->st[k], src[k], sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  {
+    std::vector<long long> lab(n);
+    for (int64_t k = 0; k < n; ++k) lab[k] = k;
+    HIPCHECK(c, hipMemcpyAsync(c->label, lab.data(), sizeof(long long) * n, hipMemcpyHostToDevice,
+                               c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+  }
+  // accelerations start at zero; the initial solve (dt = 0) sets them
+  launch_counter_set(c->stream, c->ecount, 0u);
+  c->n = n;
+  c->nsys = nsys;
+  c->nrec = 0;
+  c->has_state = true;
+  r = step_impl(c, 0.0);   // initial forces and energies: record 0
+  if (r) return r;
+  return sync_impl(c, nullptr);
+}
+
+int pepc_gpu_run(void* p, int64_t nsteps, double dt, int flags) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PEPC_GPU_EINVAL;
+  if (!c->has_state) return fail(c, PEPC_GPU_EINVAL, "run: no particle state (call state_set)");
+  if (nsteps < 0 || !(dt == dt)) return fail(c, PEPC_GPU_EINVAL, "run: bad nsteps or dt");
+  if (flags & ~PEPC_GPU_RUN_GRAPH) return fail(c, PEPC_GPU_EINVAL, "run: unknown flags");
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  if (!(flags & PEPC_GPU_RUN_GRAPH)) {
+    for (int64_t k = 0; k < nsteps; ++k) {
+      int r = step_impl(c, dt);
+      if (r) return r;
+    }
+    return 0;
+  }
+  if (nsteps == 0) return 0;
+  if (!c->gexec || c->graph_dt != dt) {
+    drop_graph(c);
+    if (!c->cstream) HIPCHECK(c, hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));
+    // order the capture stream after the work already queued on the context stream
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    hipStream_t user = c->stream;
+    c->stream = c->cstream;
+    c->capturing = true;
+    hipError_t e = hipStreamBeginCapture(c->cstream, hipStreamCaptureModeThreadLocal);
+    int r = e == hipSuccess ? step_impl(c, dt) : 0;
+    hipGraph_t g = nullptr;
+    hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(c->cstream, &g) : e;
+    c->capturing = false;
+    c->stream = user;
+    c->nrec -= 1;   // the captured step did not run
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamBeginCapture");
+    if (r) {
+      if (g) (void)hipGraphDestroy(g);
+      return r;
+    }
+    if (e2 != hipSuccess) return hip_fail(c, e2, "hipStreamEndCapture");
+    c->graph = g;
+    HIPCHECK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    c->graph_dt = dt;
+  }
+  for (int64_t k = 0; k < nsteps; ++k) {
+    HIPCHECK(c, hipGraphLaunch(c->gexec, c->stream));
+    c->nrec += 1;
+  }
+  c->walked = true;
+  return 0;
+}
+
+int pepc_gpu_state_get(void* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                       double* q, double* m, int64_t* label) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PEPC_GPU_EINVAL;
+  if (!c->has_state) return fail(c, PEPC_GPU_EINVAL, "state_get: no particle state");
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  int r = sync_impl(c, nullptr);
+  if (r) return r;
+  double* dst[8] = {x, y, z, vx, vy, vz, q, m};
+  for (int k = 0; k < 8; ++k)
+    if (dst[k]) HIPCHECK(c, hipMemcpy(dst[k], c->st[k], sizeof(double) * c->n, hipMemcpyDeviceToHost));
+  if (label)
+    HIPCHECK(c, hipMemcpy(label, c->label, sizeof(int64_t) * c->n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pepc_gpu_state_device(void* p, double** arrays, const int64_t** label) {
+  Ctx* c = as_ctx(p);
+  if (!c || !arrays) return PEPC_GPU_EINVAL;
+  if (!c->has_state) return fail(c, PEPC_GPU_EINVAL, "state_device: no particle state");
+  for (int k = 0; k < 11; ++k) arrays[k] = c->st[k];
+  if (label) *label = reinterpret_cast<const int64_t*>(c->label);
+  return 0;
+}
+
+int pepc_gpu_read_energies(void* p, int64_t cap, double* out, int64_t* nrec) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PEPC_GPU_EINVAL;
+  if (!c->has_state) return fail(c, PEPC_GPU_EINVAL, "read_energies: no particle state");
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  int r = sync_impl(c, nullptr);
+  if (r) return r;
+  const int64_t k = c->nrec < c->ehist_cap ? c->nrec : c->ehist_cap;
+  if (nrec) *nrec = c->nrec;
+  if (out && k > 0) {
+    const int64_t m = (k < cap ? k : cap) * c->nsys * 2;
+    HIPCHECK(c, hipMemcpy(out, c->ehist, sizeof(double) * m, hipMemcpyDeviceToHost));
   }
   return 0;
 }

@@ -115,8 +115,16 @@ constexpr int kSortWaves = kSortThreads / kWave;
 constexpr int kSortTile = kSortThreads * kSortItems;   // 4096
 constexpr int kRadix = 256;
 
+// The digit of element e is bits [shift, shift+nbits) of its key, or, when
+// dsrc is given, of dsrc[value] (used to order by system after the keys).
+__device__ __forceinline__ uint32_t radix_digit(unsigned long long key, uint32_t val,
+                                                const uint32_t* dsrc, int shift, uint32_t mask) {
+  return dsrc ? (dsrc[val] >> shift) & mask : (uint32_t)(key >> shift) & mask;
+}
+
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(
-    const unsigned long long* __restrict__ keys, int64_t n, const uint32_t* __restrict__ ndev,
+    const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ vals,
+    const uint32_t* __restrict__ dsrc, int64_t n, const uint32_t* __restrict__ ndev,
     int shift, uint32_t mask, uint32_t* __restrict__ hist, int nblocks) {
   if (ndev && (int64_t)*ndev < n) n = *ndev;
   __shared__ uint32_t cnt[kRadix];
@@ -126,7 +134,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(
 #pragma unroll 4
   for (int j = 0; j < kSortItems; ++j) {
     int64_t e = base + j * kSortThreads + threadIdx.x;
-    if (e < n) atomicAdd(&cnt[(uint32_t)(keys[e] >> shift) & mask], 1u);
+    if (e < n) atomicAdd(&cnt[radix_digit(keys[e], dsrc ? vals[e] : 0u, dsrc, shift, mask)], 1u);
   }
   __syncthreads();
   hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
@@ -134,7 +142,8 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(
 
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const unsigned long long* __restrict__ kin, const uint32_t* __restrict__ vin,
-    unsigned long long* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n,
+    unsigned long long* __restrict__ kout, uint32_t* __restrict__ vout,
+    const uint32_t* __restrict__ dsrc, int64_t n,
     const uint32_t* __restrict__ ndev, int shift, int nbits, const uint32_t* __restrict__ hscan,
     int nblocks) {
   if (ndev && (int64_t)*ndev < n) n = *ndev;
@@ -158,7 +167,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const bool valid = e < n;
     kk[j] = valid ? kin[e] : 0ull;
     vv[j] = valid ? vin[e] : 0u;
-    const uint32_t d = (uint32_t)(kk[j] >> shift) & mask;
+    const uint32_t d = valid ? radix_digit(kk[j], vv[j], dsrc, shift, mask) : 0u;
     dig[j] = d;
     unsigned long long peers = __ballot(valid);
     for (int b = 0; b < nbits; ++b) {
@@ -213,6 +222,22 @@ __global__ __launch_bounds__(256) void gather4_kernel(int64_t n, const uint32_t*
   qs[i] = q[p];
 }
 
+__global__ __launch_bounds__(256) void copy_pairs_kernel(int64_t n,
+                                                         const unsigned long long* __restrict__ kin,
+                                                         const uint32_t* __restrict__ vin,
+                                                         unsigned long long* __restrict__ kout,
+                                                         uint32_t* __restrict__ vout) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  kout[i] = kin[i];
+  vout[i] = vin[i];
+}
+
+void launch_copy_pairs(hipStream_t s, const unsigned long long* kin, const uint32_t* vin,
+                       unsigned long long* kout, uint32_t* vout, int64_t n) {
+  copy_pairs_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, kin, vin, kout, vout);
+}
+
 }  // namespace
 
 int64_t scan_blocks(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
@@ -232,24 +257,31 @@ void launch_scan(hipStream_t s, const uint32_t* in, uint32_t* out, int64_t n, ui
 // (n then only sizes the grid), so no host round trip is needed.
 void launch_radix_pass_dev(hipStream_t s, const unsigned long long* kin, const uint32_t* vin,
                            unsigned long long* kout, uint32_t* vout, int64_t n, const uint32_t* ndev,
-                           int shift, int nbits, uint32_t* hist, uint32_t* bsum) {
+                           int shift, int nbits, uint32_t* hist, uint32_t* bsum,
+                           const uint32_t* dsrc) {
   const int nb = (int)sort_blocks(n);
-  radix_hist_kernel<<<nb, kSortThreads, 0, s>>>(kin, n, ndev, shift, (1u << nbits) - 1u, hist, nb);
+  radix_hist_kernel<<<nb, kSortThreads, 0, s>>>(kin, vin, dsrc, n, ndev, shift, (1u << nbits) - 1u,
+                                                hist, nb);
   launch_scan(s, hist, hist, (int64_t)kRadix * nb, bsum, nullptr);
-  radix_scatter_kernel<<<nb, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, ndev, shift, nbits, hist,
-                                                   nb);
+  radix_scatter_kernel<<<nb, kSortThreads, 0, s>>>(kin, vin, kout, vout, dsrc, n, ndev, shift, nbits,
+                                                   hist, nb);
 }
 
 void launch_radix_pass(hipStream_t s, const unsigned long long* kin, const uint32_t* vin,
                        unsigned long long* kout, uint32_t* vout, int64_t n, int shift, int nbits,
                        uint32_t* hist, uint32_t* bsum) {
-  launch_radix_pass_dev(s, kin, vin, kout, vout, n, nullptr, shift, nbits, hist, bsum);
+  launch_radix_pass_dev(s, kin, vin, kout, vout, n, nullptr, shift, nbits, hist, bsum, nullptr);
 }
 
 // Full sort of particle keys over bits [0, 60).  Input in (k0, v0); the sorted
-// result ends in (k0, v0) again after an even number of passes.
+// result ends in (k0, v0) again after an even number of passes.  With a system
+// index per input particle (sysid != nullptr, values < nsys <= 65536), two more
+// stable passes over its 16 bits order the particles by (system, key); the
+// upper pass only runs when there are more than 256 systems, and then the
+// lower one goes to (k1, v1) and the upper one back.
 void launch_sort_keys(hipStream_t s, unsigned long long* k0, uint32_t* v0, unsigned long long* k1,
-                      uint32_t* v1, int64_t n, uint32_t* hist, uint32_t* bsum) {
+                      uint32_t* v1, int64_t n, uint32_t* hist, uint32_t* bsum,
+                      const uint32_t* sysid, int64_t nsys) {
   static const int shifts[8] = {0, 8, 16, 24, 32, 40, 48, 56};
   static const int widths[8] = {8, 8, 8, 8, 8, 8, 8, 4};
   for (int p = 0; p < 8; ++p) {
@@ -257,6 +289,14 @@ void launch_sort_keys(hipStream_t s, unsigned long long* k0, uint32_t* v0, unsig
       launch_radix_pass(s, k0, v0, k1, v1, n, shifts[p], widths[p], hist, bsum);
     else
       launch_radix_pass(s, k1, v1, k0, v0, n, shifts[p], widths[p], hist, bsum);
+  }
+  if (sysid) {
+    launch_radix_pass_dev(s, k0, v0, k1, v1, n, nullptr, 0, 8, hist, bsum, sysid);
+    if (nsys > 256) {
+      launch_radix_pass_dev(s, k1, v1, k0, v0, n, nullptr, 8, 8, hist, bsum, sysid);
+    } else {
+      launch_copy_pairs(s, k1, v1, k0, v0, n);
+    }
   }
 }
 

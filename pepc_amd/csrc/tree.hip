@@ -20,6 +20,11 @@
 // Groups (walk tiles, module_tree_walk_simple.f90:135-212): the maximal key
 // cells holding at most G particles = children with count <= G of twigs with
 // count > G (or the root).
+//
+// Batched systems: particles sorted by (system, key) form one forest with a
+// root per system at level 0; the level-synchronous steps above need no other
+// change, every system keeps its own box (keys, boxlength2) and groups never
+// cross a system boundary.
 #include "kernels.h"
 
 namespace pepc {
@@ -29,23 +34,31 @@ namespace {
 constexpr int kT = 256;
 
 // ------------------------------------------------------------ duplicate keys
+// Batched systems (sys != nullptr): keys are ordered within each system's
+// range only, and every check and fix-up stays inside one range.
+__device__ __forceinline__ bool same_sys(const uint32_t* sys, int64_t i, int64_t j) {
+  return sys == nullptr || sys[i] == sys[j];
+}
+
 __global__ void dup_detect_kernel(int64_t n, const unsigned long long* __restrict__ keys,
-                                  DevScalars* __restrict__ sc) {
+                                  const uint32_t* __restrict__ sys, DevScalars* __restrict__ sc) {
   uint32_t c = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i + 1 < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    c += keys[i + 1] <= keys[i] ? 1u : 0u;
+    c += (keys[i + 1] <= keys[i] && same_sys(sys, i, i + 1)) ? 1u : 0u;
   c = wave_sum(c);
   if (lane_id() == 0 && c) atomicAdd(&sc->dup_count, c);
 }
 
 __global__ void dup_prepare_kernel(int64_t n, const unsigned long long* __restrict__ keys,
+                                   const uint32_t* __restrict__ sys,
                                    long long* __restrict__ kd, unsigned long long* __restrict__ old,
                                    uint32_t* __restrict__ flag, const DevScalars* __restrict__ sc) {
   if (sc->dup_count == 0) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    long long d = (i + 1 < n) ? (long long)(keys[i + 1] - keys[i]) : 1ll;  // last rank
+    // the last particle of a range behaves as the last particle of a rank
+    long long d = (i + 1 < n && same_sys(sys, i, i + 1)) ? (long long)(keys[i + 1] - keys[i]) : 1ll;
     kd[i] = d;
     old[i] = keys[i];
     flag[i] = d < 1 ? 1u : 0u;
@@ -64,18 +77,33 @@ __global__ void dup_compact_kernel(int64_t n, const long long* __restrict__ kd,
 // module_domains.f90:248-297, one thread, in the reference's loop order.
 // Only positions that start as duplicates are ever modified to a smaller
 // key difference, so visiting the compacted list in order equals visiting
-// every position.
-__global__ void dup_serial_kernel(int64_t n, unsigned long long* __restrict__ keys,
-                                  long long* __restrict__ kd, const uint32_t* __restrict__ list,
+// every position.  With batched systems the loop runs on each system's range
+// (base pointer and length of that range), systems in increasing order.
+__global__ void dup_serial_kernel(int64_t n_all, unsigned long long* __restrict__ keys_all,
+                                  long long* __restrict__ kd_all, const uint32_t* __restrict__ list,
+                                  const uint32_t* __restrict__ sys, const int64_t* __restrict__ soff,
                                   DevScalars* __restrict__ sc) {
   const uint32_t nd = sc->dup_count;
   if (nd == 0 || threadIdx.x != 0) return;
-  if ((long long)(keys[n - 1] - keys[0]) + 1 < n) {   // :254-262
-    atomicOr(&sc->flags, kFlagDupKeys);
-    return;
-  }
+  int64_t cur = -1;
+  long long n = n_all;
+  unsigned long long* keys = keys_all;
+  long long* kd = kd_all;
   for (uint32_t r = 0; r < nd; ++r) {
-    const long long i1 = (long long)list[r] + 1;        // 1-based as in the source
+    const int64_t gi = list[r];
+    const int64_t sid = sys ? (int64_t)sys[gi] : 0;
+    if (sid != cur) {
+      cur = sid;
+      const int64_t b = sys ? soff[sid] : 0;
+      n = sys ? (long long)(soff[sid + 1] - b) : n_all;
+      keys = keys_all + b;
+      kd = kd_all + b;
+      if ((long long)(keys[n - 1] - keys[0]) + 1 < n) {   // :254-262
+        atomicOr(&sc->flags, kFlagDupKeys);
+        return;
+      }
+    }
+    const long long i1 = (long long)(gi - (keys - keys_all)) + 1;   // 1-based as in the source
     if (kd[i1 - 1] >= 1) continue;
     const long long jmax = (n - i1) > i1 ? (n - i1) : i1;
     for (long long j = 1; j <= jmax; ++j) {
@@ -124,12 +152,14 @@ __device__ void key_to_intcoord(unsigned long long key, long long ic[3]) {
 __global__ void dup_recoord_kernel(int64_t n, const unsigned long long* __restrict__ keys,
                                    const unsigned long long* __restrict__ old,
                                    double* __restrict__ x, double* __restrict__ y,
-                                   double* __restrict__ z, const DevBox* __restrict__ box,
+                                   double* __restrict__ z, const DevBox* __restrict__ boxes,
+                                   const uint32_t* __restrict__ sys,
                                    const DevScalars* __restrict__ sc) {
   if (sc->dup_count == 0) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     if (keys[i] == old[i]) continue;
+    const DevBox* __restrict__ box = boxes + (sys ? sys[i] : 0u);
     long long ic[3];
     key_to_intcoord(keys[i], ic);
     x[i] = ((double)ic[0] + 0.5) * box->cell[0] + box->boxmin[0];
@@ -146,17 +176,20 @@ __device__ __forceinline__ bool tree_broken(const DevScalars* sc) {
   return (sc->flags & (kFlagNodeOverflow | kFlagDupKeys | kFlagBadBox)) != 0u;
 }
 
+// A system boundary acts like the ends of the array: the first particle of a
+// system owns that system's root (level 0).
 __global__ void levels_kernel(int64_t n, const unsigned long long* __restrict__ keys,
+                              const uint32_t* __restrict__ sys,
                               uint8_t* __restrict__ lo, uint8_t* __restrict__ hi,
                               uint32_t* __restrict__ cnt, DevScalars* __restrict__ sc) {
   for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < n;
        a += (int64_t)gridDim.x * blockDim.x) {
     int la = 0, lb = 0;
     bool dup = false;
-    if (a > 0) {
+    if (a > 0 && same_sys(sys, a - 1, a)) {
       if (keys[a - 1] == keys[a]) dup = true; else la = split_level(keys[a - 1], keys[a]);
     }
-    if (a + 1 < n) {
+    if (a + 1 < n && same_sys(sys, a, a + 1)) {
       if (keys[a + 1] == keys[a]) dup = true; else lb = split_level(keys[a], keys[a + 1]);
     }
     if (dup) {
@@ -314,15 +347,17 @@ __global__ __launch_bounds__(kT) void upward_kernel(int level, int prec, NodeLin
                                                    double* mom, NodeGeom* __restrict__ geom,
                                                    NodeMoments<float>* __restrict__ wm32,
                                                    NodeMoments<double>* __restrict__ wm64,
-                                                   const DevBox* __restrict__ box,
+                                                   const DevBox* __restrict__ boxes,
+                                                   const uint32_t* __restrict__ sys,
+                                                   const uint32_t* __restrict__ nfirst,
                                                    const DevScalars* __restrict__ sc) {
   if (tree_broken(sc)) return;
   const int64_t b = sc->level_offset[level], e = sc->level_offset[level + 1];
-  const double b2 = box->b2[level];
   for (int64_t i = b + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < e;
        i += (int64_t)gridDim.x * blockDim.x) {
     const NodeLink lk = link[i];
     if (lk.nchild == 0) continue;
+    const double b2 = boxes[sys ? sys[nfirst[i]] : 0u].b2[level];
     const uint32_t c0 = lk.first, nc = lk.nchild;
     // child j as (is_leaf, particle or node index)
     uint32_t cidx[8];
@@ -418,8 +453,8 @@ __global__ void gmark_clear_kernel(int64_t n, uint32_t* __restrict__ gmark) {
     gmark[a] = 0;
 }
 
-// A twig holding more than G particles marks each child with at most G as a
-// group start (the child's first particle gets the group size).  With
+// A root holding at most G particles is one group.  A twig holding more than
+// G particles marks each child with at most G as a group start (the child's first particle gets the group size).  With
 // pack > 0, runs of consecutive such children are merged while the merged
 // size stays <= pack; a child that is split further ends a run.
 __global__ void group_mark_kernel(int group_size, int pack, const NodeLink* __restrict__ link,
@@ -427,11 +462,12 @@ __global__ void group_mark_kernel(int group_size, int pack, const NodeLink* __re
                                   const DevScalars* __restrict__ sc) {
   if (tree_broken(sc)) return;
   const int64_t m = sc->nnodes;
+  const int64_t nroot = sc->level_offset[1];
   const uint32_t G = (uint32_t)group_size, P = (uint32_t)pack;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && m > 0 && link[0].count <= G) gmark[0] = link[0].count;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
        i += (int64_t)gridDim.x * blockDim.x) {
     const NodeLink lk = link[i];
+    if (i < nroot && lk.count <= G) gmark[nfirst[i]] = lk.count;   // a root that is one group
     if (lk.nchild == 0 || lk.count <= G) continue;
     uint32_t run_first = 0xFFFFFFFFu, run_cnt = 0;
     for (uint32_t j = 0; j < lk.nchild; ++j) {
@@ -475,6 +511,16 @@ __global__ void group_emit_kernel(int64_t n, const uint32_t* __restrict__ gmark,
   }
 }
 
+// Root node of each system (level-0 nodes, one per non-empty system).
+__global__ void sysroot_kernel(const uint32_t* __restrict__ sys, const uint32_t* __restrict__ nfirst,
+                               uint32_t* __restrict__ sysroot, const DevScalars* __restrict__ sc) {
+  if (tree_broken(sc)) return;
+  const int64_t nroot = sc->level_offset[1];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nroot;
+       i += (int64_t)gridDim.x * blockDim.x)
+    sysroot[sys[nfirst[i]]] = (uint32_t)i;
+}
+
 }  // namespace
 
 static unsigned grid_for(int64_t n) {
@@ -484,28 +530,30 @@ static unsigned grid_for(int64_t n) {
   return (unsigned)b;
 }
 
-void launch_dup_detect(hipStream_t s, int64_t n, const unsigned long long* keys, DevScalars* sc) {
-  dup_detect_kernel<<<grid_for(n), kT, 0, s>>>(n, keys, sc);
+void launch_dup_detect(hipStream_t s, int64_t n, const unsigned long long* keys, const uint32_t* sys,
+                       DevScalars* sc) {
+  dup_detect_kernel<<<grid_for(n), kT, 0, s>>>(n, keys, sys, sc);
 }
 
 void launch_dupfix(hipStream_t s, int64_t n, unsigned long long* keys, double* x, double* y,
-                   double* z, const DevBox* box, long long* kd, unsigned long long* oldkeys,
-                   uint32_t* dflag, uint32_t* dlist, uint32_t* bsum, DevScalars* sc) {
-  dup_prepare_kernel<<<grid_for(n), kT, 0, s>>>(n, keys, kd, oldkeys, dflag, sc);
+                   double* z, const DevBox* boxes, const uint32_t* sys, const int64_t* soff,
+                   long long* kd, unsigned long long* oldkeys, uint32_t* dflag, uint32_t* dlist,
+                   uint32_t* bsum, DevScalars* sc) {
+  dup_prepare_kernel<<<grid_for(n), kT, 0, s>>>(n, keys, sys, kd, oldkeys, dflag, sc);
   launch_scan(s, dflag, dflag, n, bsum, nullptr);
   dup_compact_kernel<<<grid_for(n), kT, 0, s>>>(n, kd, dflag, dlist, sc);
-  dup_serial_kernel<<<1, 1, 0, s>>>(n, keys, kd, dlist, sc);
-  dup_recoord_kernel<<<grid_for(n), kT, 0, s>>>(n, keys, oldkeys, x, y, z, box, sc);
+  dup_serial_kernel<<<1, 1, 0, s>>>(n, keys, kd, dlist, sys, soff, sc);
+  dup_recoord_kernel<<<grid_for(n), kT, 0, s>>>(n, keys, oldkeys, x, y, z, boxes, sys, sc);
 }
 
 void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size, int group_pack) {
   const int64_t n = t.n, cap = t.cap;
-  levels_kernel<<<grid_for(n), kT, 0, s>>>(n, t.keys, t.lo, t.hi, t.cnt, t.sc);
+  levels_kernel<<<grid_for(n), kT, 0, s>>>(n, t.keys, t.sys, t.lo, t.hi, t.cnt, t.sc);
   launch_scan(s, t.cnt, t.cnt, n, t.bsum, &t.sc->nnodes);
   dfs_emit_kernel<<<grid_for(n), kT, 0, s>>>(n, cap, t.lo, t.hi, t.cnt, t.dkey[0], t.dval[0], t.dpart, t.sc);
   // one stable pass keyed on the level; entries beyond nnodes are never read
   launch_radix_pass_dev(s, t.dkey[0], t.dval[0], t.dkey[1], t.dval[1], cap, &t.sc->nnodes, 0, 5,
-                        t.hist, t.bsum);
+                        t.hist, t.bsum, nullptr);
   level_offsets_kernel<<<1, 64, 0, s>>>(t.hist, (int)sort_blocks(cap), t.sc);
   inv_kernel<<<grid_for(cap), kT, 0, s>>>(t.dval[1], t.inv, t.sc);
   node_fill_kernel<<<grid_for(cap), kT, 0, s>>>(cap, t.dkey[1], t.dval[1], t.dpart, t.inv, t.hi,
@@ -516,13 +564,15 @@ void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size, int 
   nchild_kernel<<<grid_for(cap), kT, 0, s>>>(t.link, t.fc_rank, t.node_rank, t.sc);
   for (int l = kMaxLevel; l >= 0; --l)
     upward_kernel<<<grid_for(cap / 4), kT, 0, s>>>(l, t.prec, t.link, t.x, t.y, t.z, t.q, t.mom64,
-                                                   t.geom, t.wm32, t.wm64, t.box, t.sc);
+                                                   t.geom, t.wm32, t.wm64, t.box, t.sys, t.nfirst,
+                                                   t.sc);
   gmark_clear_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark);
   group_mark_kernel<<<grid_for(cap), kT, 0, s>>>(group_size, group_pack, t.link, t.nfirst, t.gmark,
                                                  t.sc);
   group_flag_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark, t.cnt);
   launch_scan(s, t.cnt, t.cnt, n, t.bsum, &t.sc->ngroups);
   group_emit_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark, t.cnt, t.gfirst, t.gcount);
+  if (t.sys) sysroot_kernel<<<grid_for(t.nsys), kT, 0, s>>>(t.sys, t.nfirst, t.sysroot, t.sc);
 }
 
 void launch_leaf_moments(hipStream_t s, const TreeBuffers& t) {

@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
     uint32_t sp = 1, nm = 0, np = 0, cnt_p2p = 0, cnt_m2p = 0, namb = 0, nsteps = 0;
     unsigned long long nmac = 0;
     bool overflow = false;
-    if (lane == 0) stack[0] = 0;
+    if (lane == 0) stack[0] = a.sys ? a.sysroot[a.sys[a0]] : 0u;   // the group's own root
     __syncthreads();
 
     // Batches: entry k goes to slice k mod S; the loop count is uniform.

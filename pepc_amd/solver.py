@@ -94,6 +94,24 @@ class Solver:
             return self.sync()
         return None
 
+    def grow_batch(self, x, y, z, q, offsets, sync=True):
+        """Builds one tree per system: system s holds particles
+        [offsets[s], offsets[s+1]) of the concatenated inputs."""
+        self._bind_stream()
+        xs = [self._dev(v) for v in (x, y, z, q)]
+        self._inputs = xs
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        nsys = len(off) - 1
+        if nsys < 1 or off[-1] != xs[0].numel():
+            raise ValueError("grow_batch: offsets must have nsys + 1 entries ending at n")
+        self._check(self.lib.pepc_gpu_grow_batch_device(self.h, nsys, off.ctypes.data_as(C.c_void_p),
+                                                        *[_ptr(v) for v in xs]))
+        self.n = int(off[-1])
+        self.offsets = off
+        if sync:
+            return self.sync()
+        return None
+
     def traverse(self, out=None, sync=True):
         """Fields, potentials and work for the grown particles (sorted order)."""
         t = self.torch
@@ -118,6 +136,55 @@ class Solver:
         a, b = C.c_double(), C.c_double()
         self._check(self.lib.pepc_gpu_timings(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    # -------------------------------------------------------------- timestep
+    def state_set(self, x, y, z, v, q, m, offsets=None, max_records=1024):
+        """Loads particle state (host arrays, input order) for the on-device
+        kick-drift-kick timestep and runs the initial solve."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                (x, y, z, v[:, 0], v[:, 1], v[:, 2], q, m)]
+        n = len(arrs[0])
+        if offsets is None:
+            off = np.array([0, n], np.int64)
+        else:
+            off = np.ascontiguousarray(offsets, dtype=np.int64)
+        self._bind_stream()
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)
+        self._check(self.lib.pepc_gpu_state_set(self.h, n, len(off) - 1, vp(off), *[vp(a) for a in arrs],
+                                                int(max_records)))
+        self.n = n
+        self.offsets = off
+        self._state_arrays = arrs
+        self._max_records = int(max_records)
+
+    def run(self, nsteps, dt, graph=False, sync=True):
+        self._bind_stream()
+        self._check(self.lib.pepc_gpu_run(self.h, int(nsteps), float(dt),
+                                          _lib.RUN_GRAPH if graph else 0))
+        if sync:
+            return self.sync()
+        return None
+
+    def state_get(self):
+        """Host copies of the state: dict of x, y, z, v (n, 3), q, m, label."""
+        n = self.n
+        a = [np.empty(n) for _ in range(8)]
+        lab = np.empty(n, np.int64)
+        vp = lambda t: t.ctypes.data_as(C.c_void_p)
+        self._check(self.lib.pepc_gpu_state_get(self.h, *[vp(t) for t in a], vp(lab)))
+        return {"x": a[0], "y": a[1], "z": a[2], "v": np.stack(a[3:6], 1), "q": a[6], "m": a[7],
+                "label": lab}
+
+    def energies(self):
+        """(records, nsys, 2) array of {1/2 sum q phi, 1/2 sum m v^2} per solve."""
+        nr = C.c_int64()
+        self._check(self.lib.pepc_gpu_read_energies(self.h, 0, None, C.byref(nr)))
+        nsys = len(self.offsets) - 1
+        k = min(nr.value, self._max_records)
+        out = np.empty((k, nsys, 2))
+        self._check(self.lib.pepc_gpu_read_energies(self.h, k, out.ctypes.data_as(C.c_void_p),
+                                                    C.byref(nr)))
+        return out
 
     # ------------------------------------------------------------- read-back
     def sorted_host(self):

@@ -117,6 +117,15 @@ def ensemble(nsys=256, seed=4, nmin=1024, nmax=65536):
     return systems
 
 
+def concat(systems):
+    """Concatenates independent systems into one batched input: returns
+    (x, y, z, q, m, v, offsets) with system s in [offsets[s], offsets[s+1])."""
+    off = np.zeros(len(systems) + 1, np.int64)
+    off[1:] = np.cumsum([s.n for s in systems])
+    cat = lambda name: np.concatenate([getattr(s, name) for s in systems])
+    return cat("x"), cat("y"), cat("z"), cat("q"), cat("m"), cat("v"), off
+
+
 def adjoint(n=262144, seed=5):
     r = _rng(seed)
     p = r.random((n, 3))
