@@ -203,6 +203,18 @@ enum { PEPC_GPU_RUN_GRAPH = 1 };
  * when dt changes) and replayed; results are identical to the eager form. */
 int pepc_gpu_run(void *ctx, int64_t nsteps, double dt, int flags);
 
+/* ---------------------------------------------------------------------------
+ * Adjoint of the solve (no reference counterpart).  For a loss L with
+ * gex,gey,gez = dL/dE and gpot = dL/dpot at the grown particles (device
+ * arrays, sorted order), writes gx,gy,gz = dL/dx and gq = dL/dq (sorted
+ * order), FP64, with the tree and the interaction lists of the last grow held
+ * fixed (the lists are piecewise constant in the positions).  Particles moved
+ * by the duplicate-key fix-up are treated as fixed inputs.  Asynchronous on
+ * the context's stream. */
+int pepc_gpu_adjoint_device(void *ctx, const double *gex, const double *gey, const double *gez,
+                            const double *gpot, double *gx, double *gy, double *gz,
+                            double *gq);
+
 /* Host copies of the state (each may be NULL), current order. */
 int pepc_gpu_state_get(void *ctx, double *x, double *y, double *z, double *vx, double *vy,
                        double *vz, double *q, double *m, int64_t *label);

@@ -79,6 +79,17 @@ def lib():
         L.or_m2p.argtypes = [_f64, _f64, C.c_double, _f64, _f64]
         L.or_direct.argtypes = [C.c_int64, _f64, _f64, _f64, _f64, C.c_int64, _i64,
                                 C.c_double, C.c_double, C.c_int, _f64, _f64, _f64, _f64]
+        L.or_walk_lists.argtypes = [_f64, _f64, _f64, _i32, _i64, _i64, _f64, _f64, C.c_double,
+                                    C.c_int64, _i64, _i64, C.c_int64, C.c_void_p, C.c_void_p]
+        L.or_walk_lists.restype = C.c_int64
+        L.or_tree_moments.argtypes = [C.c_int64, _i64, _i64, _i64, _f64, _f64, _f64, _f64, _f64]
+        L.or_eval_lists.argtypes = [_f64, _f64, _f64, _i64, _i64, _f64, C.c_double, C.c_double,
+                                    C.c_int64, _i64, _i64, _i64, _i64, _f64, _f64, _f64, _f64]
+        L.or_adjoint_term.argtypes = [_f64, C.c_double, C.c_double, _f64, C.c_double, _f64, _f64,
+                                      _f64, _f64, _f64, _f64]
+        L.or_adjoint_lists.argtypes = [C.c_int64, _f64, _f64, _f64, _f64, C.c_int64, _i64, _i64,
+                                       _i64, _f64, C.c_double, C.c_double, C.c_int64, _i64, _i64,
+                                       _i64, _i64, _f64, _f64, _f64, _f64, _f64, _f64, _f64, _f64]
         _lib = L
     return _lib
 
@@ -284,6 +295,72 @@ def direct(x, y, z, q, testidx, eps2, sign=1.0, nthreads=0):
                     _c(q, np.float64), m, t, float(eps2), float(sign), int(nthreads),
                     ex, ey, ez, pot)
     return np.stack([ex, ey, ez], 1), pot
+
+
+# ------------------------------------------------- frozen lists and adjoint
+@dataclass
+class Lists:
+    tile_first: np.ndarray
+    tile_count: np.ndarray
+    off: np.ndarray        # ntiles + 1
+    node: np.ndarray       # entries in traversal order (leaf: P2P, twig: M2P)
+
+
+def walk_lists(x, y, z, tree: Tree, theta2, tiles) -> Lists:
+    """Interaction lists of the group walk (same decisions as walk_group)."""
+    tf, tc = (_c(a, np.int64) for a in tiles)
+    args = (_c(x, np.float64), _c(y, np.float64), _c(z, np.float64), tree.level,
+            tree.first_child, tree.next_sibling, np.ascontiguousarray(tree.mom.ravel()), tree.b2,
+            float(theta2), len(tf), tf, tc)
+    need = lib().or_walk_lists(*args, 0, None, None)
+    need = -need if need < 0 else need
+    off = np.zeros(len(tf) + 1, np.int64)
+    node = np.zeros(max(need, 1), np.int64)
+    got = lib().or_walk_lists(*args, len(node), off.ctypes.data_as(C.c_void_p),
+                              node.ctypes.data_as(C.c_void_p))
+    assert got == need
+    return Lists(tf, tc, off, node[:need])
+
+
+def tree_moments(tree: Tree, x, y, z, q) -> np.ndarray:
+    """Node moments of the same topology at other particle positions/charges."""
+    mom = np.zeros(tree.nnodes * MOM)
+    lib().or_tree_moments(tree.nnodes, tree.first, tree.first_child, tree.next_sibling,
+                          _c(x, np.float64), _c(y, np.float64), _c(z, np.float64),
+                          _c(q, np.float64), mom)
+    return mom.reshape(tree.nnodes, MOM)
+
+
+def eval_lists(x, y, z, tree: Tree, mom, lists: Lists, eps2, sign=1.0):
+    n = len(x)
+    ex, ey, ez, pot = (np.zeros(n) for _ in range(4))
+    lib().or_eval_lists(_c(x, np.float64), _c(y, np.float64), _c(z, np.float64), tree.first,
+                        tree.first_child, np.ascontiguousarray(np.asarray(mom).ravel()),
+                        float(eps2), float(sign), len(lists.tile_first), lists.tile_first,
+                        lists.tile_count, lists.off, lists.node, ex, ey, ez, pot)
+    return np.stack([ex, ey, ez], 1), pot
+
+
+def adjoint_term(d, r2, gphi, u, q, p=(0.0, 0.0, 0.0), qt=(0.0,) * 6):
+    D, gq, gp, M = np.zeros(3), np.zeros(1), np.zeros(3), np.zeros(6)
+    lib().or_adjoint_term(_c(d, np.float64), float(r2), float(gphi), _c(u, np.float64), float(q),
+                          _c(p, np.float64), _c(qt, np.float64), D, gq, gp, M)
+    return D, float(gq[0]), gp, M
+
+
+def adjoint_lists(x, y, z, q, tree: Tree, lists: Lists, eps2, ge, gpot, sign=1.0):
+    """dL/d(x, y, z, q) of L = sum gpot.pot + ge.E over frozen lists."""
+    n = len(x)
+    ge = np.asarray(ge, np.float64)
+    gx, gy, gz, gq = (np.zeros(n) for _ in range(4))
+    lib().or_adjoint_lists(n, _c(x, np.float64), _c(y, np.float64), _c(z, np.float64),
+                           _c(q, np.float64), tree.nnodes, tree.first, tree.leaves,
+                           tree.first_child, np.ascontiguousarray(tree.mom.ravel()), float(eps2),
+                           float(sign), len(lists.tile_first), lists.tile_first, lists.tile_count,
+                           lists.off, lists.node, _c(ge[:, 0], np.float64),
+                           _c(ge[:, 1], np.float64), _c(ge[:, 2], np.float64),
+                           _c(gpot, np.float64), gx, gy, gz, gq)
+    return gx, gy, gz, gq
 
 
 def solve(x, y, z, q, theta2, eps2, tile_size=16, sign=1.0, nthreads=0, force_cubic=False,

@@ -678,3 +678,254 @@ void or_direct(int64_t n, const double *x, const double *y, const double *z,
         ex[j] = sign * e0; ey[j] = sign * e1; ez[j] = sign * e2; pot[j] = sign * ph;
     }
 }
+
+/* ------------------------------------------------------------------------- */
+/* Frozen interaction lists and the adjoint of the walk (no reference         */
+/* counterpart: the reference has no derivatives).  A list holds, per tile,  */
+/* the nodes the group walk interacts with, in traversal order: a leaf is a  */
+/* P2P entry, an accepted twig an M2P entry.  Holding the lists and the tree */
+/* topology fixed, the fields are smooth functions of the particle positions */
+/* and charges; or_eval_lists evaluates them (moments recomputed from the    */
+/* particles), or_adjoint_lists returns their exact gradient.                 */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    const double *x, *y, *z;
+    const int32_t *nlevel; const int64_t *nfc, *nns; const double *mom;
+    const double *b2; double theta2;
+    int64_t t0, np, cnt, cap; int64_t *out;
+} wl_ctx;
+
+/* Same decisions as walk_aux (module_tree_walk_simple.f90:301-384). */
+static void lists_aux(wl_ctx *w, int64_t node)
+{
+    if (w->nfc[node] < 0) {
+        if (w->out && w->cnt < w->cap) w->out[w->cnt] = node;
+        w->cnt++;
+        return;
+    }
+    const double *m = w->mom + OR_MOM * node;
+    for (int64_t ip = 0; ip < w->np; ++ip) {
+        int64_t p = w->t0 + ip;
+        double d0 = w->x[p] - m[OR_COCX], d1 = w->y[p] - m[OR_COCY], d2 = w->z[p] - m[OR_COCZ];
+        double dd = d0 * d0 + d1 * d1 + d2 * d2;
+        if (!(w->theta2 * dd > w->b2[w->nlevel[node]])) {
+            for (int64_t c = w->nfc[node]; c >= 0; c = w->nns[c]) lists_aux(w, c);
+            return;
+        }
+    }
+    if (w->out && w->cnt < w->cap) w->out[w->cnt] = node;
+    w->cnt++;
+}
+
+int64_t or_walk_lists(const double *x, const double *y, const double *z,
+                      const int32_t *nlevel, const int64_t *nfirst_child,
+                      const int64_t *nnext_sibling, const double *nmom,
+                      const double b2[OR_MAXLEVEL + 1], double theta2,
+                      int64_t ntiles, const int64_t *tile_first, const int64_t *tile_count,
+                      int64_t cap, int64_t *list_off, int64_t *list_node)
+{
+    wl_ctx w = {x, y, z, nlevel, nfirst_child, nnext_sibling, nmom, b2, theta2, 0, 0, 0, cap,
+                list_node};
+    for (int64_t t = 0; t < ntiles; ++t) {
+        if (list_off) list_off[t] = w.cnt;
+        w.t0 = tile_first[t];
+        w.np = tile_count[t];
+        lists_aux(&w, 0);
+    }
+    if (list_off) list_off[ntiles] = w.cnt;
+    return w.cnt > cap ? -w.cnt : w.cnt;
+}
+
+/* Moments of every node for a fixed topology (pre-order numbering: a child
+ * has a larger index than its parent), leaves from their particle
+ * (coulomb/module_interaction_specific.f90:67-79), twigs by
+ * shift_multipoles_up. */
+void or_tree_moments(int64_t nnodes, const int64_t *nfirst, const int64_t *nfirst_child,
+                     const int64_t *nnext_sibling, const double *x, const double *y,
+                     const double *z, const double *q, double *nmom)
+{
+    for (int64_t i = nnodes - 1; i >= 0; --i) {
+        double *m = nmom + OR_MOM * i;
+        if (nfirst_child[i] < 0) {
+            int64_t p = nfirst[i];
+            memset(m, 0, sizeof(double) * OR_MOM);
+            m[OR_COCX] = x[p]; m[OR_COCY] = y[p]; m[OR_COCZ] = z[p];
+            m[OR_CHARGE] = q[p]; m[OR_ABSCHARGE] = fabs(q[p]);
+            continue;
+        }
+        const double *cm[8];
+        int nch = 0;
+        for (int64_t c = nfirst_child[i]; c >= 0 && nch < 8; c = nnext_sibling[c]) cm[nch++] = nmom + OR_MOM * c;
+        shift_multipoles_up(m, cm, nch);
+    }
+}
+
+/* Fields from frozen lists.  A P2P entry skips the target's own leaf (by
+ * index; the walk skips d^2 = 0, the same for distinct positions). */
+void or_eval_lists(const double *x, const double *y, const double *z,
+                   const int64_t *nfirst, const int64_t *nfirst_child, const double *nmom,
+                   double eps2, double sign, int64_t ntiles, const int64_t *tile_first,
+                   const int64_t *tile_count, const int64_t *list_off, const int64_t *list_node,
+                   double *ex, double *ey, double *ez, double *pot)
+{
+    for (int64_t t = 0; t < ntiles; ++t) {
+        for (int64_t ip = 0; ip < tile_count[t]; ++ip) {
+            int64_t p = tile_first[t] + ip;
+            double e[3] = {0.0, 0.0, 0.0}, ph = 0.0;
+            for (int64_t k = list_off[t]; k < list_off[t + 1]; ++k) {
+                const int64_t node = list_node[k];
+                const double *m = nmom + OR_MOM * node;
+                double d[3] = {x[p] - m[OR_COCX], y[p] - m[OR_COCY], z[p] - m[OR_COCZ]};
+                double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                double exyz[3], phi;
+                if (nfirst_child[node] < 0) {
+                    if (nfirst[node] == p) continue;
+                    or_p2p(&m[OR_CHARGE], d, d2 + eps2, exyz, &phi);
+                } else {
+                    or_m2p(m, d, d2 + eps2, exyz, &phi);
+                }
+                e[0] += sign * exyz[0]; e[1] += sign * exyz[1]; e[2] += sign * exyz[2];
+                ph += sign * phi;
+            }
+            ex[p] = e[0]; ey[p] = e[1]; ez[p] = e[2]; pot[p] = ph;
+        }
+    }
+}
+
+/* Adjoint of one interaction.  With G = (d^2 + eps^2)^(-1/2) the M2P term is
+ * phi = q G + p.grad G + Qt : grad grad G and E = -grad phi (the polynomial
+ * of module_coulomb_kernels.f90:60-145; Qt = [[qx, xy/2, zx/2], [xy/2, qy,
+ * yz/2], [zx/2, yz/2, qz]]).  For the loss term gphi*phi + u.E it returns
+ *   gq = dL/dq, gp = dL/dp, M = dL/dQt (symmetric, full double sum),
+ *   D  = dL/dd (d = target - centre).
+ * P2P is the case p = Qt = 0. */
+void or_adjoint_term(const double d[3], double r2, double gphi, const double u[3], double q,
+                     const double p[3], const double qt[6], double D[3], double *gq,
+                     double gp[3], double M[6])
+{
+    const double r = sqrt(r2), ri = 1.0 / r;
+    const double r3 = ri * ri * ri, r5 = r3 * ri * ri, r7 = r5 * ri * ri, r9 = r7 * ri * ri;
+    const double ud = u[0] * d[0] + u[1] * d[1] + u[2] * d[2];
+    const double pd = p[0] * d[0] + p[1] * d[1] + p[2] * d[2];
+    const double up = u[0] * p[0] + u[1] * p[1] + u[2] * p[2];
+    /* Qt as a full symmetric matrix */
+    const double Q[3][3] = {{qt[0], 0.5 * qt[3], 0.5 * qt[5]},
+                            {0.5 * qt[3], qt[1], 0.5 * qt[4]},
+                            {0.5 * qt[5], 0.5 * qt[4], qt[2]}};
+    const double tr = qt[0] + qt[1] + qt[2];
+    double Qd[3], Qu[3];
+    for (int k = 0; k < 3; ++k) {
+        Qd[k] = Q[k][0] * d[0] + Q[k][1] * d[1] + Q[k][2] * d[2];
+        Qu[k] = Q[k][0] * u[0] + Q[k][1] * u[1] + Q[k][2] * u[2];
+    }
+    const double dQd = d[0] * Qd[0] + d[1] * Qd[1] + d[2] * Qd[2];
+    const double uQd = u[0] * Qd[0] + u[1] * Qd[1] + u[2] * Qd[2];
+    *gq = gphi * ri + ud * r3;
+    for (int k = 0; k < 3; ++k) gp[k] = (u[k] - gphi * d[k]) * r3 - 3.0 * ud * d[k] * r5;
+    static const int mi[6] = {0, 1, 2, 0, 1, 2}, ni[6] = {0, 1, 2, 1, 2, 0};
+    for (int k = 0; k < 6; ++k) {
+        const int a = mi[k], b = ni[k];
+        const double dab = a == b ? 1.0 : 0.0;
+        M[k] = gphi * (-dab * r3 + 3.0 * d[a] * d[b] * r5)
+               - (3.0 * r5 * (u[a] * d[b] + u[b] * d[a] + dab * ud) - 15.0 * r7 * ud * d[a] * d[b]);
+    }
+    for (int k = 0; k < 3; ++k) {
+        double v = q * gp[k];
+        v += gphi * (-p[k] * r3 + 3.0 * d[k] * pd * r5)
+             - (3.0 * r5 * (u[k] * pd + p[k] * ud + d[k] * up) - 15.0 * r7 * d[k] * ud * pd);
+        v += gphi * (3.0 * r5 * (2.0 * Qd[k] + tr * d[k]) - 15.0 * r7 * d[k] * dQd)
+             - (3.0 * r5 * (u[k] * tr + 2.0 * Qu[k])
+                - 15.0 * r7 * (u[k] * dQd + 2.0 * ud * Qd[k] + 2.0 * d[k] * uQd + d[k] * ud * tr)
+                + 105.0 * r9 * d[k] * ud * dQd);
+        D[k] = v;
+    }
+}
+
+/* Gradient of L = sum_i gpot_i pot_i + gE_i . E_i (the or_eval_lists
+ * outputs, sign included) with respect to the sorted particles' x, y, z, q.
+ * Node moments enter through their closed forms over the node's particles
+ * (q = sum q_j, p = sum q_j s_j, Qt = 1/2 sum q_j s_j s_j^T, s_j = c - x_j,
+ * c = sum |q_j| x_j / sum |q_j|); a node whose charges are all zero keeps its
+ * centre fixed. */
+void or_adjoint_lists(int64_t n, const double *x, const double *y, const double *z,
+                      const double *q, int64_t nnodes, const int64_t *nfirst,
+                      const int64_t *nleaves, const int64_t *nfirst_child, const double *nmom,
+                      double eps2, double sign, int64_t ntiles, const int64_t *tile_first,
+                      const int64_t *tile_count, const int64_t *list_off,
+                      const int64_t *list_node, const double *gex, const double *gey,
+                      const double *gez, const double *gpot, double *gx, double *gy, double *gz,
+                      double *gq)
+{
+    double *adj = (double *)calloc((size_t)nnodes * 16, sizeof(double));
+    for (int64_t i = 0; i < n; ++i) gx[i] = gy[i] = gz[i] = gq[i] = 0.0;
+    static const double zero3[3] = {0.0, 0.0, 0.0}, zero6[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t t = 0; t < ntiles; ++t) {
+        for (int64_t ip = 0; ip < tile_count[t]; ++ip) {
+            const int64_t p = tile_first[t] + ip;
+            const double gphi = sign * gpot[p];
+            const double u[3] = {sign * gex[p], sign * gey[p], sign * gez[p]};
+            for (int64_t k = list_off[t]; k < list_off[t + 1]; ++k) {
+                const int64_t node = list_node[k];
+                const double *m = nmom + OR_MOM * node;
+                double d[3] = {x[p] - m[OR_COCX], y[p] - m[OR_COCY], z[p] - m[OR_COCZ]};
+                double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + eps2;
+                double D[3], g, gp[3], M[6];
+                if (nfirst_child[node] < 0) {
+                    const int64_t s = nfirst[node];
+                    if (s == p) continue;
+                    or_adjoint_term(d, r2, gphi, u, q[s], zero3, zero6, D, &g, gp, M);
+                    gx[s] -= D[0]; gy[s] -= D[1]; gz[s] -= D[2];
+                    gq[s] += g;
+                } else {
+                    const double pm[3] = {m[OR_DIPX], m[OR_DIPY], m[OR_DIPZ]};
+                    const double qt[6] = {m[OR_QUADX], m[OR_QUADY], m[OR_QUADZ],
+                                          m[OR_XYQUAD], m[OR_YZQUAD], m[OR_ZXQUAD]};
+                    or_adjoint_term(d, r2, gphi, u, m[OR_CHARGE], pm, qt, D, &g, gp, M);
+                    double *a = adj + 16 * node;
+                    a[0] += g;
+                    for (int j = 0; j < 3; ++j) a[1 + j] += gp[j];
+                    for (int j = 0; j < 6; ++j) a[4 + j] += M[j];
+                    for (int j = 0; j < 3; ++j) a[10 + j] += D[j];
+                }
+                gx[p] += D[0]; gy[p] += D[1]; gz[p] += D[2];
+            }
+        }
+    }
+    /* node centre adjoint: dL/dc = q gp + M p - sum D */
+    for (int64_t i = 0; i < nnodes; ++i) {
+        if (nfirst_child[i] < 0) continue;
+        double *a = adj + 16 * i;
+        const double *m = nmom + OR_MOM * i;
+        const double Mf[3][3] = {{a[4], a[7], a[9]}, {a[7], a[5], a[8]}, {a[9], a[8], a[6]}};
+        for (int k = 0; k < 3; ++k) {
+            double mp = Mf[k][0] * m[OR_DIPX] + Mf[k][1] * m[OR_DIPY] + Mf[k][2] * m[OR_DIPZ];
+            a[13 + k] = m[OR_CHARGE] * a[1 + k] + mp - a[10 + k];
+        }
+    }
+    /* every twig to its particles */
+    for (int64_t i = 0; i < nnodes; ++i) {
+        if (nfirst_child[i] < 0) continue;
+        const double *a = adj + 16 * i;
+        const double *m = nmom + OR_MOM * i;
+        const double A = m[OR_ABSCHARGE];
+        const double Mf[3][3] = {{a[4], a[7], a[9]}, {a[7], a[5], a[8]}, {a[9], a[8], a[6]}};
+        for (int64_t j = nfirst[i]; j < nfirst[i] + nleaves[i]; ++j) {
+            const double xj[3] = {x[j], y[j], z[j]};
+            double s[3], Ms[3];
+            for (int k = 0; k < 3; ++k) s[k] = m[OR_COCX + k] - xj[k];
+            for (int k = 0; k < 3; ++k) Ms[k] = Mf[k][0] * s[0] + Mf[k][1] * s[1] + Mf[k][2] * s[2];
+            double dq = a[0] + a[1] * s[0] + a[2] * s[1] + a[3] * s[2]
+                        + 0.5 * (s[0] * Ms[0] + s[1] * Ms[1] + s[2] * Ms[2]);
+            double dx[3];
+            for (int k = 0; k < 3; ++k) dx[k] = -q[j] * (a[1 + k] + Ms[k]);
+            if (A != 0.0) {
+                const double sg = q[j] > 0.0 ? 1.0 : (q[j] < 0.0 ? -1.0 : 0.0);
+                dq += sg / A * (-(s[0] * a[13]) - s[1] * a[14] - s[2] * a[15]);
+                for (int k = 0; k < 3; ++k) dx[k] += fabs(q[j]) / A * a[13 + k];
+            }
+            gq[j] += dq;
+            gx[j] += dx[0]; gy[j] += dx[1]; gz[j] += dx[2];
+        }
+    }
+    free(adj);
+}

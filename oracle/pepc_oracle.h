@@ -28,6 +28,11 @@
  *   or_p2p / or_m2p         closed-form Plummer pair; M2P with zero dip/quad
  *                           equals P2P; finite-difference E = -grad(phi)
  *   or_direct               textbook pair sum (numpy float64)
+ *   or_walk_lists           list evaluation equals or_walk_group bit for bit
+ *   or_tree_moments         equals or_tree_build's moments at the same input
+ *   or_adjoint_term/_lists  central finite differences of or_eval_lists with
+ *                           frozen lists (h = 1e-5); at theta = 0 the
+ *                           closed-form gradient of the pair sum (numpy)
  */
 #ifndef PEPC_ORACLE_H
 #define PEPC_ORACLE_H
@@ -99,6 +104,33 @@ int or_walk_group(int64_t n, const double *x, const double *y, const double *z,
                   double *ex, double *ey, double *ez, double *pot, double *work,
                   int32_t *cnt_p2p, int32_t *cnt_m2p, int64_t *tile_mac,
                   int64_t *totals);
+
+/* ---- frozen lists and the adjoint (no reference counterpart) --------------- */
+int64_t or_walk_lists(const double *x, const double *y, const double *z,
+                      const int32_t *nlevel, const int64_t *nfirst_child,
+                      const int64_t *nnext_sibling, const double *nmom,
+                      const double b2[OR_MAXLEVEL + 1], double theta2,
+                      int64_t ntiles, const int64_t *tile_first, const int64_t *tile_count,
+                      int64_t cap, int64_t *list_off, int64_t *list_node);
+void or_tree_moments(int64_t nnodes, const int64_t *nfirst, const int64_t *nfirst_child,
+                     const int64_t *nnext_sibling, const double *x, const double *y,
+                     const double *z, const double *q, double *nmom);
+void or_eval_lists(const double *x, const double *y, const double *z,
+                   const int64_t *nfirst, const int64_t *nfirst_child, const double *nmom,
+                   double eps2, double sign, int64_t ntiles, const int64_t *tile_first,
+                   const int64_t *tile_count, const int64_t *list_off, const int64_t *list_node,
+                   double *ex, double *ey, double *ez, double *pot);
+void or_adjoint_term(const double d[3], double r2, double gphi, const double u[3], double q,
+                     const double p[3], const double qt[6], double D[3], double *gq,
+                     double gp[3], double M[6]);
+void or_adjoint_lists(int64_t n, const double *x, const double *y, const double *z,
+                      const double *q, int64_t nnodes, const int64_t *nfirst,
+                      const int64_t *nleaves, const int64_t *nfirst_child, const double *nmom,
+                      double eps2, double sign, int64_t ntiles, const int64_t *tile_first,
+                      const int64_t *tile_count, const int64_t *list_off,
+                      const int64_t *list_node, const double *gex, const double *gey,
+                      const double *gez, const double *gpot, double *gx, double *gy, double *gz,
+                      double *gq);
 
 /* ---- kernels and direct sum ------------------------------------------------ */
 void or_p2p(const double *t_charge, const double d[3], double dist2,

@@ -26,6 +26,7 @@ EXPORTS = [
     "pepc_gpu_read_sorted", "pepc_gpu_read_tree", "pepc_gpu_read_walk", "pepc_gpu_direct_device",
     "pepc_gpu_timings", "pepc_gpu_grow_batch_device", "pepc_gpu_state_set", "pepc_gpu_run",
     "pepc_gpu_state_get", "pepc_gpu_state_device", "pepc_gpu_read_energies",
+    "pepc_gpu_adjoint_device",
 ]
 
 RUN_GRAPH = 1
@@ -99,6 +100,7 @@ def load(path: str = LIB_PATH):
     L.pepc_gpu_state_get.argtypes = [_vp] * 10
     L.pepc_gpu_state_device.argtypes = [_vp, C.POINTER(_vp), C.POINTER(_vp)]
     L.pepc_gpu_read_energies.argtypes = [_vp, _i64, _vp, C.POINTER(_i64)]
+    L.pepc_gpu_adjoint_device.argtypes = [_vp] * 9
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is not None and name != "pepc_gpu_config_default":

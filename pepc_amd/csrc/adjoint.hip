@@ -1,0 +1,331 @@
+// adjoint.hip -- gradient of the tree solve with respect to particle
+// positions and charges (no reference counterpart: PEPC computes no
+// derivatives).
+//
+// With the tree topology and the interaction lists held fixed ("frozen"),
+// the fields are smooth functions of the particles.  For a loss
+// L = sum_i gpot_i pot_i + gE_i . E_i, every interaction of target i with a
+// node (centre c, moments q, p, Qt) contributes, with d = x_i - c:
+//   dL/dd (to the target, and minus it to the centre),
+//   dL/dq, dL/dp, dL/dQt (to the node's moments),
+// where phi = q G + p.grad G + Qt : grad grad G, E = -grad phi and
+// G = (d^2 + eps^2)^(-1/2) (the polynomial of module_coulomb_kernels.f90:
+// 60-145).  A leaf (P2P) is the case p = Qt = 0 with the source particle in
+// place of the node.  Node moments are closed forms over the node's
+// particles (q = sum q_j, p = sum q_j s_j, Qt = 1/2 sum q_j s_j s_j^T,
+// s_j = c - x_j, c = sum |q_j| x_j / sum |q_j|), which is what the upward pass
+// computes (shift_multipoles_up, coulomb/module_interaction_specific.f90:
+// 84-140); the node adjoints therefore go to each particle below the node
+// directly.
+//
+// Kernels: (1) a group walk with the reference acceptance rule (a node is
+// accepted when every target of the group passes the test, as
+// module_tree_walk_simple.f90:301-384) that evaluates the per-interaction
+// derivatives for the group's targets, reduces them across the wavefront
+// and adds them to the node and source-particle adjoints with FP64 atomics;
+// (2) the centre adjoint of every twig; (3) one thread per particle climbing
+// its ancestors.  FP64 throughout.
+#include "kernels.h"
+
+namespace pepc {
+
+namespace {
+
+constexpr int kT = 256;
+constexpr int kStackCap = 512;
+constexpr int kStackSlack = 192;   // >= 7 * 21 + 8, as the forward walk
+
+__device__ __forceinline__ bool broken(const DevScalars* sc) {
+  return (sc->flags & (kFlagNodeOverflow | kFlagDupKeys | kFlagBadBox)) != 0u;
+}
+
+// Derivatives of gphi * phi + u . E for one interaction (see header comment).
+// qt = {qx, qy, qz, xy, yz, zx} (Qt_xx = qx, Qt_xy = xy / 2, ...).  Outputs:
+// D = dL/dd, gq, gp, M = dL/dQt as the symmetric matrix {xx, yy, zz, xy, yz, zx}.
+__device__ void adjoint_term(const double d[3], double r2, double gphi, const double u[3],
+                             double q, const double p[3], const double qt[6], double D[3],
+                             double& gq, double gp[3], double M[6]) {
+  const double ri = rsqrt(r2);
+  const double ri2 = ri * ri;
+  const double r3 = ri * ri2, r5 = r3 * ri2, r7 = r5 * ri2, r9 = r7 * ri2;
+  const double ud = u[0] * d[0] + u[1] * d[1] + u[2] * d[2];
+  const double pd = p[0] * d[0] + p[1] * d[1] + p[2] * d[2];
+  const double up = u[0] * p[0] + u[1] * p[1] + u[2] * p[2];
+  const double hxy = 0.5 * qt[3], hyz = 0.5 * qt[4], hzx = 0.5 * qt[5];
+  const double Qd[3] = {qt[0] * d[0] + hxy * d[1] + hzx * d[2],
+                        hxy * d[0] + qt[1] * d[1] + hyz * d[2],
+                        hzx * d[0] + hyz * d[1] + qt[2] * d[2]};
+  const double Qu[3] = {qt[0] * u[0] + hxy * u[1] + hzx * u[2],
+                        hxy * u[0] + qt[1] * u[1] + hyz * u[2],
+                        hzx * u[0] + hyz * u[1] + qt[2] * u[2]};
+  const double tr = qt[0] + qt[1] + qt[2];
+  const double dQd = d[0] * Qd[0] + d[1] * Qd[1] + d[2] * Qd[2];
+  const double uQd = u[0] * Qd[0] + u[1] * Qd[1] + u[2] * Qd[2];
+  gq = gphi * ri + ud * r3;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) gp[k] = (u[k] - gphi * d[k]) * r3 - 3.0 * ud * d[k] * r5;
+  const int ma[6] = {0, 1, 2, 0, 1, 2}, mb[6] = {0, 1, 2, 1, 2, 0};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int a = ma[k], b = mb[k];
+    const double dab = a == b ? 1.0 : 0.0;
+    M[k] = gphi * (-dab * r3 + 3.0 * d[a] * d[b] * r5) -
+           (3.0 * r5 * (u[a] * d[b] + u[b] * d[a] + dab * ud) - 15.0 * r7 * ud * d[a] * d[b]);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double v = q * gp[k];
+    v += gphi * (-p[k] * r3 + 3.0 * d[k] * pd * r5) -
+         (3.0 * r5 * (u[k] * pd + p[k] * ud + d[k] * up) - 15.0 * r7 * d[k] * ud * pd);
+    v += gphi * (3.0 * r5 * (2.0 * Qd[k] + tr * d[k]) - 15.0 * r7 * d[k] * dQd) -
+         (3.0 * r5 * (u[k] * tr + 2.0 * Qu[k]) -
+          15.0 * r7 * (u[k] * dQd + 2.0 * ud * Qd[k] + 2.0 * d[k] * uQd + d[k] * ud * tr) +
+          105.0 * r9 * d[k] * ud * dQd);
+    D[k] = v;
+  }
+}
+
+__device__ __forceinline__ double wsum(double v) { return wave_sum(v); }
+
+// The adjoint buffers are plain device allocations (coarse grained), where the
+// hardware FP64 add atomic is exact.
+
+__global__ void adjoint_clear_kernel(int64_t n, int64_t cap, double* __restrict__ gx,
+                                     double* __restrict__ gy, double* __restrict__ gz,
+                                     double* __restrict__ gq, double* __restrict__ nadj,
+                                     uint32_t* __restrict__ queue, const DevScalars* __restrict__ sc) {
+  const int64_t nn = (int64_t)sc->nnodes < cap ? (int64_t)sc->nnodes : cap;
+  const int64_t m = broken(sc) ? 0 : nn * 16;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *queue = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n || i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < n) { gx[i] = 0.0; gy[i] = 0.0; gz[i] = 0.0; gq[i] = 0.0; }
+    if (i < m) nadj[i] = 0.0;
+  }
+}
+
+struct AdjArgs {
+  const double *x, *y, *z, *q;
+  const NodeGeom* geom;
+  const NodeLink* link;
+  const double* mom64;
+  const uint32_t *gfirst, *gcount;
+  const uint32_t *sys, *sysroot;
+  DevScalars* sc;
+  double theta2, eps2, sign;
+  const double *gex, *gey, *gez, *gpot;
+  double *gx, *gy, *gz, *gq;
+  double* nadj;
+  uint32_t* queue;
+};
+
+__global__ __launch_bounds__(kWave) void adjoint_walk_kernel(AdjArgs a) {
+  __shared__ uint32_t stack[kStackCap + kStackSlack];
+  __shared__ uint32_t gsh;
+  const int lane = threadIdx.x;
+  const uint32_t ngroups = a.sc->ngroups;
+  const double theta2 = a.theta2, eps2 = a.eps2;
+  for (;;) {
+    if (lane == 0) gsh = atomicAdd(a.queue, 1u);
+    __syncthreads();
+    const uint32_t g = gsh;
+    __syncthreads();
+    if (g >= ngroups) break;
+    const uint32_t a0 = a.gfirst[g], cnt = a.gcount[g];
+    const bool valid = (uint32_t)lane < cnt;
+    const uint32_t tid = a0 + (valid ? (uint32_t)lane : 0u);
+    const double xt = a.x[tid], yt = a.y[tid], zt = a.z[tid];
+    const double gphi = valid ? a.sign * a.gpot[tid] : 0.0;
+    const double u[3] = {valid ? a.sign * a.gex[tid] : 0.0, valid ? a.sign * a.gey[tid] : 0.0,
+                         valid ? a.sign * a.gez[tid] : 0.0};
+    double tD[3] = {0.0, 0.0, 0.0};
+    uint32_t sp = 1;
+    bool overflow = false;
+    if (lane == 0) stack[0] = a.sys ? a.sysroot[a.sys[a0]] : 0u;
+    __syncthreads();
+    while (sp > 0) {
+      int room = ((int)kStackCap - (int)sp) / 7;
+      room = room < 1 ? 1 : room;
+      uint32_t m = sp < (uint32_t)room ? sp : (uint32_t)room;
+      m = m > (uint32_t)kWave ? (uint32_t)kWave : m;
+      const bool has = (uint32_t)lane < m;
+      const uint32_t mynode = has ? stack[sp - 1 - lane] : 0u;
+      NodeLink mylk = {0u, 0u, 0u, 0u};
+      NodeGeom mygm = {0.0, 0.0, 0.0, 0.0};
+      if (has) { mylk = a.link[mynode]; mygm = a.geom[mynode]; }
+      __syncthreads();
+      sp -= m;
+      for (uint32_t k = 0; k < m; ++k) {
+        const uint32_t node = __shfl(mynode, (int)k);
+        const uint32_t first = __shfl(mylk.first, (int)k);
+        const uint32_t nchild = __shfl(mylk.nchild, (int)k);
+        const double cx = __shfl(mygm.cx, (int)k), cy = __shfl(mygm.cy, (int)k),
+                     cz = __shfl(mygm.cz, (int)k), b2 = __shfl(mygm.b2, (int)k);
+        if (nchild == 0u) {   // leaf: P2P with source particle `first`
+          const double q = a.q[first];
+          const bool self = first == tid || !valid;
+          double d[3] = {xt - cx, yt - cy, zt - cz};
+          const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + eps2;
+          double D[3], gq, gp[3], M[6];
+          const double p0[3] = {0.0, 0.0, 0.0}, q0[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+          adjoint_term(d, self ? 1.0 : r2, gphi, u, q, p0, q0, D, gq, gp, M);
+          if (self) { D[0] = D[1] = D[2] = 0.0; gq = 0.0; }
+          tD[0] += D[0]; tD[1] += D[1]; tD[2] += D[2];
+          const double s0 = wsum(D[0]), s1 = wsum(D[1]), s2 = wsum(D[2]), s3 = wsum(gq);
+          if (lane == 0) {
+            unsafeAtomicAdd(&a.gx[first], -s0);
+            unsafeAtomicAdd(&a.gy[first], -s1);
+            unsafeAtomicAdd(&a.gz[first], -s2);
+            unsafeAtomicAdd(&a.gq[first], s3);
+          }
+          continue;
+        }
+        // twig: accepted iff every target passes theta2 d^2 > b^2(level)
+        bool fail;
+        {
+#pragma clang fp contract(off)
+          const double dx = xt - cx, dy = yt - cy, dz = zt - cz;
+          const double d2 = dx * dx + dy * dy + dz * dz;
+          fail = valid && !(theta2 * d2 > b2);
+        }
+        if (__ballot(fail) != 0ull) {
+          if (sp + nchild > (uint32_t)(kStackCap + kStackSlack)) { overflow = true; break; }
+          if ((uint32_t)lane < nchild) stack[sp + lane] = first + (uint32_t)lane;
+          sp += nchild;
+          continue;
+        }
+        const double* mm = a.mom64 + (size_t)node * 16;
+        const double q = mm[3];
+        const double p[3] = {mm[5], mm[6], mm[7]};
+        const double qt[6] = {mm[8], mm[9], mm[10], mm[11], mm[12], mm[13]};
+        double d[3] = {xt - cx, yt - cy, zt - cz};
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + eps2;
+        double D[3], gq, gp[3], M[6];
+        adjoint_term(d, r2, gphi, u, q, p, qt, D, gq, gp, M);
+        if (!valid) {
+          D[0] = D[1] = D[2] = 0.0; gq = 0.0; gp[0] = gp[1] = gp[2] = 0.0;
+#pragma unroll
+          for (int j = 0; j < 6; ++j) M[j] = 0.0;
+        }
+        tD[0] += D[0]; tD[1] += D[1]; tD[2] += D[2];
+        double v[13] = {gq, gp[0], gp[1], gp[2], M[0], M[1], M[2], M[3], M[4], M[5], D[0], D[1], D[2]};
+#pragma unroll
+        for (int j = 0; j < 13; ++j) v[j] = wsum(v[j]);
+        if (lane == 0) {
+          double* o = a.nadj + (size_t)node * 16;
+#pragma unroll
+          for (int j = 0; j < 13; ++j) unsafeAtomicAdd(&o[j], v[j]);
+        }
+      }
+      if (overflow) break;
+      __syncthreads();
+    }
+    if (overflow && lane == 0) atomicOr(&a.sc->flags, kFlagStackOverflow);
+    if (valid) {
+      unsafeAtomicAdd(&a.gx[tid], tD[0]);
+      unsafeAtomicAdd(&a.gy[tid], tD[1]);
+      unsafeAtomicAdd(&a.gz[tid], tD[2]);
+    }
+    __syncthreads();
+  }
+}
+
+// Centre adjoint of every twig: dL/dc = q gp + M p - sum D (slots 13..15),
+// and parent links for the particle pass.
+__global__ void adjoint_nodes_kernel(const NodeLink* __restrict__ link, const double* __restrict__ mom64,
+                                     double* __restrict__ nadj, uint32_t* __restrict__ parent,
+                                     uint32_t* __restrict__ leafof, const DevScalars* __restrict__ sc) {
+  if (broken(sc)) return;
+  const int64_t m = sc->nnodes;
+  const int64_t nroot = sc->level_offset[1];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const NodeLink lk = link[i];
+    if (i < nroot) parent[i] = 0xFFFFFFFFu;
+    if (lk.nchild == 0u) { leafof[lk.first] = (uint32_t)i; continue; }
+    for (uint32_t c = 0; c < lk.nchild; ++c) parent[lk.first + c] = (uint32_t)i;
+    double* o = nadj + (size_t)i * 16;
+    const double* mm = mom64 + (size_t)i * 16;
+    const double Mxx = o[4], Myy = o[5], Mzz = o[6], Mxy = o[7], Myz = o[8], Mzx = o[9];
+    const double px = mm[5], py = mm[6], pz = mm[7], q = mm[3];
+    o[13] = q * o[1] + (Mxx * px + Mxy * py + Mzx * pz) - o[10];
+    o[14] = q * o[2] + (Mxy * px + Myy * py + Myz * pz) - o[11];
+    o[15] = q * o[3] + (Mzx * px + Myz * py + Mzz * pz) - o[12];
+  }
+}
+
+// Each particle collects the adjoints of its twig ancestors.
+__global__ void adjoint_particles_kernel(int64_t n, const double* __restrict__ x,
+                                         const double* __restrict__ y, const double* __restrict__ z,
+                                         const double* __restrict__ q,
+                                         const double* __restrict__ mom64,
+                                         const double* __restrict__ nadj,
+                                         const uint32_t* __restrict__ parent,
+                                         const uint32_t* __restrict__ leafof,
+                                         double* __restrict__ gx, double* __restrict__ gy,
+                                         double* __restrict__ gz, double* __restrict__ gq,
+                                         const DevScalars* __restrict__ sc) {
+  if (broken(sc)) return;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const double xj[3] = {x[j], y[j], z[j]};
+    const double qj = q[j];
+    const double sg = qj > 0.0 ? 1.0 : (qj < 0.0 ? -1.0 : 0.0);
+    double dq = 0.0, dx[3] = {0.0, 0.0, 0.0};
+    uint32_t node = parent[leafof[j]];
+    for (int guard = 0; node != 0xFFFFFFFFu && guard <= kMaxLevel; ++guard) {
+      const double* o = nadj + (size_t)node * 16;
+      const double* mm = mom64 + (size_t)node * 16;
+      const double s[3] = {mm[0] - xj[0], mm[1] - xj[1], mm[2] - xj[2]};
+      const double Ms[3] = {o[4] * s[0] + o[7] * s[1] + o[9] * s[2],
+                            o[7] * s[0] + o[5] * s[1] + o[8] * s[2],
+                            o[9] * s[0] + o[8] * s[1] + o[6] * s[2]};
+      dq += o[0] + o[1] * s[0] + o[2] * s[1] + o[3] * s[2] +
+            0.5 * (s[0] * Ms[0] + s[1] * Ms[1] + s[2] * Ms[2]);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) dx[k] += -qj * (o[1 + k] + Ms[k]);
+      const double A = mm[4];
+      if (A != 0.0) {
+        dq += sg / A * -(s[0] * o[13] + s[1] * o[14] + s[2] * o[15]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) dx[k] += fabs(qj) / A * o[13 + k];
+      }
+      node = parent[node];
+    }
+    gq[j] += dq;
+    gx[j] += dx[0];
+    gy[j] += dx[1];
+    gz[j] += dx[2];
+  }
+}
+
+unsigned grid_for(int64_t n) {
+  int64_t b = (n + kT - 1) / kT;
+  if (b < 1) b = 1;
+  if (b > 8192) b = 8192;
+  return (unsigned)b;
+}
+
+}  // namespace
+
+void launch_adjoint(hipStream_t s, const AdjointArgs& p, int blocks) {
+  const int64_t n = p.n;
+  adjoint_clear_kernel<<<grid_for(p.cap * 16 > n ? p.cap * 16 : n), kT, 0, s>>>(
+      n, p.cap, p.gx, p.gy, p.gz, p.gq, p.nadj, p.queue, p.sc);
+  AdjArgs a;
+  a.x = p.x; a.y = p.y; a.z = p.z; a.q = p.q;
+  a.geom = p.geom; a.link = p.link; a.mom64 = p.mom64;
+  a.gfirst = p.gfirst; a.gcount = p.gcount; a.sys = p.sys; a.sysroot = p.sysroot;
+  a.sc = p.sc; a.theta2 = p.theta2; a.eps2 = p.eps2; a.sign = p.sign;
+  a.gex = p.gex; a.gey = p.gey; a.gez = p.gez; a.gpot = p.gpot;
+  a.gx = p.gx; a.gy = p.gy; a.gz = p.gz; a.gq = p.gq;
+  a.nadj = p.nadj; a.queue = p.queue;
+  adjoint_walk_kernel<<<blocks, kWave, 0, s>>>(a);
+  adjoint_nodes_kernel<<<grid_for(p.cap), kT, 0, s>>>(p.link, p.mom64, p.nadj, p.parent, p.leafof,
+                                                      p.sc);
+  adjoint_particles_kernel<<<grid_for(n), kT, 0, s>>>(n, p.x, p.y, p.z, p.q, p.mom64, p.nadj,
+                                                      p.parent, p.leafof, p.gx, p.gy, p.gz, p.gq,
+                                                      p.sc);
+}
+
+}  // namespace pepc

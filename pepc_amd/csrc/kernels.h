@@ -130,6 +130,24 @@ void launch_energy(hipStream_t s, int nsys, const int64_t* soff, int64_t n, cons
                    const StepState& st, double* hist, int64_t cap, uint32_t* counter);
 void launch_counter_set(hipStream_t s, uint32_t* counter, uint32_t v);
 
+// adjoint.hip: gradient of the frozen-list solve (sorted order in and out)
+struct AdjointArgs {
+  int64_t n, cap;
+  const double *x, *y, *z, *q;
+  const NodeGeom* geom;
+  const NodeLink* link;
+  const double* mom64;
+  const uint32_t *gfirst, *gcount;
+  const uint32_t *sys, *sysroot;
+  DevScalars* sc;
+  double theta2, eps2, sign;
+  const double *gex, *gey, *gez, *gpot;   // dL/dE, dL/dpot
+  double *gx, *gy, *gz, *gq;              // dL/dx, dL/dq
+  double* nadj;                           // 16 doubles per node
+  uint32_t *parent, *leafof, *queue;
+};
+void launch_adjoint(hipStream_t s, const AdjointArgs& a, int blocks);
+
 // walk.hip: FP64 direct sum
 void launch_direct(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
                    const double* q, int64_t ntest, const int64_t* testidx, double eps2, double sign,

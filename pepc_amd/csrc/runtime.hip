@@ -76,6 +76,10 @@ struct Ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   double graph_dt = 0.0;
+  // adjoint
+  double* nadj = nullptr;
+  uint32_t *parent = nullptr, *leafof = nullptr, *aqueue = nullptr;
+  int64_t adj_mcap = 0, adj_ncap = 0;
 };
 
 int fail(Ctx* c, int code, const std::string& msg) {
@@ -123,7 +127,8 @@ void free_all(Ctx* c) {
                   c->res[2], c->res[3], c->res[4], c->dkey[0], c->dkey[1], c->dval[0], c->dval[1],
                   c->dpart, c->inv, c->flag, c->fc_rank, c->node_rank, c->nfirst, c->link, c->geom,
                   c->mom64, c->wm32, c->wm64, c->hist, c->bsum, c->partial, c->box, c->sc,
-                  c->sys, c->soff, c->sysroot, c->label, c->tlabel, c->ehist, c->ecount};
+                  c->sys, c->soff, c->sysroot, c->label, c->tlabel, c->ehist, c->ecount,
+                  c->nadj, c->parent, c->leafof, c->aqueue};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (double* p : c->st)
@@ -823,6 +828,39 @@ int pepc_gpu_state_device(void* p, double** arrays, const int64_t** label) {
   if (!c->has_state) return fail(c, PEPC_GPU_EINVAL, "state_device: no particle state");
   for (int k = 0; k < 11; ++k) arrays[k] = c->st[k];
   if (label) *label = reinterpret_cast<const int64_t*>(c->label);
+  return 0;
+}
+
+int pepc_gpu_adjoint_device(void* p, const double* gex, const double* gey, const double* gez,
+                            const double* gpot, double* gx, double* gy, double* gz, double* gq) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PEPC_GPU_EINVAL;
+  if (!gex || !gey || !gez || !gpot || !gx || !gy || !gz || !gq)
+    return fail(c, PEPC_GPU_EINVAL, "adjoint: null argument");
+  if (!c->grown) return fail(c, PEPC_GPU_EINVAL, "adjoint: no tree (call grow first)");
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  if (c->adj_mcap != c->mcap || c->adj_ncap != c->ncap) {
+    ALLOC(c, c->nadj, c->mcap * 16);
+    ALLOC(c, c->parent, c->mcap);
+    ALLOC(c, c->leafof, c->ncap);
+    ALLOC(c, c->aqueue, 64);
+    c->adj_mcap = c->mcap;
+    c->adj_ncap = c->ncap;
+  }
+  AdjointArgs a;
+  a.n = c->n; a.cap = c->mcap;
+  a.x = c->xs; a.y = c->ys; a.z = c->zs; a.q = c->qs;
+  a.geom = c->geom; a.link = c->link; a.mom64 = c->mom64;
+  a.gfirst = c->gfirst; a.gcount = c->gcount;
+  a.sys = c->nsys > 1 ? c->sys : nullptr;
+  a.sysroot = c->sysroot;
+  a.sc = c->sc;
+  a.theta2 = c->cfg.theta2; a.eps2 = c->cfg.eps2; a.sign = c->cfg.sign;
+  a.gex = gex; a.gey = gey; a.gez = gez; a.gpot = gpot;
+  a.gx = gx; a.gy = gy; a.gz = gz; a.gq = gq;
+  a.nadj = c->nadj; a.parent = c->parent; a.leafof = c->leafof; a.queue = c->aqueue;
+  launch_adjoint(c->stream, a, c->walk_blocks);
+  HIPCHECK(c, hipGetLastError());
   return 0;
 }
 

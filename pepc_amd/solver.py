@@ -137,6 +137,25 @@ class Solver:
         self._check(self.lib.pepc_gpu_timings(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def adjoint(self, ge, gpot, sync=True):
+        """dL/dx, dL/dy, dL/dz, dL/dq (sorted order, float64 device tensors)
+        for upstream gradients ge = dL/dE (3, n) and gpot = dL/dpot (n) of the
+        last grow, sorted order; tree and interaction lists held fixed."""
+        t = self.torch
+        n = self.n
+        ge = self._dev(ge)
+        gp = self._dev(gpot)
+        if ge.shape != (3, n) or gp.shape != (n,):
+            raise ValueError("adjoint: ge must be (3, n) and gpot (n,)")
+        out = [t.empty(n, dtype=t.float64, device=self.device) for _ in range(4)]
+        self._bind_stream()
+        self._check(self.lib.pepc_gpu_adjoint_device(self.h, _ptr(ge[0]), _ptr(ge[1]), _ptr(ge[2]),
+                                                     _ptr(gp), *[_ptr(o) for o in out]))
+        self._keep = (ge, gp)
+        if sync:
+            self.sync()
+        return out
+
     # -------------------------------------------------------------- timestep
     def state_set(self, x, y, z, v, q, m, offsets=None, max_records=1024):
         """Loads particle state (host arrays, input order) for the on-device
