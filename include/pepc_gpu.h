@@ -64,7 +64,8 @@ typedef struct {
     int32_t group_pack;     /* 0 = reference tiles; 1..64 = merge runs of sibling tiles
                                up to this many particles per group (no reference
                                counterpart; fuller wavefronts) */
-    int32_t reserved;
+    int32_t list_tile;      /* M2P interaction list entries staged in LDS per flush:
+                               0 = default (128 FP32, 64 FP64); FP32 64|128, FP64 32|64 */
 } pepc_gpu_config;
 
 /* Counters of the last grow/traverse (tree_stats, module_tree.f90:622-699;
@@ -214,6 +215,43 @@ int pepc_gpu_run(void *ctx, int64_t nsteps, double dt, int flags);
 int pepc_gpu_adjoint_device(void *ctx, const double *gex, const double *gey, const double *gez,
                             const double *gpot, double *gx, double *gy, double *gz,
                             double *gq);
+
+/* Frozen interaction lists.  freeze records, for the last grow, every walk
+ * group's interaction list (the nodes of its leaf and accepted-twig
+ * interactions, in traversal order, by the reference acceptance rule);
+ * *nentries = total entries.  Synchronous.  replay_device then evaluates the
+ * frozen lists for new particle data x,y,z,q (device arrays, the grow's
+ * INPUT order): gathered through the grow permutation, moments recomputed on
+ * the same tree topology (FP64 upward pass), fields and potentials in FP64,
+ * sorted order out.  This is the function whose derivative
+ * pepc_gpu_adjoint_device returns (central differences of replay check it).
+ * Replay replaces the context's particle data; the next grow invalidates the
+ * lists. */
+int pepc_gpu_freeze(void *ctx, int64_t *nentries);
+int pepc_gpu_replay_device(void *ctx, const double *x, const double *y, const double *z,
+                           const double *q, double *ex, double *ey, double *ez, double *pot);
+
+/* ---------------------------------------------------------------------------
+ * Autotuner (no reference counterpart).  Times grow + traverse of the given
+ * device arrays (one system, or nsys batched systems with host offsets) for
+ * every candidate of group size {16, 32, 64} x group pack {0, 64} x list tile
+ * (FP32 {64, 128}, FP64 {32, 64}) x persistent grid {full, half}: one
+ * warm-up and reps timed runs each, device time (HIP events), best of reps.
+ * The fastest becomes the context's configuration (read it back with
+ * pepc_gpu_get_config); the tree is invalidated, so grow again.  out[0] is
+ * the winner, out[1..] every candidate in the order tried (up to cap
+ * entries); *ncand = number of candidates.  Synchronous. */
+typedef struct {
+    int32_t group_size, group_pack, list_tile, walk_blocks;
+    double t_grow_ms, t_walk_ms;
+} pepc_gpu_tune_entry;
+
+int pepc_gpu_autotune(void *ctx, int64_t n, int64_t nsys, const int64_t *offsets,
+                      const double *x, const double *y, const double *z, const double *q,
+                      int32_t reps, pepc_gpu_tune_entry *out, int64_t cap, int64_t *ncand);
+
+/* Copies the context's current configuration (walk_blocks = the grid in use). */
+int pepc_gpu_get_config(const void *ctx, pepc_gpu_config *cfg);
 
 /* Host copies of the state (each may be NULL), current order. */
 int pepc_gpu_state_get(void *ctx, double *x, double *y, double *z, double *vx, double *vy,

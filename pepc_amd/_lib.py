@@ -26,7 +26,8 @@ EXPORTS = [
     "pepc_gpu_read_sorted", "pepc_gpu_read_tree", "pepc_gpu_read_walk", "pepc_gpu_direct_device",
     "pepc_gpu_timings", "pepc_gpu_grow_batch_device", "pepc_gpu_state_set", "pepc_gpu_run",
     "pepc_gpu_state_get", "pepc_gpu_state_device", "pepc_gpu_read_energies",
-    "pepc_gpu_adjoint_device",
+    "pepc_gpu_adjoint_device", "pepc_gpu_autotune", "pepc_gpu_get_config", "pepc_gpu_freeze",
+    "pepc_gpu_replay_device",
 ]
 
 RUN_GRAPH = 1
@@ -37,7 +38,7 @@ class Config(C.Structure):
                 ("node_factor", C.c_double), ("mac_select", C.c_int32),
                 ("group_size", C.c_int32), ("precision", C.c_int32),
                 ("force_cubic_domain", C.c_int32), ("walk_blocks", C.c_int32),
-                ("device", C.c_int32), ("group_pack", C.c_int32), ("reserved", C.c_int32)]
+                ("device", C.c_int32), ("group_pack", C.c_int32), ("list_tile", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -52,6 +53,14 @@ class Stats(C.Structure):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "level_offset"}
         d["level_offset"] = list(self.level_offset)
         return d
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("group_size", C.c_int32), ("group_pack", C.c_int32), ("list_tile", C.c_int32),
+                ("walk_blocks", C.c_int32), ("t_grow_ms", C.c_double), ("t_walk_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class PepcGpuError(RuntimeError):
@@ -101,6 +110,11 @@ def load(path: str = LIB_PATH):
     L.pepc_gpu_state_device.argtypes = [_vp, C.POINTER(_vp), C.POINTER(_vp)]
     L.pepc_gpu_read_energies.argtypes = [_vp, _i64, _vp, C.POINTER(_i64)]
     L.pepc_gpu_adjoint_device.argtypes = [_vp] * 9
+    L.pepc_gpu_autotune.argtypes = [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, C.c_int32,
+                                    C.POINTER(TuneEntry), _i64, C.POINTER(_i64)]
+    L.pepc_gpu_get_config.argtypes = [_vp, C.POINTER(Config)]
+    L.pepc_gpu_freeze.argtypes = [_vp, C.POINTER(_i64)]
+    L.pepc_gpu_replay_device.argtypes = [_vp] * 9
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is not None and name != "pepc_gpu_config_default":

@@ -104,6 +104,63 @@ __global__ void adjoint_clear_kernel(int64_t n, int64_t cap, double* __restrict_
   }
 }
 
+// Group traversal with the reference acceptance rule (module_tree_walk_simple.f90:
+// 301-384): a twig is accepted when every target of the group passes
+// theta2 d^2 > b^2(level) (exact FP64, no contraction), else its children are
+// visited; a leaf is always an interaction.  on_entry(node, first, nchild,
+// cx, cy, cz) runs, uniformly across the wave, for every leaf and accepted
+// twig.  The pop policy is the forward walk's (at most 64 nodes per step,
+// room for 8 children each), so the stack never overflows; returns true if
+// it would have.
+template <typename OnEntry>
+__device__ bool traverse_group(uint32_t root, double xt, double yt, double zt, bool valid,
+                               double theta2, const NodeLink* __restrict__ link,
+                               const NodeGeom* __restrict__ geom, uint32_t* stack,
+                               OnEntry&& on_entry) {
+  const int lane = threadIdx.x;
+  uint32_t sp = 1;
+  if (lane == 0) stack[0] = root;
+  __syncthreads();
+  while (sp > 0) {
+    int room = ((int)kStackCap - (int)sp) / 7;
+    room = room < 1 ? 1 : room;
+    uint32_t m = sp < (uint32_t)room ? sp : (uint32_t)room;
+    m = m > (uint32_t)kWave ? (uint32_t)kWave : m;
+    const bool has = (uint32_t)lane < m;
+    const uint32_t mynode = has ? stack[sp - 1 - lane] : 0u;
+    NodeLink mylk = {0u, 0u, 0u, 0u};
+    NodeGeom mygm = {0.0, 0.0, 0.0, 0.0};
+    if (has) { mylk = link[mynode]; mygm = geom[mynode]; }
+    __syncthreads();
+    sp -= m;
+    for (uint32_t k = 0; k < m; ++k) {
+      const uint32_t node = __shfl(mynode, (int)k);
+      const uint32_t first = __shfl(mylk.first, (int)k);
+      const uint32_t nchild = __shfl(mylk.nchild, (int)k);
+      const double cx = __shfl(mygm.cx, (int)k), cy = __shfl(mygm.cy, (int)k),
+                   cz = __shfl(mygm.cz, (int)k), b2 = __shfl(mygm.b2, (int)k);
+      if (nchild != 0u) {
+        bool fail;
+        {
+#pragma clang fp contract(off)
+          const double dx = xt - cx, dy = yt - cy, dz = zt - cz;
+          const double d2 = dx * dx + dy * dy + dz * dz;
+          fail = valid && !(theta2 * d2 > b2);
+        }
+        if (__ballot(fail) != 0ull) {
+          if (sp + nchild > (uint32_t)(kStackCap + kStackSlack)) return true;
+          if ((uint32_t)lane < nchild) stack[sp + lane] = first + (uint32_t)lane;
+          sp += nchild;
+          continue;
+        }
+      }
+      on_entry(node, first, nchild, cx, cy, cz);
+    }
+    __syncthreads();
+  }
+  return false;
+}
+
 struct AdjArgs {
   const double *x, *y, *z, *q;
   const NodeGeom* geom;
@@ -124,7 +181,7 @@ __global__ __launch_bounds__(kWave) void adjoint_walk_kernel(AdjArgs a) {
   __shared__ uint32_t gsh;
   const int lane = threadIdx.x;
   const uint32_t ngroups = a.sc->ngroups;
-  const double theta2 = a.theta2, eps2 = a.eps2;
+  const double eps2 = a.eps2;
   for (;;) {
     if (lane == 0) gsh = atomicAdd(a.queue, 1u);
     __syncthreads();
@@ -139,92 +196,189 @@ __global__ __launch_bounds__(kWave) void adjoint_walk_kernel(AdjArgs a) {
     const double u[3] = {valid ? a.sign * a.gex[tid] : 0.0, valid ? a.sign * a.gey[tid] : 0.0,
                          valid ? a.sign * a.gez[tid] : 0.0};
     double tD[3] = {0.0, 0.0, 0.0};
-    uint32_t sp = 1;
-    bool overflow = false;
-    if (lane == 0) stack[0] = a.sys ? a.sysroot[a.sys[a0]] : 0u;
-    __syncthreads();
-    while (sp > 0) {
-      int room = ((int)kStackCap - (int)sp) / 7;
-      room = room < 1 ? 1 : room;
-      uint32_t m = sp < (uint32_t)room ? sp : (uint32_t)room;
-      m = m > (uint32_t)kWave ? (uint32_t)kWave : m;
-      const bool has = (uint32_t)lane < m;
-      const uint32_t mynode = has ? stack[sp - 1 - lane] : 0u;
-      NodeLink mylk = {0u, 0u, 0u, 0u};
-      NodeGeom mygm = {0.0, 0.0, 0.0, 0.0};
-      if (has) { mylk = a.link[mynode]; mygm = a.geom[mynode]; }
-      __syncthreads();
-      sp -= m;
-      for (uint32_t k = 0; k < m; ++k) {
-        const uint32_t node = __shfl(mynode, (int)k);
-        const uint32_t first = __shfl(mylk.first, (int)k);
-        const uint32_t nchild = __shfl(mylk.nchild, (int)k);
-        const double cx = __shfl(mygm.cx, (int)k), cy = __shfl(mygm.cy, (int)k),
-                     cz = __shfl(mygm.cz, (int)k), b2 = __shfl(mygm.b2, (int)k);
-        if (nchild == 0u) {   // leaf: P2P with source particle `first`
-          const double q = a.q[first];
-          const bool self = first == tid || !valid;
-          double d[3] = {xt - cx, yt - cy, zt - cz};
-          const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + eps2;
-          double D[3], gq, gp[3], M[6];
-          const double p0[3] = {0.0, 0.0, 0.0}, q0[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-          adjoint_term(d, self ? 1.0 : r2, gphi, u, q, p0, q0, D, gq, gp, M);
-          if (self) { D[0] = D[1] = D[2] = 0.0; gq = 0.0; }
-          tD[0] += D[0]; tD[1] += D[1]; tD[2] += D[2];
-          const double s0 = wsum(D[0]), s1 = wsum(D[1]), s2 = wsum(D[2]), s3 = wsum(gq);
-          if (lane == 0) {
-            unsafeAtomicAdd(&a.gx[first], -s0);
-            unsafeAtomicAdd(&a.gy[first], -s1);
-            unsafeAtomicAdd(&a.gz[first], -s2);
-            unsafeAtomicAdd(&a.gq[first], s3);
-          }
-          continue;
-        }
-        // twig: accepted iff every target passes theta2 d^2 > b^2(level)
-        bool fail;
-        {
-#pragma clang fp contract(off)
-          const double dx = xt - cx, dy = yt - cy, dz = zt - cz;
-          const double d2 = dx * dx + dy * dy + dz * dz;
-          fail = valid && !(theta2 * d2 > b2);
-        }
-        if (__ballot(fail) != 0ull) {
-          if (sp + nchild > (uint32_t)(kStackCap + kStackSlack)) { overflow = true; break; }
-          if ((uint32_t)lane < nchild) stack[sp + lane] = first + (uint32_t)lane;
-          sp += nchild;
-          continue;
-        }
-        const double* mm = a.mom64 + (size_t)node * 16;
-        const double q = mm[3];
-        const double p[3] = {mm[5], mm[6], mm[7]};
-        const double qt[6] = {mm[8], mm[9], mm[10], mm[11], mm[12], mm[13]};
-        double d[3] = {xt - cx, yt - cy, zt - cz};
-        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + eps2;
-        double D[3], gq, gp[3], M[6];
-        adjoint_term(d, r2, gphi, u, q, p, qt, D, gq, gp, M);
-        if (!valid) {
-          D[0] = D[1] = D[2] = 0.0; gq = 0.0; gp[0] = gp[1] = gp[2] = 0.0;
-#pragma unroll
-          for (int j = 0; j < 6; ++j) M[j] = 0.0;
-        }
+    const uint32_t root = a.sys ? a.sysroot[a.sys[a0]] : 0u;
+    auto entry = [&](uint32_t node, uint32_t first, uint32_t nchild, double cx, double cy,
+                     double cz) {
+      double d[3] = {xt - cx, yt - cy, zt - cz};
+      const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + eps2;
+      double D[3], gq, gp[3], M[6];
+      if (nchild == 0u) {   // leaf: P2P with source particle `first`
+        const double q = a.q[first];
+        const bool self = first == tid || !valid;
+        const double p0[3] = {0.0, 0.0, 0.0}, q0[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        adjoint_term(d, self ? 1.0 : r2, gphi, u, q, p0, q0, D, gq, gp, M);
+        if (self) { D[0] = D[1] = D[2] = 0.0; gq = 0.0; }
         tD[0] += D[0]; tD[1] += D[1]; tD[2] += D[2];
-        double v[13] = {gq, gp[0], gp[1], gp[2], M[0], M[1], M[2], M[3], M[4], M[5], D[0], D[1], D[2]};
-#pragma unroll
-        for (int j = 0; j < 13; ++j) v[j] = wsum(v[j]);
+        const double s0 = wsum(D[0]), s1 = wsum(D[1]), s2 = wsum(D[2]), s3 = wsum(gq);
         if (lane == 0) {
-          double* o = a.nadj + (size_t)node * 16;
-#pragma unroll
-          for (int j = 0; j < 13; ++j) unsafeAtomicAdd(&o[j], v[j]);
+          unsafeAtomicAdd(&a.gx[first], -s0);
+          unsafeAtomicAdd(&a.gy[first], -s1);
+          unsafeAtomicAdd(&a.gz[first], -s2);
+          unsafeAtomicAdd(&a.gq[first], s3);
         }
+        return;
       }
-      if (overflow) break;
-      __syncthreads();
-    }
+      const double* mm = a.mom64 + (size_t)node * 16;
+      const double q = mm[3];
+      const double p[3] = {mm[5], mm[6], mm[7]};
+      const double qt[6] = {mm[8], mm[9], mm[10], mm[11], mm[12], mm[13]};
+      adjoint_term(d, r2, gphi, u, q, p, qt, D, gq, gp, M);
+      if (!valid) {
+        D[0] = D[1] = D[2] = 0.0; gq = 0.0; gp[0] = gp[1] = gp[2] = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) M[j] = 0.0;
+      }
+      tD[0] += D[0]; tD[1] += D[1]; tD[2] += D[2];
+      double v[13] = {gq, gp[0], gp[1], gp[2], M[0], M[1], M[2], M[3], M[4], M[5], D[0], D[1], D[2]};
+#pragma unroll
+      for (int j = 0; j < 13; ++j) v[j] = wsum(v[j]);
+      if (lane == 0) {
+        double* o = a.nadj + (size_t)node * 16;
+#pragma unroll
+        for (int j = 0; j < 13; ++j) unsafeAtomicAdd(&o[j], v[j]);
+      }
+    };
+    const bool overflow = traverse_group(root, xt, yt, zt, valid, a.theta2, a.link, a.geom, stack,
+                                         entry);
     if (overflow && lane == 0) atomicOr(&a.sc->flags, kFlagStackOverflow);
     if (valid) {
       unsafeAtomicAdd(&a.gx[tid], tD[0]);
       unsafeAtomicAdd(&a.gy[tid], tD[1]);
       unsafeAtomicAdd(&a.gz[tid], tD[2]);
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------- frozen lists
+// Count (kRecord = false) or write (true) each group's interaction list: the
+// node of every leaf and accepted twig in traversal order.  Both passes make
+// the same decisions, so entries land in [off[g], off[g] + count[g]).
+template <bool kRecord>
+__global__ __launch_bounds__(kWave) void list_walk_kernel(ListArgs a) {
+  __shared__ uint32_t stack[kStackCap + kStackSlack];
+  __shared__ uint32_t gsh;
+  const int lane = threadIdx.x;
+  const uint32_t ngroups = a.sc->ngroups;
+  for (;;) {
+    if (lane == 0) gsh = atomicAdd(a.queue, 1u);
+    __syncthreads();
+    const uint32_t g = gsh;
+    __syncthreads();
+    if (g >= ngroups) break;
+    const uint32_t a0 = a.gfirst[g], cnt = a.gcount[g];
+    const bool valid = (uint32_t)lane < cnt;
+    const uint32_t tid = a0 + (valid ? (uint32_t)lane : 0u);
+    const double xt = a.x[tid], yt = a.y[tid], zt = a.z[tid];
+    const uint32_t root = a.sys ? a.sysroot[a.sys[a0]] : 0u;
+    uint32_t k = 0;
+    const uint32_t base = kRecord ? a.off[g] : 0u;
+    auto entry = [&](uint32_t node, uint32_t, uint32_t, double, double, double) {
+      if (kRecord && lane == 0) a.entries[base + k] = node;
+      ++k;
+    };
+    const bool overflow = traverse_group(root, xt, yt, zt, valid, a.theta2, a.link, a.geom, stack,
+                                         entry);
+    if (overflow && lane == 0) atomicOr(&a.sc->flags, kFlagStackOverflow);
+    if (!kRecord && lane == 0) a.count[g] = k;
+    __syncthreads();
+  }
+}
+
+// 64-bit total of the list lengths (the 32-bit offsets must not wrap).
+__global__ void list_total_kernel(const uint32_t* __restrict__ count,
+                                  unsigned long long* __restrict__ total,
+                                  const DevScalars* __restrict__ sc) {
+  const int64_t ng = sc->ngroups;
+  unsigned long long v = 0;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ng;
+       g += (int64_t)gridDim.x * blockDim.x)
+    v += count[g];
+  v = wave_sum(v);
+  if (lane_id() == 0 && v) atomicAdd(total, v);
+}
+
+__global__ void list_clear_kernel(unsigned long long* total, uint32_t* queue) {
+  if (threadIdx.x == 0) { *total = 0ull; *queue = 0u; }
+}
+
+__global__ void queue_clear_kernel(uint32_t* queue) {
+  if (threadIdx.x == 0) *queue = 0u;
+}
+
+// Leaf geometry from (new) particle data: centre = particle, last slot = charge.
+__global__ void leaf_geom_kernel(const NodeLink* __restrict__ link, const double* __restrict__ x,
+                                 const double* __restrict__ y, const double* __restrict__ z,
+                                 const double* __restrict__ q, NodeGeom* __restrict__ geom,
+                                 const DevScalars* __restrict__ sc) {
+  if (broken(sc)) return;
+  const int64_t m = sc->nnodes;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const NodeLink lk = link[i];
+    if (lk.nchild != 0u) continue;
+    NodeGeom gm;
+    gm.cx = x[lk.first]; gm.cy = y[lk.first]; gm.cz = z[lk.first]; gm.b2 = q[lk.first];
+    geom[i] = gm;
+  }
+}
+
+// Fields of the frozen lists, FP64, one group per wavefront, lane = target.
+// M2P from the reference-layout moments: phi = q/r - p.d/r^3 - tr/r^3 +
+// 3 d.Qt.d/r^5 and E = -grad phi (module_coulomb_kernels.f90:60-145, with
+// r^2 = d^2 + eps^2); P2P phi = q/r, E = q d/r^3, self excluded by index.
+__global__ __launch_bounds__(kWave) void replay_kernel(ReplayArgs a) {
+  __shared__ uint32_t gsh;
+  const int lane = threadIdx.x;
+  const uint32_t ngroups = a.sc->ngroups;
+  for (;;) {
+    if (lane == 0) gsh = atomicAdd(a.queue, 1u);
+    __syncthreads();
+    const uint32_t g = gsh;
+    __syncthreads();
+    if (g >= ngroups) break;
+    const uint32_t a0 = a.gfirst[g], cnt = a.gcount[g];
+    const bool valid = (uint32_t)lane < cnt;
+    const uint32_t tid = a0 + (valid ? (uint32_t)lane : 0u);
+    const double xt = a.x[tid], yt = a.y[tid], zt = a.z[tid];
+    double e[3] = {0.0, 0.0, 0.0}, ph = 0.0;
+    const uint32_t b = a.off[g], nent = a.count[g];
+    for (uint32_t k = 0; k < nent; ++k) {
+      const uint32_t node = a.entries[b + k];
+      const NodeLink lk = a.link[node];
+      const NodeGeom gm = a.geom[node];
+      const double d[3] = {xt - gm.cx, yt - gm.cy, zt - gm.cz};
+      const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + a.eps2;
+      if (lk.nchild == 0u) {
+        if (lk.first == tid || !valid) continue;
+        const double ri = 1.0 / sqrt(r2);
+        const double qr3 = gm.b2 * ri * ri * ri;
+        ph += gm.b2 * ri;
+        for (int c = 0; c < 3; ++c) e[c] += qr3 * d[c];
+        continue;
+      }
+      const double* mm = a.mom64 + (size_t)node * 16;
+      const double q = mm[3];
+      const double p[3] = {mm[5], mm[6], mm[7]};
+      const double hxy = 0.5 * mm[11], hyz = 0.5 * mm[12], hzx = 0.5 * mm[13];
+      const double Qd[3] = {mm[8] * d[0] + hxy * d[1] + hzx * d[2],
+                            hxy * d[0] + mm[9] * d[1] + hyz * d[2],
+                            hzx * d[0] + hyz * d[1] + mm[10] * d[2]};
+      const double tr = mm[8] + mm[9] + mm[10];
+      const double ri = 1.0 / sqrt(r2);
+      const double r3 = ri * ri * ri, r5 = r3 * ri * ri, r7 = r5 * ri * ri;
+      const double pd = p[0] * d[0] + p[1] * d[1] + p[2] * d[2];
+      const double dQd = d[0] * Qd[0] + d[1] * Qd[1] + d[2] * Qd[2];
+      ph += q * ri - pd * r3 - tr * r3 + 3.0 * dQd * r5;
+      for (int c = 0; c < 3; ++c)
+        e[c] += q * d[c] * r3 + p[c] * r3 - 3.0 * pd * d[c] * r5 - 3.0 * tr * d[c] * r5 -
+                6.0 * Qd[c] * r5 + 15.0 * dQd * d[c] * r7;
+    }
+    if (valid) {
+      a.ex[tid] = a.sign * e[0];
+      a.ey[tid] = a.sign * e[1];
+      a.ez[tid] = a.sign * e[2];
+      a.pot[tid] = a.sign * ph;
     }
     __syncthreads();
   }
@@ -307,6 +461,29 @@ unsigned grid_for(int64_t n) {
 }
 
 }  // namespace
+
+void launch_lists(hipStream_t s, const ListArgs& a, bool record, int blocks,
+                  unsigned long long* total) {
+  if (record) {
+    queue_clear_kernel<<<1, 64, 0, s>>>(a.queue);
+    list_walk_kernel<true><<<blocks, kWave, 0, s>>>(a);
+  } else {
+    list_clear_kernel<<<1, 64, 0, s>>>(total, a.queue);
+    list_walk_kernel<false><<<blocks, kWave, 0, s>>>(a);
+    list_total_kernel<<<256, kT, 0, s>>>(a.count, total, a.sc);
+  }
+}
+
+void launch_leaf_geom(hipStream_t s, const NodeLink* link, const double* x, const double* y,
+                      const double* z, const double* q, NodeGeom* geom, int64_t cap,
+                      const DevScalars* sc) {
+  leaf_geom_kernel<<<grid_for(cap), kT, 0, s>>>(link, x, y, z, q, geom, sc);
+}
+
+void launch_replay(hipStream_t s, const ReplayArgs& a, int blocks) {
+  queue_clear_kernel<<<1, 64, 0, s>>>(a.queue);
+  replay_kernel<<<blocks, kWave, 0, s>>>(a);
+}
 
 void launch_adjoint(hipStream_t s, const AdjointArgs& p, int blocks) {
   const int64_t n = p.n;

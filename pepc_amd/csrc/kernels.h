@@ -56,6 +56,8 @@ struct TreeBuffers {
   uint32_t* fc_rank;                 // first child by twig rank (+1 sentinel)
   uint32_t* node_rank;               // node index by twig rank
   uint32_t* nfirst;                  // first particle of each node
+  uint32_t* parent;                  // parent node (roots: 0xFFFFFFFF)
+  uint32_t* arrive;                  // upward pass: children counted in
   NodeLink* link;
   NodeGeom* geom;
   double* mom64;                     // 16 doubles per node, reference layout
@@ -84,6 +86,8 @@ void launch_dupfix(hipStream_t s, int64_t n, unsigned long long* keys, double* x
                    uint32_t* bsum, DevScalars* sc);
 void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size, int group_pack);
 void launch_leaf_moments(hipStream_t s, const TreeBuffers& t);
+// Upward pass alone (all levels) over an existing topology.
+void launch_upward(hipStream_t s, const TreeBuffers& t);
 
 // walk.hip
 struct WalkArgs {
@@ -110,9 +114,12 @@ struct WalkArgs {
   const uint32_t* sysroot;           // batched systems: root node per system
 };
 void launch_walk_reset(hipStream_t s, const WalkArgs& a);
-void launch_walk(hipStream_t s, const WalkArgs& a, int precision, int blocks);
+// list_tile: M2P list capacity in LDS (0 = default: 128 FP32, 64 FP64;
+// FP32 takes 64 or 128, FP64 32 or 64).
+bool walk_list_tile_ok(int precision, int list_tile);
+void launch_walk(hipStream_t s, const WalkArgs& a, int precision, int blocks, int list_tile);
 void launch_walk_stats(hipStream_t s, const WalkArgs& a);
-int walk_default_blocks();
+int walk_default_blocks(int precision, int list_tile);
 
 // step.hip: particle state of the on-device timestep (state order = key
 // order of the last grow) and the gather targets of the reorder.
@@ -147,6 +154,41 @@ struct AdjointArgs {
   uint32_t *parent, *leafof, *queue;
 };
 void launch_adjoint(hipStream_t s, const AdjointArgs& a, int blocks);
+
+// Frozen interaction lists (adjoint.hip): per group, the nodes of its leaf
+// and accepted-twig interactions in traversal order.
+struct ListArgs {
+  const double *x, *y, *z;
+  const NodeGeom* geom;
+  const NodeLink* link;
+  const uint32_t *gfirst, *gcount;
+  const uint32_t *sys, *sysroot;
+  DevScalars* sc;
+  double theta2;
+  uint32_t* count;      // per group (count pass)
+  const uint32_t* off;  // per group (record pass)
+  uint32_t* entries;    // record pass
+  uint32_t* queue;
+};
+// record = false: counts and their 64-bit total; record = true: entries.
+void launch_lists(hipStream_t s, const ListArgs& a, bool record, int blocks,
+                  unsigned long long* total);
+void launch_leaf_geom(hipStream_t s, const NodeLink* link, const double* x, const double* y,
+                      const double* z, const double* q, NodeGeom* geom, int64_t cap,
+                      const DevScalars* sc);
+struct ReplayArgs {
+  const double *x, *y, *z;
+  const NodeGeom* geom;
+  const NodeLink* link;
+  const double* mom64;
+  const uint32_t *gfirst, *gcount;
+  const uint32_t *count, *off, *entries;
+  DevScalars* sc;
+  double eps2, sign;
+  double *ex, *ey, *ez, *pot;
+  uint32_t* queue;
+};
+void launch_replay(hipStream_t s, const ReplayArgs& a, int blocks);
 
 // walk.hip: FP64 direct sum
 void launch_direct(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,

@@ -43,7 +43,7 @@ struct Ctx {
   unsigned long long *dkey[2] = {nullptr, nullptr};
   uint32_t *dval[2] = {nullptr, nullptr};
   uint32_t *dpart = nullptr, *inv = nullptr, *flag = nullptr, *fc_rank = nullptr,
-           *node_rank = nullptr, *nfirst = nullptr;
+           *node_rank = nullptr, *nfirst = nullptr, *nparent = nullptr, *arrive = nullptr;
   NodeLink* link = nullptr;
   NodeGeom* geom = nullptr;
   double* mom64 = nullptr;
@@ -80,6 +80,11 @@ struct Ctx {
   double* nadj = nullptr;
   uint32_t *parent = nullptr, *leafof = nullptr, *aqueue = nullptr;
   int64_t adj_mcap = 0, adj_ncap = 0;
+  // frozen interaction lists
+  uint32_t *lcount = nullptr, *loff = nullptr, *lent = nullptr;
+  unsigned long long* ltotal = nullptr;
+  int64_t list_ncap = 0, lent_cap = 0, frozen_entries = 0;
+  bool frozen = false;
 };
 
 int fail(Ctx* c, int code, const std::string& msg) {
@@ -128,7 +133,8 @@ void free_all(Ctx* c) {
                   c->dpart, c->inv, c->flag, c->fc_rank, c->node_rank, c->nfirst, c->link, c->geom,
                   c->mom64, c->wm32, c->wm64, c->hist, c->bsum, c->partial, c->box, c->sc,
                   c->sys, c->soff, c->sysroot, c->label, c->tlabel, c->ehist, c->ecount,
-                  c->nadj, c->parent, c->leafof, c->aqueue};
+                  c->nadj, c->parent, c->leafof, c->aqueue, c->lcount, c->loff, c->lent,
+                  c->ltotal, c->nparent, c->arrive};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (double* p : c->st)
@@ -165,6 +171,7 @@ int ensure_particles(Ctx* c, int64_t n) {
   ALLOC(c, c->dval[0], mc); ALLOC(c, c->dval[1], mc);
   ALLOC(c, c->dpart, mc); ALLOC(c, c->inv, mc); ALLOC(c, c->flag, mc);
   ALLOC(c, c->fc_rank, mc + 1); ALLOC(c, c->node_rank, mc); ALLOC(c, c->nfirst, mc);
+  ALLOC(c, c->nparent, mc); ALLOC(c, c->arrive, mc);
   ALLOC(c, c->link, mc); ALLOC(c, c->geom, mc); ALLOC(c, c->mom64, mc * 16);
   ALLOC(c, c->wm32, mc); ALLOC(c, c->wm64, mc);
   // radix histogram: 256 digits x sort blocks of the largest array sorted
@@ -293,6 +300,7 @@ int grow_device_impl(Ctx* c, int64_t n, const double* x, const double* y, const 
   t.dval[0] = c->dval[0]; t.dval[1] = c->dval[1];
   t.dpart = c->dpart; t.inv = c->inv; t.flag = c->flag; t.fc_rank = c->fc_rank;
   t.node_rank = c->node_rank; t.nfirst = c->nfirst; t.link = c->link; t.geom = c->geom;
+  t.parent = c->nparent; t.arrive = c->arrive;
   t.mom64 = c->mom64; t.wm32 = c->wm32; t.wm64 = c->wm64;
   t.gmark = c->gmark; t.gfirst = c->gfirst; t.gcount = c->gcount;
   t.hist = c->hist; t.bsum = c->bsum; t.box = c->box; t.sc = c->sc;
@@ -307,6 +315,7 @@ int grow_device_impl(Ctx* c, int64_t n, const double* x, const double* y, const 
   RECORD(c, 1);
   c->n = n;
   c->nsys = nsys;
+  c->frozen = false;
   c->grown = true;
   c->walked = false;
   return 0;
@@ -328,7 +337,7 @@ int traverse_device_impl(Ctx* c, double* ex, double* ey, double* ez, double* pot
   a.sysroot = c->sysroot;
   launch_walk_reset(s, a);
   RECORD(c, 2);
-  launch_walk(s, a, c->cfg.precision, c->walk_blocks);
+  launch_walk(s, a, c->cfg.precision, c->walk_blocks, c->cfg.list_tile);
   RECORD(c, 3);
   launch_walk_stats(s, a);
   HIPCHECK(c, hipGetLastError());
@@ -391,6 +400,8 @@ int pepc_gpu_create(void** out, const pepc_gpu_config* cfg) {
       cfg->group_pack < 0 || cfg->group_pack > 64 ||
       !(cfg->theta2 >= 0.0) || !(cfg->eps2 >= 0.0) || !(cfg->node_factor >= 1.0) ||
       (cfg->precision != PEPC_GPU_FP32_KAHAN && cfg->precision != PEPC_GPU_FP64) ||
+      !walk_list_tile_ok(cfg->precision == PEPC_GPU_FP64 ? kWalkFP64 : kWalkFP32Kahan,
+                         cfg->list_tile) ||
       !(cfg->sign == 1.0 || cfg->sign == -1.0))
     return PEPC_GPU_EINVAL;
   int ndev = 0;
@@ -416,7 +427,10 @@ int pepc_gpu_create(void** out, const pepc_gpu_config* cfg) {
     delete c;
     return PEPC_GPU_ENOMEM;
   }
-  c->walk_blocks = cfg->walk_blocks > 0 ? cfg->walk_blocks : walk_default_blocks();
+  c->walk_blocks = cfg->walk_blocks > 0
+                       ? cfg->walk_blocks
+                       : walk_default_blocks(cfg->precision == PEPC_GPU_FP64 ? kWalkFP64 : kWalkFP32Kahan,
+                                             cfg->list_tile);
   *out = c;
   return 0;
 }
@@ -831,14 +845,11 @@ int pepc_gpu_state_device(void* p, double** arrays, const int64_t** label) {
   return 0;
 }
 
-int pepc_gpu_adjoint_device(void* p, const double* gex, const double* gey, const double* gez,
-                            const double* gpot, double* gx, double* gy, double* gz, double* gq) {
-  Ctx* c = as_ctx(p);
-  if (!c) return PEPC_GPU_EINVAL;
-  if (!gex || !gey || !gez || !gpot || !gx || !gy || !gz || !gq)
-    return fail(c, PEPC_GPU_EINVAL, "adjoint: null argument");
-  if (!c->grown) return fail(c, PEPC_GPU_EINVAL, "adjoint: no tree (call grow first)");
-  HIPCHECK(c, hipSetDevice(c->cfg.device));
+}  // extern "C"
+
+namespace {
+
+int ensure_adjoint(Ctx* c) {
   if (c->adj_mcap != c->mcap || c->adj_ncap != c->ncap) {
     ALLOC(c, c->nadj, c->mcap * 16);
     ALLOC(c, c->parent, c->mcap);
@@ -847,6 +858,96 @@ int pepc_gpu_adjoint_device(void* p, const double* gex, const double* gey, const
     c->adj_mcap = c->mcap;
     c->adj_ncap = c->ncap;
   }
+  if (c->list_ncap != c->ncap) {
+    ALLOC(c, c->lcount, c->ncap);
+    ALLOC(c, c->loff, c->ncap + 1);
+    ALLOC(c, c->ltotal, 8);
+    c->list_ncap = c->ncap;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pepc_gpu_freeze(void* p, int64_t* nentries) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PEPC_GPU_EINVAL;
+  if (!c->grown) return fail(c, PEPC_GPU_EINVAL, "freeze: no tree (call grow first)");
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  int r = ensure_adjoint(c);
+  if (r) return r;
+  r = sync_impl(c, nullptr);
+  if (r) return r;
+  const int64_t ng = c->sc_host->ngroups;
+  ListArgs a;
+  a.x = c->xs; a.y = c->ys; a.z = c->zs;
+  a.geom = c->geom; a.link = c->link;
+  a.gfirst = c->gfirst; a.gcount = c->gcount;
+  a.sys = c->nsys > 1 ? c->sys : nullptr;
+  a.sysroot = c->sysroot;
+  a.sc = c->sc; a.theta2 = c->cfg.theta2;
+  a.count = c->lcount; a.off = c->loff; a.entries = c->lent; a.queue = c->aqueue;
+  launch_lists(c->stream, a, false, c->walk_blocks, c->ltotal);
+  HIPCHECK(c, hipGetLastError());
+  unsigned long long total = 0;
+  HIPCHECK(c, hipMemcpyAsync(&total, c->ltotal, sizeof(total), hipMemcpyDeviceToHost, c->stream));
+  r = sync_impl(c, nullptr);
+  if (r) return r;
+  if (total > 0xFFFFFFFFull)
+    return fail(c, PEPC_GPU_EOVERFLOW, "freeze: interaction lists exceed 2^32 entries");
+  launch_scan(c->stream, c->lcount, c->loff, ng, c->bsum, nullptr);
+  if ((int64_t)total > c->lent_cap) {
+    ALLOC(c, c->lent, (int64_t)total);
+    c->lent_cap = (int64_t)total;
+  }
+  a.entries = c->lent;
+  launch_lists(c->stream, a, true, c->walk_blocks, c->ltotal);
+  HIPCHECK(c, hipGetLastError());
+  r = sync_impl(c, nullptr);
+  if (r) return r;
+  c->frozen = true;
+  c->frozen_entries = (int64_t)total;
+  if (nentries) *nentries = (int64_t)total;
+  return 0;
+}
+
+int pepc_gpu_replay_device(void* p, const double* x, const double* y, const double* z,
+                           const double* q, double* ex, double* ey, double* ez, double* pot) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PEPC_GPU_EINVAL;
+  if (!x || !y || !z || !q || !ex || !ey || !ez || !pot)
+    return fail(c, PEPC_GPU_EINVAL, "replay: null argument");
+  if (!c->frozen) return fail(c, PEPC_GPU_EINVAL, "replay: no frozen lists (call freeze after grow)");
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  hipStream_t s = c->stream;
+  launch_gather4(s, c->n, c->v0, x, y, z, q, c->xs, c->ys, c->zs, c->qs);
+  launch_leaf_geom(s, c->link, c->xs, c->ys, c->zs, c->qs, c->geom, c->mcap, c->sc);
+  launch_upward(s, c->tb);
+  ReplayArgs a;
+  a.x = c->xs; a.y = c->ys; a.z = c->zs;
+  a.geom = c->geom; a.link = c->link; a.mom64 = c->mom64;
+  a.gfirst = c->gfirst; a.gcount = c->gcount;
+  a.count = c->lcount; a.off = c->loff; a.entries = c->lent;
+  a.sc = c->sc; a.eps2 = c->cfg.eps2; a.sign = c->cfg.sign;
+  a.ex = ex; a.ey = ey; a.ez = ez; a.pot = pot;
+  a.queue = c->aqueue;
+  launch_replay(s, a, c->walk_blocks);
+  HIPCHECK(c, hipGetLastError());
+  return 0;
+}
+
+int pepc_gpu_adjoint_device(void* p, const double* gex, const double* gey, const double* gez,
+                            const double* gpot, double* gx, double* gy, double* gz, double* gq) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PEPC_GPU_EINVAL;
+  if (!gex || !gey || !gez || !gpot || !gx || !gy || !gz || !gq)
+    return fail(c, PEPC_GPU_EINVAL, "adjoint: null argument");
+  if (!c->grown) return fail(c, PEPC_GPU_EINVAL, "adjoint: no tree (call grow first)");
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  int r = ensure_adjoint(c);
+  if (r) return r;
   AdjointArgs a;
   a.n = c->n; a.cap = c->mcap;
   a.x = c->xs; a.y = c->ys; a.z = c->zs; a.q = c->qs;
@@ -861,6 +962,96 @@ int pepc_gpu_adjoint_device(void* p, const double* gex, const double* gey, const
   a.nadj = c->nadj; a.parent = c->parent; a.leafof = c->leafof; a.queue = c->aqueue;
   launch_adjoint(c->stream, a, c->walk_blocks);
   HIPCHECK(c, hipGetLastError());
+  return 0;
+}
+
+int pepc_gpu_get_config(const void* p, pepc_gpu_config* cfg) {
+  const Ctx* c = reinterpret_cast<const Ctx*>(p);
+  if (!c || !cfg) return PEPC_GPU_EINVAL;
+  *cfg = c->cfg;
+  cfg->walk_blocks = c->walk_blocks;
+  return 0;
+}
+
+int pepc_gpu_autotune(void* p, int64_t n, int64_t nsys, const int64_t* offsets, const double* x,
+                      const double* y, const double* z, const double* q, int32_t reps,
+                      pepc_gpu_tune_entry* out, int64_t cap, int64_t* ncand) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PEPC_GPU_EINVAL;
+  if (!x || !y || !z || !q || n < 1 || reps < 1 || nsys < 1 || (nsys > 1 && !offsets))
+    return fail(c, PEPC_GPU_EINVAL, "autotune: bad argument");
+  if (nsys > 1 && offsets[nsys] != n) return fail(c, PEPC_GPU_EINVAL, "autotune: offsets must end at n");
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  int r = ensure_particles(c, n);
+  if (r) return r;
+  r = ensure_host_io(c, n);
+  if (r) return r;
+  const int prec = c->cfg.precision == PEPC_GPU_FP64 ? kWalkFP64 : kWalkFP32Kahan;
+  const int tiles[2] = {prec == kWalkFP64 ? 32 : 64, prec == kWalkFP64 ? 64 : 128};
+  const int sizes[3] = {16, 32, 64};
+  const int packs[2] = {0, 64};
+  struct Cand { int g, pk, lt, wb; double tg, tw; };
+  std::vector<Cand> cands;
+  for (int g : sizes)
+    for (int pk : packs)
+      for (int lt : tiles) {
+        const int full = walk_default_blocks(prec, lt);
+        for (int wb : {full, full / 2}) cands.push_back({g, pk, lt, wb > 0 ? wb : 1, 0.0, 0.0});
+      }
+  const pepc_gpu_config saved = c->cfg;
+  const int saved_wb = c->walk_blocks;
+  const int64_t ns = nsys > 1 ? nsys : 1;
+  int best = -1;
+  for (size_t k = 0; k < cands.size(); ++k) {
+    Cand& cd = cands[k];
+    c->cfg.group_size = cd.g;
+    c->cfg.group_pack = cd.pk;
+    c->cfg.list_tile = cd.lt;
+    c->walk_blocks = cd.wb;
+    double tbest = 1e300;
+    for (int rep = 0; rep <= reps; ++rep) {   // rep 0 warms up
+      r = grow_device_impl(c, n, x, y, z, q, ns, nsys > 1 ? offsets : nullptr);
+      if (!r) r = traverse_device_impl(c, c->res[0], c->res[1], c->res[2], c->res[3], c->res[4]);
+      if (!r) r = sync_impl(c, nullptr);
+      float tg = 0.f, tw = 0.f;
+      if (!r && (hipEventElapsedTime(&tg, c->ev[0], c->ev[1]) != hipSuccess ||
+                 hipEventElapsedTime(&tw, c->ev[2], c->ev[3]) != hipSuccess))
+        r = fail(c, PEPC_GPU_EHIP, "autotune: event timing failed");
+      if (r) {
+        c->cfg = saved;
+        c->walk_blocks = saved_wb;
+        c->grown = false;
+        return r;
+      }
+      if (rep > 0 && (double)tg + (double)tw < tbest) {
+        tbest = (double)tg + (double)tw;
+        cd.tg = tg;
+        cd.tw = tw;
+      }
+    }
+    if (best < 0 || cd.tg + cd.tw < cands[best].tg + cands[best].tw) best = (int)k;
+  }
+  c->cfg = saved;
+  c->cfg.group_size = cands[best].g;
+  c->cfg.group_pack = cands[best].pk;
+  c->cfg.list_tile = cands[best].lt;
+  c->cfg.walk_blocks = cands[best].wb;
+  c->walk_blocks = cands[best].wb;
+  c->grown = false;   // the next grow builds groups with the chosen configuration
+  c->walked = false;
+  if (ncand) *ncand = (int64_t)cands.size();
+  if (out) {
+    // best first, then the candidates in the order tried
+    int64_t j = 0;
+    if (j < cap) {
+      const Cand& b = cands[best];
+      out[j++] = {b.g, b.pk, b.lt, b.wb, b.tg, b.tw};
+    }
+    for (size_t k = 0; k < cands.size() && j < cap; ++k) {
+      const Cand& d = cands[k];
+      out[j++] = {d.g, d.pk, d.lt, d.wb, d.tg, d.tw};
+    }
+  }
   return 0;
 }
 

@@ -140,6 +140,9 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(
   hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// Ranks each element within its block (stable), stages the block's elements
+// in LDS in digit order, then writes them out so that consecutive threads
+// store consecutive positions of one digit's run (coalesced stores).
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const unsigned long long* __restrict__ kin, const uint32_t* __restrict__ vin,
     unsigned long long* __restrict__ kout, uint32_t* __restrict__ vout,
@@ -147,15 +150,20 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ ndev, int shift, int nbits, const uint32_t* __restrict__ hscan,
     int nblocks) {
   if (ndev && (int64_t)*ndev < n) n = *ndev;
-  if ((int64_t)blockIdx.x * kSortTile >= n) return;
+  const int64_t tile0 = (int64_t)blockIdx.x * kSortTile;
+  if (tile0 >= n) return;
   __shared__ uint32_t wcnt[kSortWaves][kRadix];
+  __shared__ uint32_t lbase[kRadix];
+  __shared__ unsigned long long skey[kSortTile];
+  __shared__ uint32_t sval[kSortTile];
+  __shared__ uint8_t sdig[kSortTile];
   const int w = threadIdx.x / kWave;
   const uint32_t mask = (1u << nbits) - 1u;
 #pragma unroll
   for (int k = 0; k < kSortWaves; ++k) wcnt[k][threadIdx.x] = 0;
   __syncthreads();
 
-  const int64_t base = (int64_t)blockIdx.x * kSortTile + (int64_t)w * (kSortItems * kWave);
+  const int64_t base = tile0 + (int64_t)w * (kSortItems * kWave);
   unsigned long long kk[kSortItems];
   uint32_t vv[kSortItems];
   uint32_t dig[kSortItems];
@@ -185,24 +193,37 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   }
   __syncthreads();
   {
+    // thread d: offsets of digit d per wave, and the block-local start of d
     const uint32_t d = threadIdx.x;
-    uint32_t run = hscan[(int64_t)d * nblocks + blockIdx.x];
+    uint32_t run = 0;
 #pragma unroll
     for (int k = 0; k < kSortWaves; ++k) {
-      uint32_t t = wcnt[k][d];
+      const uint32_t t = wcnt[k][d];
       wcnt[k][d] = run;
       run += t;
     }
+    uint32_t tot;
+    lbase[d] = block_exclusive_scan(run, &tot);
   }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
     const int64_t e = base + j * kWave + lane_id();
     if (e < n) {
-      const uint32_t pos = wcnt[w][dig[j]] + loc[j];
-      kout[pos] = kk[j];
-      vout[pos] = vv[j];
+      const uint32_t lp = lbase[dig[j]] + wcnt[w][dig[j]] + loc[j];
+      skey[lp] = kk[j];
+      sval[lp] = vv[j];
+      sdig[lp] = (uint8_t)dig[j];
     }
+  }
+  __syncthreads();
+  const int64_t rem = n - tile0;
+  const int cnt = rem < kSortTile ? (int)rem : kSortTile;
+  for (int i = threadIdx.x; i < cnt; i += kSortThreads) {
+    const uint32_t d = sdig[i];
+    const uint32_t pos = hscan[(int64_t)d * nblocks + blockIdx.x] + ((uint32_t)i - lbase[d]);
+    kout[pos] = skey[i];
+    vout[pos] = sval[i];
   }
 }
 

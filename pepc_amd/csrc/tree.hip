@@ -298,12 +298,25 @@ __global__ void twig_rank_kernel(const NodeLink* __restrict__ link, const uint32
 }
 
 __global__ void nchild_kernel(NodeLink* __restrict__ link, const uint32_t* __restrict__ fc_rank,
-                              const uint32_t* __restrict__ node_rank, const DevScalars* __restrict__ sc) {
+                              const uint32_t* __restrict__ node_rank, uint32_t* __restrict__ parent,
+                              const DevScalars* __restrict__ sc) {
   if (tree_broken(sc)) return;
   const int64_t nt = sc->ntwig;
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nt;
-       r += (int64_t)gridDim.x * blockDim.x)
-    link[node_rank[r]].nchild = fc_rank[r + 1] - fc_rank[r];
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t node = node_rank[r], f = fc_rank[r], nc = fc_rank[r + 1] - f;
+    link[node].nchild = nc;
+    for (uint32_t j = 0; j < nc; ++j) parent[f + j] = node;
+  }
+}
+
+// Roots have no parent; every other entry is written by nchild_kernel.
+__global__ void root_parent_kernel(uint32_t* __restrict__ parent, const DevScalars* __restrict__ sc) {
+  if (tree_broken(sc)) return;
+  const int64_t nroot = sc->level_offset[1];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nroot;
+       i += (int64_t)gridDim.x * blockDim.x)
+    parent[i] = 0xFFFFFFFFu;
 }
 
 // Moment record slots, reference layout (t_tree_node_interaction_data order).
@@ -335,95 +348,126 @@ __device__ __forceinline__ void write_walk_forms(int64_t i, const double* m, dou
   }
 }
 
-// One launch per level, deepest first; a thread per twig.  Children are
-// contiguous; a leaf child's moments are its particle's (coc = x, charge = q,
-// |charge| = |q|, higher moments and bmax 0), a twig child's come from its
-// record of the level below.
-__global__ __launch_bounds__(kT) void upward_kernel(int level, int prec, NodeLink* link,
-                                                   const double* __restrict__ x,
-                                                   const double* __restrict__ y,
-                                                   const double* __restrict__ z,
-                                                   const double* __restrict__ q,
-                                                   double* mom, NodeGeom* __restrict__ geom,
-                                                   NodeMoments<float>* __restrict__ wm32,
-                                                   NodeMoments<double>* __restrict__ wm64,
-                                                   const DevBox* __restrict__ boxes,
-                                                   const uint32_t* __restrict__ sys,
-                                                   const uint32_t* __restrict__ nfirst,
-                                                   const DevScalars* __restrict__ sc) {
-  if (tree_broken(sc)) return;
-  const int64_t b = sc->level_offset[level], e = sc->level_offset[level + 1];
-  for (int64_t i = b + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < e;
+// Moments of twig i from its children (contiguous from lk.first).  A leaf
+// child's moments are its particle's (coc = x, charge = q, |charge| = |q|,
+// higher moments and bmax 0), a twig child's come from its record.
+struct UpArgs {
+  int prec;
+  NodeLink* link;
+  const double *x, *y, *z, *q;
+  double* mom;
+  NodeGeom* geom;
+  NodeMoments<float>* wm32;
+  NodeMoments<double>* wm64;
+  const DevBox* boxes;
+  const uint32_t* sys;
+  const uint32_t* nfirst;
+  const uint32_t* parent;
+  uint32_t* arrive;
+  const DevScalars* sc;
+};
+
+__device__ void twig_moments(int64_t i, const NodeLink lk, const UpArgs& u) {
+  const uint32_t c0 = lk.first, nc = lk.nchild;
+  const double b2 = u.boxes[u.sys ? u.sys[u.nfirst[i]] : 0u].b2[lk.level];
+  // child j as (is_leaf, particle or node index)
+  uint32_t cidx[8];
+  uint32_t leafmask = 0, count = 0;
+  for (uint32_t j = 0; j < nc; ++j) {
+    const NodeLink cl = u.link[c0 + j];
+    if (cl.nchild == 0) { leafmask |= 1u << j; cidx[j] = cl.first; }
+    else cidx[j] = c0 + j;
+    count += cl.count;
+  }
+  const double* __restrict__ mom = u.mom;
+  auto val = [&](uint32_t j, int slot) -> double {
+    if (leafmask & (1u << j)) {
+      const uint32_t p = cidx[j];
+      switch (slot) {
+        case M_CX: return u.x[p];
+        case M_CY: return u.y[p];
+        case M_CZ: return u.z[p];
+        case M_Q: return u.q[p];
+        case M_AQ: return fabs(u.q[p]);
+        default: return 0.0;
+      }
+    }
+    return mom[(int64_t)cidx[j] * 16 + slot];
+  };
+  double charge = 0.0, absq = 0.0;
+  for (uint32_t j = 0; j < nc; ++j) charge = charge + val(j, M_Q);
+  for (uint32_t j = 0; j < nc; ++j) absq = absq + val(j, M_AQ);
+  double coc[3] = {0.0, 0.0, 0.0};
+  if (absq != 0.0) {
+    for (uint32_t j = 0; j < nc; ++j) {
+      const double aq = val(j, M_AQ);
+      for (int d = 0; d < 3; ++d) coc[d] = coc[d] + (val(j, M_CX + d) * aq);
+    }
+    for (int d = 0; d < 3; ++d) coc[d] = coc[d] / absq;
+  } else {
+    for (uint32_t j = 0; j < nc; ++j)
+      for (int d = 0; d < 3; ++d) coc[d] = coc[d] + val(j, M_CX + d);
+    for (int d = 0; d < 3; ++d) coc[d] = coc[d] / (double)nc;
+  }
+  double dip[3] = {0.0, 0.0, 0.0}, quad[3] = {0.0, 0.0, 0.0}, xy = 0.0, yz = 0.0, zx = 0.0;
+  double bmax = 0.0;
+  for (uint32_t j = 0; j < nc; ++j) {
+    double c[15];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) c[k] = val(j, k);
+    double s[3];
+    for (int d = 0; d < 3; ++d) s[d] = coc[d] - c[M_CX + d];
+    for (int d = 0; d < 3; ++d) dip[d] = dip[d] + c[M_PX + d] + c[M_Q] * s[d];
+    for (int d = 0; d < 3; ++d)
+      quad[d] = quad[d] + c[M_QX + d] + c[M_PX + d] * s[d] + 0.5 * c[M_Q] * (s[d] * s[d]);
+    xy = xy + c[M_XY] + c[M_PX] * s[1] + c[M_PY] * s[0] + c[M_Q] * s[0] * s[1];
+    yz = yz + c[M_YZ] + c[M_PY] * s[2] + c[M_PZ] * s[1] + c[M_Q] * s[1] * s[2];
+    zx = zx + c[M_ZX] + c[M_PZ] * s[0] + c[M_PX] * s[2] + c[M_Q] * s[2] * s[0];
+    const double v = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + c[M_BMAX];
+    if (j == 0 || v > bmax) bmax = v;
+  }
+  double m[16];
+  m[M_CX] = coc[0]; m[M_CY] = coc[1]; m[M_CZ] = coc[2];
+  m[M_Q] = charge; m[M_AQ] = absq;
+  for (int d = 0; d < 3; ++d) { m[M_PX + d] = dip[d]; m[M_QX + d] = quad[d]; }
+  m[M_XY] = xy; m[M_YZ] = yz; m[M_ZX] = zx; m[M_BMAX] = bmax; m[15] = 0.0;
+  u.link[i].count = count;
+  double2* out = reinterpret_cast<double2*>(u.mom + i * 16);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) out[k] = make_double2(m[2 * k], m[2 * k + 1]);
+  write_walk_forms(i, m, b2, u.prec, u.geom, u.wm32, u.wm64);
+}
+
+__global__ void arrive_clear_kernel(int64_t cap, uint32_t* __restrict__ arrive) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cap;
+       i += (int64_t)gridDim.x * blockDim.x)
+    arrive[i] = 0u;
+}
+
+// Upward pass in one launch.  One thread per leaf climbs towards the root;
+// at each parent it counts itself in with an atomic, and only the child that
+// arrives last (all siblings' records complete) computes the parent and
+// climbs on.  The release fence before the count and the acquire fence after
+// it make the siblings' records visible to the computing thread.  A twig is
+// computed exactly once, from its children in order, so the moments equal
+// those of a level-by-level pass bit for bit.  No thread waits, so the kernel
+// always drains.
+__global__ __launch_bounds__(kT) void upward_refit_kernel(UpArgs u) {
+  if (tree_broken(u.sc)) return;
+  const int64_t m = u.sc->nnodes;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const NodeLink lk = link[i];
-    if (lk.nchild == 0) continue;
-    const double b2 = boxes[sys ? sys[nfirst[i]] : 0u].b2[level];
-    const uint32_t c0 = lk.first, nc = lk.nchild;
-    // child j as (is_leaf, particle or node index)
-    uint32_t cidx[8];
-    uint32_t leafmask = 0, count = 0;
-    for (uint32_t j = 0; j < nc; ++j) {
-      const NodeLink cl = link[c0 + j];
-      if (cl.nchild == 0) { leafmask |= 1u << j; cidx[j] = cl.first; }
-      else cidx[j] = c0 + j;
-      count += cl.count;
+    if (u.link[i].nchild != 0u) continue;
+    uint32_t node = u.parent[i];
+    for (int guard = 0; node != 0xFFFFFFFFu && guard <= kMaxLevel; ++guard) {
+      const NodeLink lk = u.link[node];
+      __threadfence();
+      const uint32_t seen = atomicAdd(&u.arrive[node], 1u);
+      if (seen + 1u != lk.nchild) break;
+      __threadfence();
+      twig_moments(node, lk, u);
+      node = u.parent[node];
     }
-    auto val = [&](uint32_t j, int slot) -> double {
-      if (leafmask & (1u << j)) {
-        const uint32_t p = cidx[j];
-        switch (slot) {
-          case M_CX: return x[p];
-          case M_CY: return y[p];
-          case M_CZ: return z[p];
-          case M_Q: return q[p];
-          case M_AQ: return fabs(q[p]);
-          default: return 0.0;
-        }
-      }
-      return mom[(int64_t)cidx[j] * 16 + slot];
-    };
-    double charge = 0.0, absq = 0.0;
-    for (uint32_t j = 0; j < nc; ++j) charge = charge + val(j, M_Q);
-    for (uint32_t j = 0; j < nc; ++j) absq = absq + val(j, M_AQ);
-    double coc[3] = {0.0, 0.0, 0.0};
-    if (absq != 0.0) {
-      for (uint32_t j = 0; j < nc; ++j) {
-        const double aq = val(j, M_AQ);
-        for (int d = 0; d < 3; ++d) coc[d] = coc[d] + (val(j, M_CX + d) * aq);
-      }
-      for (int d = 0; d < 3; ++d) coc[d] = coc[d] / absq;
-    } else {
-      for (uint32_t j = 0; j < nc; ++j)
-        for (int d = 0; d < 3; ++d) coc[d] = coc[d] + val(j, M_CX + d);
-      for (int d = 0; d < 3; ++d) coc[d] = coc[d] / (double)nc;
-    }
-    double dip[3] = {0.0, 0.0, 0.0}, quad[3] = {0.0, 0.0, 0.0}, xy = 0.0, yz = 0.0, zx = 0.0;
-    double bmax = 0.0;
-    for (uint32_t j = 0; j < nc; ++j) {
-      double c[15];
-#pragma unroll
-      for (int k = 0; k < 15; ++k) c[k] = val(j, k);
-      double s[3];
-      for (int d = 0; d < 3; ++d) s[d] = coc[d] - c[M_CX + d];
-      for (int d = 0; d < 3; ++d) dip[d] = dip[d] + c[M_PX + d] + c[M_Q] * s[d];
-      for (int d = 0; d < 3; ++d)
-        quad[d] = quad[d] + c[M_QX + d] + c[M_PX + d] * s[d] + 0.5 * c[M_Q] * (s[d] * s[d]);
-      xy = xy + c[M_XY] + c[M_PX] * s[1] + c[M_PY] * s[0] + c[M_Q] * s[0] * s[1];
-      yz = yz + c[M_YZ] + c[M_PY] * s[2] + c[M_PZ] * s[1] + c[M_Q] * s[1] * s[2];
-      zx = zx + c[M_ZX] + c[M_PZ] * s[0] + c[M_PX] * s[2] + c[M_Q] * s[2] * s[0];
-      const double v = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) + c[M_BMAX];
-      if (j == 0 || v > bmax) bmax = v;
-    }
-    double m[16];
-    m[M_CX] = coc[0]; m[M_CY] = coc[1]; m[M_CZ] = coc[2];
-    m[M_Q] = charge; m[M_AQ] = absq;
-    for (int d = 0; d < 3; ++d) { m[M_PX + d] = dip[d]; m[M_QX + d] = quad[d]; }
-    m[M_XY] = xy; m[M_YZ] = yz; m[M_ZX] = zx; m[M_BMAX] = bmax; m[15] = 0.0;
-    link[i].count = count;
-    double2* out = reinterpret_cast<double2*>(mom + i * 16);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) out[k] = make_double2(m[2 * k], m[2 * k + 1]);
-    write_walk_forms(i, m, b2, prec, geom, wm32, wm64);
   }
 }
 
@@ -561,11 +605,9 @@ void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size, int 
                                                 t.flag, t.sc);
   launch_scan(s, t.flag, t.flag, cap, t.bsum, &t.sc->ntwig);
   twig_rank_kernel<<<grid_for(cap), kT, 0, s>>>(t.link, t.flag, t.fc_rank, t.node_rank, t.sc);
-  nchild_kernel<<<grid_for(cap), kT, 0, s>>>(t.link, t.fc_rank, t.node_rank, t.sc);
-  for (int l = kMaxLevel; l >= 0; --l)
-    upward_kernel<<<grid_for(cap / 4), kT, 0, s>>>(l, t.prec, t.link, t.x, t.y, t.z, t.q, t.mom64,
-                                                   t.geom, t.wm32, t.wm64, t.box, t.sys, t.nfirst,
-                                                   t.sc);
+  root_parent_kernel<<<grid_for(n), kT, 0, s>>>(t.parent, t.sc);
+  nchild_kernel<<<grid_for(cap), kT, 0, s>>>(t.link, t.fc_rank, t.node_rank, t.parent, t.sc);
+  launch_upward(s, t);
   gmark_clear_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark);
   group_mark_kernel<<<grid_for(cap), kT, 0, s>>>(group_size, group_pack, t.link, t.nfirst, t.gmark,
                                                  t.sc);
@@ -573,6 +615,17 @@ void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size, int 
   launch_scan(s, t.cnt, t.cnt, n, t.bsum, &t.sc->ngroups);
   group_emit_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark, t.cnt, t.gfirst, t.gcount);
   if (t.sys) sysroot_kernel<<<grid_for(t.nsys), kT, 0, s>>>(t.sys, t.nfirst, t.sysroot, t.sc);
+}
+
+void launch_upward(hipStream_t s, const TreeBuffers& t) {
+  arrive_clear_kernel<<<grid_for(t.cap), kT, 0, s>>>(t.cap, t.arrive);
+  UpArgs u;
+  u.prec = t.prec; u.link = t.link;
+  u.x = t.x; u.y = t.y; u.z = t.z; u.q = t.q;
+  u.mom = t.mom64; u.geom = t.geom; u.wm32 = t.wm32; u.wm64 = t.wm64;
+  u.boxes = t.box; u.sys = t.sys; u.nfirst = t.nfirst; u.parent = t.parent;
+  u.arrive = t.arrive; u.sc = t.sc;
+  upward_refit_kernel<<<grid_for(t.cap), kT, 0, s>>>(u);
 }
 
 void launch_leaf_moments(hipStream_t s, const TreeBuffers& t) {

@@ -42,7 +42,6 @@ constexpr int kListP = 128;
 
 template <typename R> struct Prec;
 template <> struct Prec<float> {
-  static constexpr int kListM = 128;
   using V4 = float4;
   // v_rsq_f32 (1 ulp).  Its argument is r^2 = d^2 + eps^2 of an interaction
   // that is not a self term; the input guard of rsqrtf() only matters for
@@ -51,7 +50,6 @@ template <> struct Prec<float> {
   static __device__ __forceinline__ float fm(float a, float b, float c) { return fmaf(a, b, c); }
 };
 template <> struct Prec<double> {
-  static constexpr int kListM = 64;
   using V4 = double4;
   static __device__ __forceinline__ double rsq(double x) { return rsqrt(x); }
   static __device__ __forceinline__ double fm(double a, double b, double c) { return fma(a, b, c); }
@@ -137,11 +135,12 @@ __device__ __forceinline__ bool mac_fails(double xt, double yt, double zt, doubl
   return !(theta2 * d2 > b2);
 }
 
-template <typename R>
+// LM: capacity of the M2P list staged in LDS (the list tile); the P2P list
+// holds kListP entries.
+template <typename R, int LM>
 __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
   using P = Prec<R>;
   using V4 = typename P::V4;
-  constexpr int LM = P::kListM;
   __shared__ uint32_t stack[kStackCap + kStackSlack];
   __shared__ V4 lm[LM][4];
   __shared__ V4 lp[kListP];
@@ -240,7 +239,11 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
       int room = ((int)kStackCap - (int)sp) / 7;
       room = room < 1 ? 1 : room;
       uint32_t m = sp < (uint32_t)room ? sp : (uint32_t)room;
-      m = m > (uint32_t)kWave ? (uint32_t)kWave : m;
+      // at most min(64, LM) nodes per step, so one step's appends always fit
+      // an emptied list (positions nm + rank < LM)
+      constexpr uint32_t kPop = (uint32_t)(LM < kWave ? LM : kWave);
+      static_assert(kListP >= kWave, "P2P list must hold one full step");
+      m = m > kPop ? kPop : m;
       if (nm + m > (uint32_t)LM) { __syncthreads(); flush_m2p(); nm = 0; }
       if (np + m > (uint32_t)kListP) { __syncthreads(); flush_p2p(); np = 0; }
       const bool has = (uint32_t)lane < m;
@@ -486,16 +489,24 @@ __global__ __launch_bounds__(256) void direct_kernel(int64_t n, const double* __
 
 }  // namespace
 
-int walk_default_blocks() {
+// Persistent grid: every wavefront the device can hold for this instance.
+int walk_default_blocks(int precision, int list_tile) {
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess) {
     hipDeviceProp_t p;
     if (hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
   }
   int per = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<float>, kWave, 0) != hipSuccess ||
-      per < 1)
-    per = 8;
+  hipError_t e;
+  if (precision == kWalkFP64)
+    e = list_tile == 32
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<double, 32>, kWave, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<double, 64>, kWave, 0);
+  else
+    e = list_tile == 64
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<float, 64>, kWave, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<float, 128>, kWave, 0);
+  if (e != hipSuccess || per < 1) per = 8;
   return cus * per;
 }
 
@@ -503,11 +514,24 @@ void launch_walk_reset(hipStream_t s, const WalkArgs& a) {
   walk_reset_kernel<<<1, 64, 0, s>>>(a.sc, a.queue);
 }
 
-void launch_walk(hipStream_t s, const WalkArgs& a, int precision, int blocks) {
-  if (precision == kWalkFP64)
-    walk_kernel<double><<<blocks, kWave, 0, s>>>(a);
-  else
-    walk_kernel<float><<<blocks, kWave, 0, s>>>(a);
+bool walk_list_tile_ok(int precision, int list_tile) {
+  if (list_tile == 0) return true;
+  return precision == kWalkFP64 ? (list_tile == 32 || list_tile == 64)
+                                : (list_tile == 64 || list_tile == 128);
+}
+
+void launch_walk(hipStream_t s, const WalkArgs& a, int precision, int blocks, int list_tile) {
+  if (precision == kWalkFP64) {
+    if (list_tile == 32)
+      walk_kernel<double, 32><<<blocks, kWave, 0, s>>>(a);
+    else
+      walk_kernel<double, 64><<<blocks, kWave, 0, s>>>(a);
+  } else {
+    if (list_tile == 64)
+      walk_kernel<float, 64><<<blocks, kWave, 0, s>>>(a);
+    else
+      walk_kernel<float, 128><<<blocks, kWave, 0, s>>>(a);
+  }
 }
 
 void launch_walk_stats(hipStream_t s, const WalkArgs& a) {

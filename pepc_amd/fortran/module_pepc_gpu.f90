@@ -64,7 +64,7 @@ module pepc_gpu_c
    type, bind(C) :: pepc_gpu_config
       real(c_double) :: theta2, eps2, sign, node_factor
       integer(c_int32_t) :: mac_select, group_size, precision, force_cubic_domain
-      integer(c_int32_t) :: walk_blocks, device, group_pack, reserved
+      integer(c_int32_t) :: walk_blocks, device, group_pack, list_tile
    end type pepc_gpu_config
 
    type, bind(C) :: pepc_gpu_stats
@@ -161,13 +161,14 @@ module module_pepc
    ! &walk_para_gpu: the GPU walk's own parameters
    integer, public :: group_size = 16         ! walk tile size (module_tree_walk_simple TILE_SIZE)
    integer, public :: group_pack = 0
+   integer, public :: list_tile = 0           ! LDS M2P list entries (0 = default)
    character(len=8), public :: precision = 'fp32'
    real(kind_physics), public :: interaction_sign = 1.0_kind_physics
    logical, public :: print_stats = .false.   ! the reference's DBG_STATS
 
    namelist /calc_force_coulomb/ force_law, mac_select, theta2, eps2
    namelist /libpepc/ force_cubic_domain, np_mult, print_stats
-   namelist /walk_para_gpu/ group_size, group_pack, precision, interaction_sign
+   namelist /walk_para_gpu/ group_size, group_pack, list_tile, precision, interaction_sign
 
    type(c_ptr) :: ctx = c_null_ptr
    integer(c_int64_t), allocatable :: last_perm(:)
@@ -261,6 +262,7 @@ contains
       cfg%node_factor = max(1.0_c_double, real(abs(np_mult), c_double))
       cfg%group_size = group_size
       cfg%group_pack = group_pack
+      cfg%list_tile = list_tile
       cfg%force_cubic_domain = merge(1, 0, force_cubic_domain)
       if (trim(precision) == 'fp64') then
          cfg%precision = 1

@@ -31,7 +31,7 @@ class Solver:
 
     def __init__(self, theta2=0.36, eps2=0.0, group_size=16, precision="fp32",
                  sign=1.0, node_factor=2.5, force_cubic_domain=False, walk_blocks=0,
-                 device=0, group_pack=0):
+                 device=0, group_pack=0, list_tile=0):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("pepc_amd.Solver needs a GPU (no HIP device visible)")
@@ -44,7 +44,7 @@ class Solver:
                                        sign=float(sign), node_factor=float(node_factor),
                                        force_cubic_domain=int(bool(force_cubic_domain)),
                                        walk_blocks=int(walk_blocks), device=int(device),
-                                       group_pack=int(group_pack))
+                                       group_pack=int(group_pack), list_tile=int(list_tile))
         self.device = torch.device("cuda", device)
         h = C.c_void_p()
         rc = self.lib.pepc_gpu_create(C.byref(h), C.byref(self.cfg))
@@ -136,6 +136,59 @@ class Solver:
         a, b = C.c_double(), C.c_double()
         self._check(self.lib.pepc_gpu_timings(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def autotune(self, x, y, z, q, offsets=None, reps=2):
+        """Times every candidate configuration on these particles and keeps the
+        fastest (see pepc_gpu_autotune).  Returns (best, candidates) as dicts."""
+        self._bind_stream()
+        xs = [self._dev(v) for v in (x, y, z, q)]
+        n = xs[0].numel()
+        off = None if offsets is None else np.ascontiguousarray(offsets, dtype=np.int64)
+        nsys = 1 if off is None else len(off) - 1
+        if off is not None and off[-1] != n:
+            raise ValueError("autotune: offsets must end at n")
+        cap = 64
+        out = (_lib.TuneEntry * cap)()
+        nc = C.c_int64()
+        self._check(self.lib.pepc_gpu_autotune(
+            self.h, n, nsys, None if off is None else off.ctypes.data_as(C.c_void_p),
+            *[_ptr(v) for v in xs], int(reps), out, cap, C.byref(nc)))
+        self._check(self.lib.pepc_gpu_get_config(self.h, C.byref(self.cfg)))
+        k = min(nc.value + 1, cap)
+        rows = [out[i].as_dict() for i in range(k)]
+        return rows[0], rows[1:]
+
+    def config(self) -> dict:
+        cfg = _lib.Config()
+        self._check(self.lib.pepc_gpu_get_config(self.h, C.byref(cfg)))
+        return {k: getattr(cfg, k) for k, _ in cfg._fields_}
+
+    def freeze(self) -> int:
+        """Records the interaction lists of the last grow; returns their size."""
+        self._bind_stream()
+        ne = C.c_int64()
+        self._check(self.lib.pepc_gpu_freeze(self.h, C.byref(ne)))
+        return ne.value
+
+    def replay(self, x, y, z, q, out=None, sync=True):
+        """FP64 fields of the frozen lists for new particle data (input order
+        of the last grow); sorted order out, as traverse."""
+        t = self.torch
+        n = self.n
+        xs = [self._dev(v) for v in (x, y, z, q)]
+        if any(v.numel() != n for v in xs):
+            raise ValueError("replay: arrays must have the grown size")
+        if out is None:
+            out = TraverseResult(t.empty((3, n), dtype=t.float64, device=self.device),
+                                 t.empty(n, dtype=t.float64, device=self.device), None)
+        self._bind_stream()
+        e = out.e
+        self._check(self.lib.pepc_gpu_replay_device(self.h, *[_ptr(v) for v in xs], _ptr(e[0]),
+                                                    _ptr(e[1]), _ptr(e[2]), _ptr(out.pot)))
+        self._replay_inputs = xs
+        if sync:
+            self.sync()
+        return out
 
     def adjoint(self, ge, gpot, sync=True):
         """dL/dx, dL/dy, dL/dz, dL/dq (sorted order, float64 device tensors)

@@ -38,6 +38,7 @@ def test_create_rejects_bad_config_before_touching_device():
     L = _lib.load()
     h = ctypes.c_void_p()
     for kw in ({"group_size": 0}, {"group_size": 65}, {"mac_select": 1}, {"sign": 2.0},
-               {"theta2": -1.0}, {"precision": 7}):
+               {"theta2": -1.0}, {"precision": 7}, {"list_tile": 32}, {"list_tile": 100},
+               {"precision": 1, "list_tile": 128}, {"group_pack": 65}):
         cfg = _lib.default_config(**kw)
         assert L.pepc_gpu_create(ctypes.byref(h), ctypes.byref(cfg)) == _lib.EINVAL
