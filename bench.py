@@ -71,6 +71,29 @@ def run_gpu(sys_, precision, group, steps, warmup, dev_inputs, pack=0):
     return sol, out, st, ms, float(np.mean(tg)), float(np.mean(tw))
 
 
+def run_timestep(sys_, group, pack, steps, dt=1e-3):
+    """The same solve inside the on-device leapfrog step (graph replay): each
+    step pushes the particles and grows from the previous step's key order,
+    as a frontend's time loop does."""
+    import torch
+
+    from pepc_amd.solver import Solver
+    sol = Solver(theta2=sys_.theta2, eps2=sys_.eps2, group_size=group, sign=sys_.sign,
+                 group_pack=pack)
+    sol.state_set(sys_.x, sys_.y, sys_.z, sys_.v, sys_.q, sys_.m, max_records=steps + 4)
+    sol.run(2, dt, graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sol.run(steps, dt, graph=True, sync=False)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    sol.sync()
+    en = sol.energies()
+    sol.close()
+    tot = en[:, 0, 0] + en[:, 0, 1]
+    return ms, float(abs(tot[-1] - tot[0]) / abs(tot[0]))
+
+
 def force_error(sol, out, sys_, nsamp):
     from pepc_amd import workloads
     keys, perm, xs, ys, zs, qs = sol.sorted_host()
@@ -120,6 +143,8 @@ def main():
     ap.add_argument("--err-samples", type=int, default=65536)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timestep", action="store_true",
+                    help="skip the on-device leapfrog measurement (profiling runs)")
     ap.add_argument("--cpu-stride", type=int, default=1)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (reference kernel for "
@@ -173,6 +198,11 @@ def main():
             f"fp32 speedup {ms64 / ms:.2f}x")
         sol64.close()
 
+    ts_ms = ts_drift = None
+    if not args.no_timestep:
+        ts_ms, ts_drift = run_timestep(s, args.group, args.pack, max(args.steps, 5))
+        log(f"on-device leapfrog step (graph): {ts_ms:.3f} ms/step, energy drift {ts_drift:.2e}")
+
     cpu = None
     if not args.no_cpu:
         nthr = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -200,7 +230,11 @@ def main():
                    "interactions_per_s": st["interactions"] / (t_walk_ms * 1e-3),
                    "median_rel_force_err": med, "max_rel_force_err": mx,
                    "median_rel_pot_err": pmed, "err_samples": args.err_samples,
-                   "fp64_path": fp64},
+                   "fp64_path": fp64,
+                   "timestep_ms_per_step": ts_ms,
+                   "timestep_note": "kick-drift-kick step with grow from the previous step's key "
+                                    "order, HIP-graph replay (dt=1e-3)",
+                   "timestep_energy_drift": ts_drift},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VALU_TFLOPS,
                      "traffic": traffic, "kernel": "walk_kernel<float>",

@@ -57,7 +57,6 @@ struct TreeBuffers {
   uint32_t* node_rank;               // node index by twig rank
   uint32_t* nfirst;                  // first particle of each node
   uint32_t* parent;                  // parent node (roots: 0xFFFFFFFF)
-  uint32_t* arrive;                  // upward pass: children counted in
   NodeLink* link;
   NodeGeom* geom;
   double* mom64;                     // 16 doubles per node, reference layout
@@ -133,8 +132,12 @@ void launch_kick_drift(hipStream_t s, int64_t n, double dt, const StepState& st)
 void launch_reorder_kick(hipStream_t s, int64_t n, double dt, const uint32_t* perm, const double* xs,
                          const double* ys, const double* zs, const double* qs, const double* ex,
                          const double* ey, const double* ez, const StepState& st);
-void launch_energy(hipStream_t s, int nsys, const int64_t* soff, int64_t n, const double* pot,
-                   const StepState& st, double* hist, int64_t cap, uint32_t* counter);
+// Chunks: [cbeg[c], cbeg[c+1]) lies in one system; system s owns chunks
+// [sys_chunk[s], sys_chunk[s+1]).
+constexpr int64_t kEnergyChunk = 4096;
+void launch_energy(hipStream_t s, int nsys, int nchunk, const int64_t* cbeg, const int* sys_chunk,
+                   const double* pot, const StepState& st, double* partial, double* hist,
+                   int64_t cap, uint32_t* counter);
 void launch_counter_set(hipStream_t s, uint32_t* counter, uint32_t v);
 
 // adjoint.hip: gradient of the frozen-list solve (sorted order in and out)

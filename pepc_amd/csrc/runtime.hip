@@ -43,7 +43,7 @@ struct Ctx {
   unsigned long long *dkey[2] = {nullptr, nullptr};
   uint32_t *dval[2] = {nullptr, nullptr};
   uint32_t *dpart = nullptr, *inv = nullptr, *flag = nullptr, *fc_rank = nullptr,
-           *node_rank = nullptr, *nfirst = nullptr, *nparent = nullptr, *arrive = nullptr;
+           *node_rank = nullptr, *nfirst = nullptr, *nparent = nullptr;
   NodeLink* link = nullptr;
   NodeGeom* geom = nullptr;
   double* mom64 = nullptr;
@@ -72,6 +72,10 @@ struct Ctx {
   double* ehist = nullptr;          // energies: record x system x {pot, kin}
   int64_t ehist_cap = 0, nrec = 0;
   uint32_t* ecount = nullptr;
+  int64_t* echunk = nullptr;        // chunk boundaries of the energy reduction
+  int* esyschunk = nullptr;         // first chunk of each system
+  double* epartial = nullptr;
+  int nechunk = 0;
   hipStream_t cstream = nullptr;    // capture stream
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -134,7 +138,7 @@ void free_all(Ctx* c) {
                   c->mom64, c->wm32, c->wm64, c->hist, c->bsum, c->partial, c->box, c->sc,
                   c->sys, c->soff, c->sysroot, c->label, c->tlabel, c->ehist, c->ecount,
                   c->nadj, c->parent, c->leafof, c->aqueue, c->lcount, c->loff, c->lent,
-                  c->ltotal, c->nparent, c->arrive};
+                  c->ltotal, c->nparent, c->echunk, c->esyschunk, c->epartial};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (double* p : c->st)
@@ -171,7 +175,7 @@ int ensure_particles(Ctx* c, int64_t n) {
   ALLOC(c, c->dval[0], mc); ALLOC(c, c->dval[1], mc);
   ALLOC(c, c->dpart, mc); ALLOC(c, c->inv, mc); ALLOC(c, c->flag, mc);
   ALLOC(c, c->fc_rank, mc + 1); ALLOC(c, c->node_rank, mc); ALLOC(c, c->nfirst, mc);
-  ALLOC(c, c->nparent, mc); ALLOC(c, c->arrive, mc);
+  ALLOC(c, c->nparent, mc);
   ALLOC(c, c->link, mc); ALLOC(c, c->geom, mc); ALLOC(c, c->mom64, mc * 16);
   ALLOC(c, c->wm32, mc); ALLOC(c, c->wm64, mc);
   // radix histogram: 256 digits x sort blocks of the largest array sorted
@@ -300,7 +304,7 @@ int grow_device_impl(Ctx* c, int64_t n, const double* x, const double* y, const 
   t.dval[0] = c->dval[0]; t.dval[1] = c->dval[1];
   t.dpart = c->dpart; t.inv = c->inv; t.flag = c->flag; t.fc_rank = c->fc_rank;
   t.node_rank = c->node_rank; t.nfirst = c->nfirst; t.link = c->link; t.geom = c->geom;
-  t.parent = c->nparent; t.arrive = c->arrive;
+  t.parent = c->nparent;
   t.mom64 = c->mom64; t.wm32 = c->wm32; t.wm64 = c->wm64;
   t.gmark = c->gmark; t.gfirst = c->gfirst; t.gcount = c->gcount;
   t.hist = c->hist; t.bsum = c->bsum; t.box = c->box; t.sc = c->sc;
@@ -701,8 +705,8 @@ int step_impl(Ctx* c, double dt) {
   s = c->stream;
   launch_reorder_kick(s, n, dt, c->v0, c->xs, c->ys, c->zs, c->qs, c->res[0], c->res[1], c->res[2],
                       st);
-  launch_energy(s, (int)c->nsys, c->nsys > 1 ? c->soff : nullptr, n, c->res[3], st, c->ehist,
-                c->ehist_cap, c->ecount);
+  launch_energy(s, (int)c->nsys, c->nechunk, c->echunk, c->esyschunk, c->res[3], st, c->epartial,
+                c->ehist, c->ehist_cap, c->ecount);
   HIPCHECK(c, hipGetLastError());
   c->nrec += 1;
   return 0;
@@ -750,6 +754,28 @@ int pepc_gpu_state_set(void* p, int64_t n, int64_t nsys, const int64_t* offsets,
   ALLOC(c, c->tlabel, c->ncap);
   ALLOC(c, c->ehist, max_records * nsys * 2);
   ALLOC(c, c->ecount, 64);
+  {
+    // energy reduction chunks: whole systems cut into pieces of kEnergyChunk
+    std::vector<int64_t> cb;
+    std::vector<int> sc(nsys + 1);
+    for (int64_t k = 0; k < nsys; ++k) {
+      const int64_t b = nsys > 1 ? offsets[k] : 0, e = nsys > 1 ? offsets[k + 1] : n;
+      sc[k] = (int)cb.size();
+      for (int64_t i = b; i < e; i += kEnergyChunk) cb.push_back(i);
+    }
+    sc[nsys] = (int)cb.size();
+    const int nch = (int)cb.size();
+    cb.push_back(n);
+    ALLOC(c, c->echunk, nch + 1);
+    ALLOC(c, c->esyschunk, nsys + 1);
+    ALLOC(c, c->epartial, 2 * (int64_t)(nch > 0 ? nch : 1));
+    HIPCHECK(c, hipMemcpyAsync(c->echunk, cb.data(), sizeof(int64_t) * (nch + 1), hipMemcpyHostToDevice,
+                               c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->esyschunk, sc.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice,
+                               c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->nechunk = nch;
+  }
   c->ehist_cap = max_records;
   drop_graph(c);
   for (int k = 0; k < 8; ++k)

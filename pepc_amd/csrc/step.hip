@@ -71,13 +71,16 @@ __global__ void commit_kernel(int64_t n, StepState st) {
   }
 }
 
-// One block per system; record rec = *counter (records past cap are dropped).
-__global__ __launch_bounds__(kT) void energy_kernel(int nsys, const int64_t* __restrict__ soff,
-                                                   int64_t n, const double* __restrict__ pot,
-                                                   StepState st, double* __restrict__ hist,
-                                                   int64_t cap, const uint32_t* __restrict__ counter) {
-  const int sid = blockIdx.x;
-  const int64_t b = soff ? soff[sid] : 0, e = soff ? soff[sid + 1] : n;
+// Energies in two deterministic passes (fixed summation order, so graph
+// replays and eager steps agree bit for bit): one block per chunk of at most
+// kEnergyChunk particles of one system writes its partial sums; then one thread per
+// system adds its chunks' partials in order into record rec = *counter
+// (records past cap are dropped).
+__global__ __launch_bounds__(kT) void energy_chunk_kernel(const int64_t* __restrict__ cbeg,
+                                                         const double* __restrict__ pot, StepState st,
+                                                         double* __restrict__ partial) {
+  const int c = blockIdx.x;
+  const int64_t b = cbeg[c], e = cbeg[c + 1];
   double ep = 0.0, ek = 0.0;
   for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
     ep += st.q[i] * pot[i];
@@ -90,12 +93,26 @@ __global__ __launch_bounds__(kT) void energy_kernel(int nsys, const int64_t* __r
   if (lane_id() == 0) { red[0][w] = ep; red[1][w] = ek; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double a = 0.0, c = 0.0;
-    for (int k = 0; k < kT / kWave; ++k) { a += red[0][k]; c += red[1][k]; }
-    const int64_t rec = *counter;
+    double a = 0.0, d = 0.0;
+    for (int k = 0; k < kT / kWave; ++k) { a += red[0][k]; d += red[1][k]; }
+    partial[2 * c] = a;
+    partial[2 * c + 1] = d;
+  }
+}
+
+__global__ void energy_system_kernel(int nsys, const int* __restrict__ sys_chunk,
+                                     const double* __restrict__ partial, double* __restrict__ hist,
+                                     int64_t cap, const uint32_t* __restrict__ counter) {
+  const int64_t rec = *counter;
+  for (int sid = blockIdx.x * blockDim.x + threadIdx.x; sid < nsys; sid += gridDim.x * blockDim.x) {
+    double a = 0.0, d = 0.0;
+    for (int c = sys_chunk[sid]; c < sys_chunk[sid + 1]; ++c) {
+      a += partial[2 * c];
+      d += partial[2 * c + 1];
+    }
     if (rec < cap) {
       hist[(rec * nsys + sid) * 2 + 0] = 0.5 * a;
-      hist[(rec * nsys + sid) * 2 + 1] = 0.5 * c;
+      hist[(rec * nsys + sid) * 2 + 1] = 0.5 * d;
     }
   }
 }
@@ -129,9 +146,11 @@ void launch_reorder_kick(hipStream_t s, int64_t n, double dt, const uint32_t* pe
   commit_kernel<<<grid_for(n), kT, 0, s>>>(n, st);
 }
 
-void launch_energy(hipStream_t s, int nsys, const int64_t* soff, int64_t n, const double* pot,
-                   const StepState& st, double* hist, int64_t cap, uint32_t* counter) {
-  energy_kernel<<<nsys, kT, 0, s>>>(nsys, soff, n, pot, st, hist, cap, counter);
+void launch_energy(hipStream_t s, int nsys, int nchunk, const int64_t* cbeg, const int* sys_chunk,
+                   const double* pot, const StepState& st, double* partial, double* hist,
+                   int64_t cap, uint32_t* counter) {
+  if (nchunk > 0) energy_chunk_kernel<<<nchunk, kT, 0, s>>>(cbeg, pot, st, partial);
+  energy_system_kernel<<<(nsys + 255) / 256, 256, 0, s>>>(nsys, sys_chunk, partial, hist, cap, counter);
   counter_inc_kernel<<<1, 64, 0, s>>>(counter);
 }
 

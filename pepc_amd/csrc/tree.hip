@@ -362,8 +362,6 @@ struct UpArgs {
   const DevBox* boxes;
   const uint32_t* sys;
   const uint32_t* nfirst;
-  const uint32_t* parent;
-  uint32_t* arrive;
   const DevScalars* sc;
 };
 
@@ -438,36 +436,18 @@ __device__ void twig_moments(int64_t i, const NodeLink lk, const UpArgs& u) {
   write_walk_forms(i, m, b2, u.prec, u.geom, u.wm32, u.wm64);
 }
 
-__global__ void arrive_clear_kernel(int64_t cap, uint32_t* __restrict__ arrive) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cap;
-       i += (int64_t)gridDim.x * blockDim.x)
-    arrive[i] = 0u;
-}
-
-// Upward pass in one launch.  One thread per leaf climbs towards the root;
-// at each parent it counts itself in with an atomic, and only the child that
-// arrives last (all siblings' records complete) computes the parent and
-// climbs on.  The release fence before the count and the acquire fence after
-// it make the siblings' records visible to the computing thread.  A twig is
-// computed exactly once, from its children in order, so the moments equal
-// those of a level-by-level pass bit for bit.  No thread waits, so the kernel
-// always drains.
-__global__ __launch_bounds__(kT) void upward_refit_kernel(UpArgs u) {
+// One launch per level, deepest first, one thread per node of the level
+// (leaves skip).  The launch boundary orders a level's writes before the
+// next level's reads on every XCD; a single-launch scheme would need an
+// agent-scope release per node, which writes back the L2 of its XCD.
+__global__ __launch_bounds__(kT) void upward_level_kernel(int level, UpArgs u) {
   if (tree_broken(u.sc)) return;
-  const int64_t m = u.sc->nnodes;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+  const int64_t b = u.sc->level_offset[level], e = u.sc->level_offset[level + 1];
+  for (int64_t i = b + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < e;
        i += (int64_t)gridDim.x * blockDim.x) {
-    if (u.link[i].nchild != 0u) continue;
-    uint32_t node = u.parent[i];
-    for (int guard = 0; node != 0xFFFFFFFFu && guard <= kMaxLevel; ++guard) {
-      const NodeLink lk = u.link[node];
-      __threadfence();
-      const uint32_t seen = atomicAdd(&u.arrive[node], 1u);
-      if (seen + 1u != lk.nchild) break;
-      __threadfence();
-      twig_moments(node, lk, u);
-      node = u.parent[node];
-    }
+    const NodeLink lk = u.link[i];
+    if (lk.nchild == 0u) continue;
+    twig_moments(i, lk, u);
   }
 }
 
@@ -618,14 +598,14 @@ void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size, int 
 }
 
 void launch_upward(hipStream_t s, const TreeBuffers& t) {
-  arrive_clear_kernel<<<grid_for(t.cap), kT, 0, s>>>(t.cap, t.arrive);
   UpArgs u;
   u.prec = t.prec; u.link = t.link;
   u.x = t.x; u.y = t.y; u.z = t.z; u.q = t.q;
   u.mom = t.mom64; u.geom = t.geom; u.wm32 = t.wm32; u.wm64 = t.wm64;
-  u.boxes = t.box; u.sys = t.sys; u.nfirst = t.nfirst; u.parent = t.parent;
-  u.arrive = t.arrive; u.sc = t.sc;
-  upward_refit_kernel<<<grid_for(t.cap), kT, 0, s>>>(u);
+  u.boxes = t.box; u.sys = t.sys; u.nfirst = t.nfirst;
+  u.sc = t.sc;
+  for (int l = kMaxLevel; l >= 0; --l)
+    upward_level_kernel<<<grid_for(t.cap / 4), kT, 0, s>>>(l, u);
 }
 
 void launch_leaf_moments(hipStream_t s, const TreeBuffers& t) {
