@@ -110,9 +110,9 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t
 
 // ---------------------------------------------------------------- radix sort
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
+constexpr int kSortItems = 8;
 constexpr int kSortWaves = kSortThreads / kWave;
-constexpr int kSortTile = kSortThreads * kSortItems;   // 4096
+constexpr int kSortTile = kSortThreads * kSortItems;   // 2048
 constexpr int kRadix = 256;
 
 // The digit of element e is bits [shift, shift+nbits) of its key, or, when

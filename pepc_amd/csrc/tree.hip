@@ -368,51 +368,74 @@ struct UpArgs {
 __device__ void twig_moments(int64_t i, const NodeLink lk, const UpArgs& u) {
   const uint32_t c0 = lk.first, nc = lk.nchild;
   const double b2 = u.boxes[u.sys ? u.sys[u.nfirst[i]] : 0u].b2[lk.level];
-  // child j as (is_leaf, particle or node index)
-  uint32_t cidx[8];
+  // Every child's charge, |charge| and centre are requested together (a leaf
+  // child's from its particle, a twig child's from the first 48 bytes of its
+  // record), so the loads of all children overlap.
+  double cq[8], ca[8], cx[8], cy[8], cz[8];
   uint32_t leafmask = 0, count = 0;
-  for (uint32_t j = 0; j < nc; ++j) {
-    const NodeLink cl = u.link[c0 + j];
-    if (cl.nchild == 0) { leafmask |= 1u << j; cidx[j] = cl.first; }
-    else cidx[j] = c0 + j;
-    count += cl.count;
-  }
-  const double* __restrict__ mom = u.mom;
-  auto val = [&](uint32_t j, int slot) -> double {
-    if (leafmask & (1u << j)) {
-      const uint32_t p = cidx[j];
-      switch (slot) {
-        case M_CX: return u.x[p];
-        case M_CY: return u.y[p];
-        case M_CZ: return u.z[p];
-        case M_Q: return u.q[p];
-        case M_AQ: return fabs(u.q[p]);
-        default: return 0.0;
+#pragma unroll
+  for (uint32_t j = 0; j < 8; ++j) {
+    cq[j] = ca[j] = cx[j] = cy[j] = cz[j] = 0.0;
+    if (j < nc) {
+      const NodeLink cl = u.link[c0 + j];
+      count += cl.count;
+      if (cl.nchild == 0) {
+        leafmask |= 1u << j;
+        const uint32_t p = cl.first;
+        cx[j] = u.x[p]; cy[j] = u.y[p]; cz[j] = u.z[p]; cq[j] = u.q[p];
+        ca[j] = fabs(cq[j]);
+      } else {
+        const double2* r = reinterpret_cast<const double2*>(u.mom + (int64_t)(c0 + j) * 16);
+        const double2 r0 = r[0], r1 = r[1], r2 = r[2];
+        cx[j] = r0.x; cy[j] = r0.y; cz[j] = r1.x; cq[j] = r1.y; ca[j] = r2.x;
       }
     }
-    return mom[(int64_t)cidx[j] * 16 + slot];
-  };
+  }
   double charge = 0.0, absq = 0.0;
-  for (uint32_t j = 0; j < nc; ++j) charge = charge + val(j, M_Q);
-  for (uint32_t j = 0; j < nc; ++j) absq = absq + val(j, M_AQ);
+#pragma unroll
+  for (uint32_t j = 0; j < 8; ++j)
+    if (j < nc) charge = charge + cq[j];
+#pragma unroll
+  for (uint32_t j = 0; j < 8; ++j)
+    if (j < nc) absq = absq + ca[j];
   double coc[3] = {0.0, 0.0, 0.0};
   if (absq != 0.0) {
-    for (uint32_t j = 0; j < nc; ++j) {
-      const double aq = val(j, M_AQ);
-      for (int d = 0; d < 3; ++d) coc[d] = coc[d] + (val(j, M_CX + d) * aq);
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) {
+      if (j < nc) {
+        coc[0] = coc[0] + (cx[j] * ca[j]);
+        coc[1] = coc[1] + (cy[j] * ca[j]);
+        coc[2] = coc[2] + (cz[j] * ca[j]);
+      }
     }
     for (int d = 0; d < 3; ++d) coc[d] = coc[d] / absq;
   } else {
-    for (uint32_t j = 0; j < nc; ++j)
-      for (int d = 0; d < 3; ++d) coc[d] = coc[d] + val(j, M_CX + d);
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) {
+      if (j < nc) {
+        coc[0] = coc[0] + cx[j];
+        coc[1] = coc[1] + cy[j];
+        coc[2] = coc[2] + cz[j];
+      }
+    }
     for (int d = 0; d < 3; ++d) coc[d] = coc[d] / (double)nc;
   }
   double dip[3] = {0.0, 0.0, 0.0}, quad[3] = {0.0, 0.0, 0.0}, xy = 0.0, yz = 0.0, zx = 0.0;
   double bmax = 0.0;
-  for (uint32_t j = 0; j < nc; ++j) {
+#pragma unroll
+  for (uint32_t j = 0; j < 8; ++j) {
+    if (j >= nc) continue;
+    // higher moments of the child: zero for a leaf, record slots 5..14 else
     double c[15];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) c[k] = val(j, k);
+    for (int k = 0; k < 15; ++k) c[k] = 0.0;
+    c[M_CX] = cx[j]; c[M_CY] = cy[j]; c[M_CZ] = cz[j]; c[M_Q] = cq[j]; c[M_AQ] = ca[j];
+    if (!(leafmask & (1u << j))) {
+      const double2* r = reinterpret_cast<const double2*>(u.mom + (int64_t)(c0 + j) * 16);
+      const double2 r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6], r7 = r[7];
+      c[M_PX] = r2.y; c[M_PY] = r3.x; c[M_PZ] = r3.y; c[M_QX] = r4.x; c[M_QY] = r4.y;
+      c[M_QZ] = r5.x; c[M_XY] = r5.y; c[M_YZ] = r6.x; c[M_ZX] = r6.y; c[M_BMAX] = r7.x;
+    }
     double s[3];
     for (int d = 0; d < 3; ++d) s[d] = coc[d] - c[M_CX + d];
     for (int d = 0; d < 3; ++d) dip[d] = dip[d] + c[M_PX + d] + c[M_Q] * s[d];
