@@ -15,8 +15,10 @@ void launch_box_batch(hipStream_t s, int nsys, const int64_t* soff, const double
                       const double* y, const double* z, int force_cubic, DevBox* boxes,
                       uint32_t* sys, DevScalars* sc);
 // sys == nullptr: one system with boxes[0].
+// Also writes rec[i] = {x, y, z, q} (input order) for launch_gather_rec.
 void launch_keys(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
-                 const DevBox* boxes, const uint32_t* sys, unsigned long long* keys, uint32_t* idx);
+                 const double* q, const DevBox* boxes, const uint32_t* sys,
+                 unsigned long long* keys, uint32_t* idx, double4* rec);
 
 // sort.hip
 int64_t scan_blocks(int64_t n);
@@ -33,6 +35,9 @@ void launch_radix_pass_dev(hipStream_t s, const unsigned long long* kin, const u
 void launch_sort_keys(hipStream_t s, unsigned long long* k0, uint32_t* v0, unsigned long long* k1,
                       uint32_t* v1, int64_t n, uint32_t* hist, uint32_t* bsum,
                       const uint32_t* sysid, int64_t nsys);
+// Sorted SoA particles from the 32-byte records: xs[i] = rec[perm[i]].x, ...
+void launch_gather_rec(hipStream_t s, int64_t n, const uint32_t* perm, const double4* rec,
+                       double* xs, double* ys, double* zs, double* qs);
 void launch_gather4(hipStream_t s, int64_t n, const uint32_t* perm, const double* x, const double* y,
                     const double* z, const double* q, double* xs, double* ys, double* zs,
                     double* qs);

@@ -169,12 +169,17 @@ __device__ __forceinline__ unsigned long long hilbert3d(uint32_t c0, uint32_t c1
 __global__ __launch_bounds__(256) void keys_kernel(int64_t n, const double* __restrict__ x,
                                                    const double* __restrict__ y,
                                                    const double* __restrict__ z,
+                                                   const double* __restrict__ q,
                                                    const DevBox* __restrict__ boxes,
                                                    const uint32_t* __restrict__ sys,
                                                    unsigned long long* __restrict__ keys,
-                                                   uint32_t* __restrict__ idx) {
+                                                   uint32_t* __restrict__ idx,
+                                                   double4* __restrict__ rec) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
+  // one 32-byte record per particle, so the gather after the sort fetches a
+  // particle with one random access instead of four
+  rec[i] = make_double4(x[i], y[i], z[i], q[i]);
   const DevBox* __restrict__ box = boxes + (sys ? sys[i] : 0u);
   // int((x - boxmin) / s): the FP64 quotient is correctly rounded and
   // truncated toward zero, exactly as the Fortran int() conversion.
@@ -206,8 +211,9 @@ void launch_box_batch(hipStream_t s, int nsys, const int64_t* soff, const double
 }
 
 void launch_keys(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
-                 const DevBox* boxes, const uint32_t* sys, unsigned long long* keys, uint32_t* idx) {
-  keys_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, x, y, z, boxes, sys, keys, idx);
+                 const double* q, const DevBox* boxes, const uint32_t* sys,
+                 unsigned long long* keys, uint32_t* idx, double4* rec) {
+  keys_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, x, y, z, q, boxes, sys, keys, idx, rec);
 }
 
 }  // namespace pepc

@@ -33,6 +33,7 @@ struct Ctx {
   unsigned long long *k0 = nullptr, *k1 = nullptr, *oldkeys = nullptr;
   uint32_t *v0 = nullptr, *v1 = nullptr;
   double *xs = nullptr, *ys = nullptr, *zs = nullptr, *qs = nullptr;
+  double4* prec4 = nullptr;         // {x, y, z, q} records, input order
   uint8_t *lo = nullptr, *hi = nullptr;
   uint32_t *cnt = nullptr, *gmark = nullptr, *dflag = nullptr, *dlist = nullptr;
   long long* kd = nullptr;
@@ -138,7 +139,7 @@ void free_all(Ctx* c) {
                   c->mom64, c->wm32, c->wm64, c->hist, c->bsum, c->partial, c->box, c->sc,
                   c->sys, c->soff, c->sysroot, c->label, c->tlabel, c->ehist, c->ecount,
                   c->nadj, c->parent, c->leafof, c->aqueue, c->lcount, c->loff, c->lent,
-                  c->ltotal, c->nparent, c->echunk, c->esyschunk, c->epartial};
+                  c->ltotal, c->nparent, c->echunk, c->esyschunk, c->epartial, c->prec4};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (double* p : c->st)
@@ -164,6 +165,7 @@ int ensure_particles(Ctx* c, int64_t n) {
   ALLOC(c, c->k0, nc); ALLOC(c, c->k1, nc); ALLOC(c, c->oldkeys, nc);
   ALLOC(c, c->v0, nc); ALLOC(c, c->v1, nc);
   ALLOC(c, c->xs, nc); ALLOC(c, c->ys, nc); ALLOC(c, c->zs, nc); ALLOC(c, c->qs, nc);
+  ALLOC(c, c->prec4, nc);
   ALLOC(c, c->lo, nc); ALLOC(c, c->hi, nc);
   ALLOC(c, c->cnt, nc); ALLOC(c, c->gmark, nc); ALLOC(c, c->dflag, nc); ALLOC(c, c->dlist, nc);
   ALLOC(c, c->kd, nc);
@@ -288,9 +290,9 @@ int grow_device_impl(Ctx* c, int64_t n, const double* x, const double* y, const 
     launch_box_batch(s, (int)nsys, c->soff, x, y, z, c->cfg.force_cubic_domain, c->box, c->sys, c->sc);
   else
     launch_box(s, n, x, y, z, c->cfg.force_cubic_domain, c->partial, c->box, c->sc);
-  launch_keys(s, n, x, y, z, c->box, sys, c->k0, c->v0);
+  launch_keys(s, n, x, y, z, q, c->box, sys, c->k0, c->v0, c->prec4);
   launch_sort_keys(s, c->k0, c->v0, c->k1, c->v1, n, c->hist, c->bsum, sys, nsys);
-  launch_gather4(s, n, c->v0, x, y, z, q, c->xs, c->ys, c->zs, c->qs);
+  launch_gather_rec(s, n, c->v0, c->prec4, c->xs, c->ys, c->zs, c->qs);
   launch_dup_detect(s, n, c->k0, sys, c->sc);
   launch_dupfix(s, n, c->k0, c->xs, c->ys, c->zs, c->box, sys, nsys > 1 ? c->soff : nullptr, c->kd,
                 c->oldkeys, c->dflag, c->dlist, c->bsum, c->sc);

@@ -259,6 +259,19 @@ void launch_copy_pairs(hipStream_t s, const unsigned long long* kin, const uint3
   copy_pairs_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, kin, vin, kout, vout);
 }
 
+__global__ __launch_bounds__(256) void gather_rec_kernel(int64_t n, const uint32_t* __restrict__ perm,
+                                                        const double4* __restrict__ rec,
+                                                        double* __restrict__ xs, double* __restrict__ ys,
+                                                        double* __restrict__ zs, double* __restrict__ qs) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 r = rec[perm[i]];
+  xs[i] = r.x;
+  ys[i] = r.y;
+  zs[i] = r.z;
+  qs[i] = r.w;
+}
+
 }  // namespace
 
 int64_t scan_blocks(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
@@ -319,6 +332,11 @@ void launch_sort_keys(hipStream_t s, unsigned long long* k0, uint32_t* v0, unsig
       launch_copy_pairs(s, k1, v1, k0, v0, n);
     }
   }
+}
+
+void launch_gather_rec(hipStream_t s, int64_t n, const uint32_t* perm, const double4* rec,
+                       double* xs, double* ys, double* zs, double* qs) {
+  gather_rec_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, perm, rec, xs, ys, zs, qs);
 }
 
 void launch_gather4(hipStream_t s, int64_t n, const uint32_t* perm, const double* x, const double* y,
