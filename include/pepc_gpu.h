@@ -137,6 +137,14 @@ int pepc_gpu_sorted_device(void *ctx, const uint64_t **keys, const uint32_t **pe
                            const double **x, const double **y, const double **z,
                            const double **q);
 
+/* Replaces pepc_restore_particles (module_pepc.f90:525-533 -> domain_restore,
+ * module_domains.f90:372-408) for per-particle device arrays: for each of
+ * the count arrays, out[k][perm[i]] = in[k][i], i.e. sorted order back to the
+ * grow's input order (e.g. traverse results).  Arrays of the grown size, not
+ * in place.  Asynchronous on the context's stream. */
+int pepc_gpu_restore_device(void *ctx, int64_t count, const double *const *in,
+                            double *const *out);
+
 /* Host copies of the sorted particle arrays of the last grow (each may be
  * NULL): keys[n], perm[n] (input index of sorted particle i), x,y,z,q[n]. */
 int pepc_gpu_read_sorted(void *ctx, uint64_t *keys, int64_t *perm, double *x, double *y,

@@ -27,7 +27,7 @@ EXPORTS = [
     "pepc_gpu_timings", "pepc_gpu_grow_batch_device", "pepc_gpu_state_set", "pepc_gpu_run",
     "pepc_gpu_state_get", "pepc_gpu_state_device", "pepc_gpu_read_energies",
     "pepc_gpu_adjoint_device", "pepc_gpu_autotune", "pepc_gpu_get_config", "pepc_gpu_freeze",
-    "pepc_gpu_replay_device",
+    "pepc_gpu_replay_device", "pepc_gpu_restore_device",
 ]
 
 RUN_GRAPH = 1
@@ -114,6 +114,7 @@ def load(path: str = LIB_PATH):
                                     C.POINTER(TuneEntry), _i64, C.POINTER(_i64)]
     L.pepc_gpu_get_config.argtypes = [_vp, C.POINTER(Config)]
     L.pepc_gpu_freeze.argtypes = [_vp, C.POINTER(_i64)]
+    L.pepc_gpu_restore_device.argtypes = [_vp, _i64, C.POINTER(_vp), C.POINTER(_vp)]
     L.pepc_gpu_replay_device.argtypes = [_vp] * 9
     for name in EXPORTS:
         f = getattr(L, name)

@@ -35,6 +35,9 @@ void launch_radix_pass_dev(hipStream_t s, const unsigned long long* kin, const u
 void launch_sort_keys(hipStream_t s, unsigned long long* k0, uint32_t* v0, unsigned long long* k1,
                       uint32_t* v1, int64_t n, uint32_t* hist, uint32_t* bsum,
                       const uint32_t* sysid, int64_t nsys);
+// out[perm[i]] = in[i] (back to input order)
+void launch_scatter_back(hipStream_t s, int64_t n, const uint32_t* perm, const double* in,
+                         double* out);
 // Sorted SoA particles from the 32-byte records: xs[i] = rec[perm[i]].x, ...
 void launch_gather_rec(hipStream_t s, int64_t n, const uint32_t* perm, const double4* rec,
                        double* xs, double* ys, double* zs, double* qs);

@@ -578,6 +578,21 @@ int pepc_gpu_sorted_device(void* p, const uint64_t** keys, const uint32_t** perm
   return 0;
 }
 
+int pepc_gpu_restore_device(void* p, int64_t count, const double* const* in, double* const* out) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PEPC_GPU_EINVAL;
+  if (!c->grown) return fail(c, PEPC_GPU_EINVAL, "restore: no tree (call grow first)");
+  if (count < 0 || (count > 0 && (!in || !out))) return fail(c, PEPC_GPU_EINVAL, "restore: bad arrays");
+  for (int64_t k = 0; k < count; ++k) {
+    if (!in[k] || !out[k] || in[k] == out[k])
+      return fail(c, PEPC_GPU_EINVAL, "restore: null or in-place array");
+  }
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  for (int64_t k = 0; k < count; ++k) launch_scatter_back(c->stream, c->n, c->v0, in[k], out[k]);
+  HIPCHECK(c, hipGetLastError());
+  return 0;
+}
+
 int pepc_gpu_read_sorted(void* p, uint64_t* keys, int64_t* perm, double* x, double* y, double* z,
                          double* q) {
   Ctx* c = as_ctx(p);

@@ -272,6 +272,15 @@ __global__ __launch_bounds__(256) void gather_rec_kernel(int64_t n, const uint32
   qs[i] = r.w;
 }
 
+// Inverse of the grow permutation: out[perm[i]] = in[i].
+__global__ __launch_bounds__(256) void scatter_back_kernel(int64_t n, const uint32_t* __restrict__ perm,
+                                                          const double* __restrict__ in,
+                                                          double* __restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[perm[i]] = in[i];
+}
+
 }  // namespace
 
 int64_t scan_blocks(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
@@ -332,6 +341,11 @@ void launch_sort_keys(hipStream_t s, unsigned long long* k0, uint32_t* v0, unsig
       launch_copy_pairs(s, k1, v1, k0, v0, n);
     }
   }
+}
+
+void launch_scatter_back(hipStream_t s, int64_t n, const uint32_t* perm, const double* in,
+                         double* out) {
+  scatter_back_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, perm, in, out);
 }
 
 void launch_gather_rec(hipStream_t s, int64_t n, const uint32_t* perm, const double4* rec,

@@ -209,6 +209,23 @@ class Solver:
             self.sync()
         return out
 
+    def restore(self, *arrays):
+        """Device arrays in sorted order (e.g. traverse results) back to the
+        input order of the last grow; returns new tensors."""
+        t = self.torch
+        ins = [self._dev(a).contiguous() for a in arrays]
+        if any(a.numel() != self.n for a in ins):
+            raise ValueError("restore: arrays must have the grown size")
+        outs = [t.empty_like(a) for a in ins]
+        k = len(ins)
+        pin = (C.c_void_p * k)(*[a.data_ptr() for a in ins])
+        pout = (C.c_void_p * k)(*[a.data_ptr() for a in outs])
+        self._bind_stream()
+        self._check(self.lib.pepc_gpu_restore_device(self.h, k, pin, pout))
+        self._keep_restore = ins
+        t.cuda.synchronize(self.device)
+        return outs
+
     # -------------------------------------------------------------- timestep
     def state_set(self, x, y, z, v, q, m, offsets=None, max_records=1024):
         """Loads particle state (host arrays, input order) for the on-device
