@@ -28,6 +28,8 @@ void launch_scan(hipStream_t s, const uint32_t* in, uint32_t* out, int64_t n, ui
 void launch_radix_pass(hipStream_t s, const unsigned long long* kin, const uint32_t* vin,
                        unsigned long long* kout, uint32_t* vout, int64_t n, int shift, int nbits,
                        uint32_t* hist, uint32_t* bsum);
+// One stable pass; afterwards bsum[0..255] holds the element count of each
+// digit value.
 void launch_radix_pass_dev(hipStream_t s, const unsigned long long* kin, const uint32_t* vin,
                            unsigned long long* kout, uint32_t* vout, int64_t n, const uint32_t* ndev,
                            int shift, int nbits, uint32_t* hist, uint32_t* bsum,

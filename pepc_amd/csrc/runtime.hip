@@ -183,7 +183,8 @@ int ensure_particles(Ctx* c, int64_t n) {
   // radix histogram: 256 digits x sort blocks of the largest array sorted
   const int64_t hc = 256 * sort_blocks(mc > nc ? mc : nc);
   ALLOC(c, c->hist, hc);
-  const int64_t bc = scan_blocks(hc > mc ? hc : mc) + 64;
+  int64_t bc = scan_blocks(hc > mc ? hc : mc) + 64;
+  if (bc < 512) bc = 512;   // also holds the 256 digit totals of a radix pass
   ALLOC(c, c->bsum, bc);
   c->ncap = nc;
   c->mcap = mc;

@@ -148,12 +148,13 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     unsigned long long* __restrict__ kout, uint32_t* __restrict__ vout,
     const uint32_t* __restrict__ dsrc, int64_t n,
     const uint32_t* __restrict__ ndev, int shift, int nbits, const uint32_t* __restrict__ hscan,
-    int nblocks) {
+    const uint32_t* __restrict__ dtot, int nblocks) {
   if (ndev && (int64_t)*ndev < n) n = *ndev;
   const int64_t tile0 = (int64_t)blockIdx.x * kSortTile;
   if (tile0 >= n) return;
   __shared__ uint32_t wcnt[kSortWaves][kRadix];
   __shared__ uint32_t lbase[kRadix];
+  __shared__ uint32_t gbase[kRadix];
   __shared__ unsigned long long skey[kSortTile];
   __shared__ uint32_t sval[kSortTile];
   __shared__ uint8_t sdig[kSortTile];
@@ -204,6 +205,8 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     }
     uint32_t tot;
     lbase[d] = block_exclusive_scan(run, &tot);
+    // global start of digit d: digits below d over all blocks
+    gbase[d] = block_exclusive_scan(dtot[d], &tot);
   }
   __syncthreads();
 #pragma unroll
@@ -221,10 +224,35 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   const int cnt = rem < kSortTile ? (int)rem : kSortTile;
   for (int i = threadIdx.x; i < cnt; i += kSortThreads) {
     const uint32_t d = sdig[i];
-    const uint32_t pos = hscan[(int64_t)d * nblocks + blockIdx.x] + ((uint32_t)i - lbase[d]);
+    const uint32_t pos =
+        gbase[d] + hscan[(int64_t)d * nblocks + blockIdx.x] + ((uint32_t)i - lbase[d]);
     kout[pos] = skey[i];
     vout[pos] = sval[i];
   }
+}
+
+// Exclusive scan of one digit's row of the block histogram (block b's count
+// of digit d -> number of digit-d elements in blocks before b), row total to
+// dtot[d].  One block per digit.
+__global__ __launch_bounds__(kSortThreads) void digit_scan_kernel(uint32_t* __restrict__ hist,
+                                                                 int nblocks,
+                                                                 uint32_t* __restrict__ dtot) {
+  uint32_t* row = hist + (int64_t)blockIdx.x * nblocks;
+  const int per = (nblocks + kSortThreads - 1) / kSortThreads;
+  const int b0 = threadIdx.x * per;
+  uint32_t sum = 0;
+  for (int k = 0; k < per; ++k)
+    if (b0 + k < nblocks) sum += row[b0 + k];
+  uint32_t tot;
+  uint32_t run = block_exclusive_scan(sum, &tot);
+  for (int k = 0; k < per; ++k) {
+    if (b0 + k < nblocks) {
+      const uint32_t t = row[b0 + k];
+      row[b0 + k] = run;
+      run += t;
+    }
+  }
+  if (threadIdx.x == 0) dtot[blockIdx.x] = tot;
 }
 
 __global__ __launch_bounds__(256) void gather4_kernel(int64_t n, const uint32_t* __restrict__ perm,
@@ -305,9 +333,10 @@ void launch_radix_pass_dev(hipStream_t s, const unsigned long long* kin, const u
   const int nb = (int)sort_blocks(n);
   radix_hist_kernel<<<nb, kSortThreads, 0, s>>>(kin, vin, dsrc, n, ndev, shift, (1u << nbits) - 1u,
                                                 hist, nb);
-  launch_scan(s, hist, hist, (int64_t)kRadix * nb, bsum, nullptr);
+  // bsum[0..255] holds the digit totals
+  digit_scan_kernel<<<kRadix, kSortThreads, 0, s>>>(hist, nb, bsum);
   radix_scatter_kernel<<<nb, kSortThreads, 0, s>>>(kin, vin, kout, vout, dsrc, n, ndev, shift, nbits,
-                                                   hist, nb);
+                                                   hist, bsum, nb);
 }
 
 void launch_radix_pass(hipStream_t s, const unsigned long long* kin, const uint32_t* vin,

@@ -225,13 +225,16 @@ __global__ void dfs_emit_kernel(int64_t n, int64_t cap, const uint8_t* __restric
   }
 }
 
-__global__ void level_offsets_kernel(const uint32_t* __restrict__ hist, int nblocks,
+// First node of every level from the level pass's digit totals.
+__global__ void level_offsets_kernel(const uint32_t* __restrict__ dtot,
                                      DevScalars* __restrict__ sc) {
   const int l = threadIdx.x;
   const bool broken = tree_broken(sc);
   __syncthreads();
   if (l <= kMaxLevel + 1) {
-    uint32_t off = (l <= kMaxLevel) ? hist[(int64_t)l * nblocks] : sc->nnodes;
+    uint32_t off = 0;
+    for (int k = 0; k < l; ++k) off += dtot[k];
+    if (l == kMaxLevel + 1) off = sc->nnodes;
     sc->level_offset[l] = broken ? 0u : off;
   }
 }
@@ -601,7 +604,8 @@ void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size, int 
   // one stable pass keyed on the level; entries beyond nnodes are never read
   launch_radix_pass_dev(s, t.dkey[0], t.dval[0], t.dkey[1], t.dval[1], cap, &t.sc->nnodes, 0, 5,
                         t.hist, t.bsum, nullptr);
-  level_offsets_kernel<<<1, 64, 0, s>>>(t.hist, (int)sort_blocks(cap), t.sc);
+  // the level pass leaves the per-level node counts in bsum[0..31]
+  level_offsets_kernel<<<1, 64, 0, s>>>(t.bsum, t.sc);
   inv_kernel<<<grid_for(cap), kT, 0, s>>>(t.dval[1], t.inv, t.sc);
   node_fill_kernel<<<grid_for(cap), kT, 0, s>>>(cap, t.dkey[1], t.dval[1], t.dpart, t.inv, t.hi,
                                                 t.x, t.y, t.z, t.q, t.geom, t.link, t.nfirst,
