@@ -274,6 +274,57 @@ int pepc_gpu_state_device(void *ctx, double **arrays, const int64_t **label);
  * not kept). */
 int pepc_gpu_read_energies(void *ctx, int64_t cap, double *out, int64_t *nrec);
 
+/* ---------------------------------------------------------------------------
+ * Periodic boundaries.  The reference's settings: periodicity and
+ * mirror_box_layers of the &libpepc namelist (module_libpepc_main.f90:44),
+ * the lattice t_lattice_1..3 and LatticeOrigin (module_mirror_boxes.f90:
+ * 30-36; neighbour_boxes_prepare :97-153), include_far_field_if_periodic of
+ * &calc_force_coulomb (coulomb/module_interaction_specific.f90:43).
+ *
+ * With any periodic axis, traverse walks the tree once per neighbour box
+ * (lattice vectors within mirror_box_layers shells of the periodic axes, the
+ * reference's box order; module_libpepc_main.f90:184-187) and, with
+ * far_field, adds the lattice contribution of everything beyond
+ * (fmm_framework_timestep + fmm_sum_lattice_force, module_fmm_framework.f90:
+ * 132-149, 766-833: lattice coefficients to order 40, central-box multipoles
+ * to order 20, fictitious-charge extrinsic correction).  A unit-cube lattice
+ * uses the reference's pretabulated coefficients (:114-121, :260-508); other
+ * lattices compute them (calc_lattice_coefficients, :154-213) on the device
+ * in set_periodic.  The on-device timestep wraps positions back into the box
+ * after each drift (constrain_periodic, module_mirror_boxes.f90:174-210).
+ *
+ * Limits: far_field needs all three axes periodic and mirror_box_layers = 1;
+ * near field alone takes mirror_box_layers 1..3.  Not with batched systems,
+ * the adjoint or frozen lists (EINVAL).  Particles must lie in the central
+ * box (the reference warns, check_lattice_boundaries :261-281). */
+typedef struct {
+    int32_t periodicity[3];       /* nonzero = periodic along t_lattice_k */
+    int32_t mirror_box_layers;    /* default 1 */
+    double t_lattice[9];          /* rows: t_lattice_1, t_lattice_2, t_lattice_3 */
+    double origin[3];             /* LatticeOrigin */
+    int32_t far_field;            /* include_far_field_if_periodic (default 1) */
+    int32_t pad;
+} pepc_gpu_periodic;
+
+/* Fills *p with the reference defaults: unit cube at the origin, no axis
+ * periodic, one layer, far field on. */
+void pepc_gpu_periodic_default(pepc_gpu_periodic *p);
+
+/* Sets (p != NULL) or clears (p == NULL, or no periodic axis) periodic
+ * boundaries for the following traverses.  Synchronous. */
+int pepc_gpu_set_periodic(void *ctx, const pepc_gpu_periodic *p);
+
+/* Host copies of the far-field state (each may be NULL): ml[2*861] lattice
+ * coefficients (interleaved re, im; l <= 40 at l(l+1)/2 + m), and after a
+ * traverse mu[2*231] the lattice's local expansion at the box centre,
+ * fict[16] the fictitious charges {x, y, z, q} x 4 and *nfict their count. */
+int pepc_gpu_lattice_get(void *ctx, double *ml, double *mu, double *fict, int32_t *nfict);
+
+/* constrain_periodic (module_mirror_boxes.f90:174-210) on device arrays:
+ * positions of the periodic axes wrapped into the central box.
+ * Asynchronous on the context's stream. */
+int pepc_gpu_constrain_periodic_device(void *ctx, int64_t n, double *x, double *y, double *z);
+
 #ifdef __cplusplus
 }
 #endif

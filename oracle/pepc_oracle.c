@@ -544,6 +544,7 @@ typedef struct {
     const double *x, *y, *z;
     const int32_t *nlevel; const int64_t *nleaves, *nfc, *nns; const double *mom;
     const double *b2; double theta2, eps2, sign;
+    double vb[3];             /* lattice vector of the box walked (vbox) */
     int64_t t0, np;
     double e[64][3], pot[64], work[64]; int32_t p2p[64], m2p[64];
     int64_t num_int, num_mac, ni;
@@ -557,9 +558,9 @@ static void walk_aux(wk_ctx *w, int64_t node)
         w->ni += 1;
         for (int64_t ip = 0; ip < w->np; ++ip) {
             int64_t p = w->t0 + ip;
-            d[ip][0] = w->x[p] - m[OR_COCX];
-            d[ip][1] = w->y[p] - m[OR_COCY];
-            d[ip][2] = w->z[p] - m[OR_COCZ];
+            d[ip][0] = (w->x[p] - w->vb[0]) - m[OR_COCX];
+            d[ip][1] = (w->y[p] - w->vb[1]) - m[OR_COCY];
+            d[ip][2] = (w->z[p] - w->vb[2]) - m[OR_COCZ];
             d2[ip] = d[ip][0] * d[ip][0] + d[ip][1] * d[ip][1] + d[ip][2] * d[ip][2];
             w->num_int++;
             w->work[ip] += 1.0;
@@ -578,9 +579,9 @@ static void walk_aux(wk_ctx *w, int64_t node)
     for (int64_t ip = 0; ip < w->np; ++ip) {
         int64_t p = w->t0 + ip;
         w->num_mac++;
-        d[ip][0] = w->x[p] - m[OR_COCX];
-        d[ip][1] = w->y[p] - m[OR_COCY];
-        d[ip][2] = w->z[p] - m[OR_COCZ];
+        d[ip][0] = (w->x[p] - w->vb[0]) - m[OR_COCX];
+        d[ip][1] = (w->y[p] - w->vb[1]) - m[OR_COCY];
+        d[ip][2] = (w->z[p] - w->vb[2]) - m[OR_COCZ];
         d2[ip] = d[ip][0] * d[ip][0] + d[ip][1] * d[ip][1] + d[ip][2] * d[ip][2];
         if (!(w->theta2 * d2[ip] > w->b2[w->nlevel[node]])) {
             for (int64_t c = w->nfc[node]; c >= 0; c = w->nns[c]) walk_aux(w, c);
@@ -612,6 +613,27 @@ int or_walk_group(int64_t n, const double *x, const double *y, const double *z,
                   int32_t *cnt_p2p, int32_t *cnt_m2p, int64_t *tile_mac,
                   int64_t *totals)
 {
+    static const double central[3] = {0.0, 0.0, 0.0};
+    return or_walk_group_boxes(n, x, y, z, nnodes, nlevel, nleaves, nfirst_child, nnext_sibling,
+                               nmom, b2, theta2, eps2, sign, ntiles, tile_first, tile_count,
+                               tile_stride, nthreads, 1, central, ex, ey, ez, pot, work, cnt_p2p,
+                               cnt_m2p, tile_mac, totals);
+}
+
+/* Walks over a list of boxes (module_libpepc_main.f90:184-187: one
+ * tree_walk_run per neighbour box, results summed): every tile walks the tree
+ * once per lattice vector, with the targets shifted by -vbox. */
+int or_walk_group_boxes(int64_t n, const double *x, const double *y, const double *z,
+                        int64_t nnodes, const int32_t *nlevel, const int64_t *nleaves,
+                        const int64_t *nfirst_child, const int64_t *nnext_sibling,
+                        const double *nmom, const double b2[OR_MAXLEVEL + 1],
+                        double theta2, double eps2, double sign,
+                        int64_t ntiles, const int64_t *tile_first, const int64_t *tile_count,
+                        int64_t tile_stride, int nthreads, int64_t nbox, const double *vbox,
+                        double *ex, double *ey, double *ez, double *pot, double *work,
+                        int32_t *cnt_p2p, int32_t *cnt_m2p, int64_t *tile_mac,
+                        int64_t *totals)
+{
     (void)nnodes;
     if (tile_stride < 1) tile_stride = 1;
     int64_t tot_int = 0, tot_mac = 0, tot_tiles = 0, bad = 0;
@@ -634,8 +656,12 @@ int or_walk_group(int64_t n, const double *x, const double *y, const double *z,
             w->work[ip] = 1.0; /* module_domains.f90:318 resets work to 1 */
             w->p2p[ip] = 0; w->m2p[ip] = 0;
         }
-        walk_aux(w, 0);
-        if (w->ni != n) bad++; /* module_tree_walk_simple.f90:287 */
+        for (int64_t b = 0; b < nbox; ++b) {
+            w->vb[0] = vbox[3 * b]; w->vb[1] = vbox[3 * b + 1]; w->vb[2] = vbox[3 * b + 2];
+            w->ni = 0;
+            walk_aux(w, 0);
+            if (w->ni != n) bad++; /* module_tree_walk_simple.f90:287 */
+        }
         for (int64_t ip = 0; ip < w->np; ++ip) {
             int64_t p = w->t0 + ip;
             ex[p] = w->e[ip][0]; ey[p] = w->e[ip][1]; ez[p] = w->e[ip][2]; pot[p] = w->pot[ip];

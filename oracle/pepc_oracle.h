@@ -33,6 +33,15 @@
  *   or_adjoint_term/_lists  central finite differences of or_eval_lists with
  *                           frozen lists (h = 1e-5); at theta = 0 the
  *                           closed-form gradient of the pair sum (numpy)
+ *   or_neighbour_boxes      hand-listed order of module_mirror_boxes.f90:119-149
+ *   or_walk_group_boxes     one box = or_walk_group bit for bit; theta = 0 over
+ *                           27 boxes = 3x3x3 image sum (numpy)
+ *   or_lattice_coefficients the reference's pretabulated unit-cube table
+ *                           (tests/golden/lattice_unit_cube.json); scaling
+ *                           law ML(a) = ML(1) / a^(l+1) for a cube of side a
+ *   or_lattice_timestep/    near field + lattice field against an Ewald sum
+ *   or_lattice_field        (numpy, tin-foil boundary) for random neutral
+ *                           systems; rock-salt Madelung constant
  */
 #ifndef PEPC_ORACLE_H
 #define PEPC_ORACLE_H
@@ -105,6 +114,17 @@ int or_walk_group(int64_t n, const double *x, const double *y, const double *z,
                   int32_t *cnt_p2p, int32_t *cnt_m2p, int64_t *tile_mac,
                   int64_t *totals);
 
+int or_walk_group_boxes(int64_t n, const double *x, const double *y, const double *z,
+                        int64_t nnodes, const int32_t *nlevel, const int64_t *nleaves,
+                        const int64_t *nfirst_child, const int64_t *nnext_sibling,
+                        const double *nmom, const double b2[OR_MAXLEVEL + 1],
+                        double theta2, double eps2, double sign,
+                        int64_t ntiles, const int64_t *tile_first, const int64_t *tile_count,
+                        int64_t tile_stride, int nthreads, int64_t nbox, const double *vbox,
+                        double *ex, double *ey, double *ez, double *pot, double *work,
+                        int32_t *cnt_p2p, int32_t *cnt_m2p, int64_t *tile_mac,
+                        int64_t *totals);
+
 /* ---- frozen lists and the adjoint (no reference counterpart) --------------- */
 int64_t or_walk_lists(const double *x, const double *y, const double *z,
                       const int32_t *nlevel, const int64_t *nfirst_child,
@@ -131,6 +151,24 @@ void or_adjoint_lists(int64_t n, const double *x, const double *y, const double 
                       const int64_t *list_node, const double *gex, const double *gey,
                       const double *gez, const double *gpot, double *gx, double *gy, double *gz,
                       double *gq);
+
+/* ---- periodic boundaries (pepc_oracle_periodic.c) -------------------------- */
+#define OR_LMAX_MULT 20          /* Lmax_multipole, module_fmm_framework.f90:70 */
+#define OR_LMAX_TAYLOR 40        /* Lmax_taylor, :71 */
+#define OR_NMULT 231             /* (LM+1)(LM+2)/2 */
+#define OR_NTAYLOR 861           /* (LT+1)(LT+2)/2 */
+int  or_neighbour_boxes(const int32_t per[3], int32_t layers, int32_t *boxes);
+void or_lattice_vect(const double lat[9], const int32_t ijk[3], double v[3]);
+int  or_lattice_coefficients(const double lat[9], const int32_t per[3], int32_t layers,
+                             int32_t maxiter, double *ml);
+void or_lattice_timestep(const double lat[9], const double origin[3], const int32_t per[3],
+                         const double *ml, int64_t n, const double *x, const double *y,
+                         const double *z, const double *q, double *omega, double *mu,
+                         double *fict, int32_t *nfict, double dipole[3]);
+void or_lattice_field(const double lat[9], const double origin[3], const int32_t per[3],
+                      int32_t layers, const double *mu, const double *fict, int32_t nfict,
+                      int64_t n, const double *x, const double *y, const double *z,
+                      double *ex, double *ey, double *ez, double *pot);
 
 /* ---- kernels and direct sum ------------------------------------------------ */
 void or_p2p(const double *t_charge, const double d[3], double dist2,

@@ -27,7 +27,8 @@ EXPORTS = [
     "pepc_gpu_timings", "pepc_gpu_grow_batch_device", "pepc_gpu_state_set", "pepc_gpu_run",
     "pepc_gpu_state_get", "pepc_gpu_state_device", "pepc_gpu_read_energies",
     "pepc_gpu_adjoint_device", "pepc_gpu_autotune", "pepc_gpu_get_config", "pepc_gpu_freeze",
-    "pepc_gpu_replay_device", "pepc_gpu_restore_device",
+    "pepc_gpu_replay_device", "pepc_gpu_restore_device", "pepc_gpu_periodic_default",
+    "pepc_gpu_set_periodic", "pepc_gpu_lattice_get", "pepc_gpu_constrain_periodic_device",
 ]
 
 RUN_GRAPH = 1
@@ -61,6 +62,12 @@ class TuneEntry(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Periodic(C.Structure):
+    _fields_ = [("periodicity", C.c_int32 * 3), ("mirror_box_layers", C.c_int32),
+                ("t_lattice", C.c_double * 9), ("origin", C.c_double * 3),
+                ("far_field", C.c_int32), ("pad", C.c_int32)]
 
 
 class PepcGpuError(RuntimeError):
@@ -116,9 +123,15 @@ def load(path: str = LIB_PATH):
     L.pepc_gpu_freeze.argtypes = [_vp, C.POINTER(_i64)]
     L.pepc_gpu_restore_device.argtypes = [_vp, _i64, C.POINTER(_vp), C.POINTER(_vp)]
     L.pepc_gpu_replay_device.argtypes = [_vp] * 9
+    L.pepc_gpu_periodic_default.argtypes = [C.POINTER(Periodic)]
+    L.pepc_gpu_periodic_default.restype = None
+    L.pepc_gpu_set_periodic.argtypes = [_vp, C.POINTER(Periodic)]
+    L.pepc_gpu_lattice_get.argtypes = [_vp, _vp, _vp, _vp, C.POINTER(C.c_int32)]
+    L.pepc_gpu_constrain_periodic_device.argtypes = [_vp, _i64, _vp, _vp, _vp]
     for name in EXPORTS:
         f = getattr(L, name)
-        if f.restype is not None and name != "pepc_gpu_config_default":
+        if f.restype is not None and name not in ("pepc_gpu_config_default",
+                                                  "pepc_gpu_periodic_default"):
             f.restype = C.c_int
     _lib = L
     return L

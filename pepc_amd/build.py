@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, "libpepc_gpu.so")
 ARCH = os.environ.get("PEPC_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["keys.hip", "sort.hip", "tree.hip", "walk.hip", "step.hip", "adjoint.hip",
-               "runtime.hip"]
+               "lattice.hip", "runtime.hip"]
 
 # -ffp-contract=off: the key, box, moment and acceptance arithmetic must run in
 # the written operation order (the FP32 kernels request fused multiply-adds

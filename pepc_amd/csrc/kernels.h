@@ -121,6 +121,8 @@ struct WalkArgs {
   uint32_t* queue;                   // work-queue head (own cache line)
   const uint32_t* sys;               // batched systems: system per particle (else nullptr)
   const uint32_t* sysroot;           // batched systems: root node per system
+  int nbox;                          // periodic: lattice vectors walked per group (else 1)
+  const double* vbox;                // periodic: 3 * nbox lattice vectors, (0,0,0) last
 };
 void launch_walk_reset(hipStream_t s, const WalkArgs& a);
 // list_tile: M2P list capacity in LDS (0 = default: 128 FP32, 64 FP64;
@@ -129,6 +131,38 @@ bool walk_list_tile_ok(int precision, int list_tile);
 void launch_walk(hipStream_t s, const WalkArgs& a, int precision, int blocks, int list_tile);
 void launch_walk_stats(hipStream_t s, const WalkArgs& a);
 int walk_default_blocks(int precision, int list_tile);
+
+// lattice.hip: periodic boundaries (far field).  Tables of complex values
+// (double2) hold l = 0..lmax, m = 0..l at l(l+1)/2 + m.
+constexpr int kLatticeLM = 20;                  // multipole order of the box
+constexpr int kLatticeNM = 231;                 // entries for l <= 20
+constexpr int kLatticeNT = 861;                 // entries for l <= 40 (Taylor)
+constexpr int kLatticeMaxBlocks = 1024;         // partial-moment blocks
+constexpr int kLatticeFact = 81;                // factorials 0..80
+struct LatticeGeom {
+  double center[3];                             // LatticeCenter
+  double origin[3];                             // LatticeOrigin
+  double lat[9];                                // rows t_lattice_1..3
+  double latinv[9];                             // inverse of lat
+  int per[3];                                   // periodic axes
+  int simple;                                   // principal axes
+};
+// Lattice coefficients for neighbour vectors vec[3*nb] (central box last),
+// UL scale 2*layers+1.  Writes mstar, lstar (scratch) and ml.
+void launch_lattice_coefficients(hipStream_t s, const double* vec, int nb, double scale,
+                                 const double* fact, double2* mstar, double2* lstar,
+                                 int maxiter, double2* ml);
+int lattice_moment_blocks(int64_t n);
+// Per-step moments, mu and the field added to ex..pot (sign applied).
+// part: lattice_moment_blocks(n) * kLatticeNM, dpart: 3 per block, mu:
+// kLatticeNM, fict: 17 doubles.
+void launch_lattice_far(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
+                        const double* q, const LatticeGeom& g, const double2* ml, const double* fact,
+                        const double* vbox, int nb, double sign, double2* part, double* dpart,
+                        double2* mu, double* fict, double* ex, double* ey, double* ez,
+                        double* pot);
+void launch_constrain_periodic(hipStream_t s, int64_t n, double* x, double* y, double* z,
+                               const LatticeGeom& g);
 
 // step.hip: particle state of the on-device timestep (state order = key
 // order of the last grow) and the gather targets of the reorder.

@@ -1,0 +1,537 @@
+// lattice.hip -- periodic boundaries: lattice coefficients, multipole moments
+// of the central box, and the far-field (lattice) contribution per particle.
+//
+// Scheme of the reference's fast-multipole framework
+// (coulomb/module_fmm_framework.f90; Challacombe, White and Head-Gordon,
+// J. Chem. Phys. 107, 10131 (1997); Kudin and Scuseria, Chem. Phys. Lett.
+// 283, 61 (1998)).  The near field (the mirror boxes within one shell) is the
+// tree walk over every lattice vector (walk.hip); everything beyond it enters
+// through
+//   ML        lattice coefficients (Taylor table, l <= 40), fixed per lattice;
+//   omega     multipole moments (l <= 20) of the central box about its centre,
+//             particles plus the fictitious charges that cancel the box
+//             dipole, monopole removed;
+//   mu        M2L(ML, omega), the lattice's local expansion at the centre;
+// and per particle the value and gradient of mu shifted to the particle
+// (L2L to order 1) plus the fictitious charges of every near box.
+//
+// Tables hold l = 0..lmax, m = 0..l at l(l+1)/2 + m (tblinv, :863-873);
+// negative orders are (-1)^m conj (tbl, :838-858).  The harmonics are the
+// reference's: O_lm = r^l P_l^m(cos t) / (l+m)! e^{-i m p} and
+// M_lm = r^-(l+1) P_l^m(cos t) (l-m)! e^{i m p}, P without the Condon-Shortley
+// phase (:659-734, :1196-1245).
+//
+// Work per launch: the coefficient kernels run once per lattice; the moments
+// kernel is N x 231 complex terms (one grid row per order m, recurrences in
+// Cartesian form instead of Legendre evaluations per term); the field kernel
+// is N x 231 terms against mu in LDS plus 4 x 27 fictitious-charge pairs.
+#include "kernels.h"
+
+namespace pepc {
+
+namespace {
+
+constexpr int kLM = kLatticeLM;     // Lmax_multipole, :70
+constexpr int kLT = 2 * kLM;        // Lmax_taylor, :71
+constexpr int kNM = kLatticeNM;     // 231
+constexpr int kNT = kLatticeNT;     // 861
+
+__host__ __device__ constexpr int tix(int l, int m) { return l * (l + 1) / 2 + m; }
+
+__device__ __forceinline__ void tdecode(int t, int& l, int& m) {
+  int ll = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while (tix(ll + 1, 0) <= t) ++ll;
+  while (tix(ll, 0) > t) --ll;
+  l = ll;
+  m = t - tix(ll, 0);
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
+  return make_double2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+__device__ __forceinline__ double2 conj2(double2 a) { return make_double2(a.x, -a.y); }
+
+// tbl(A, l, m, lmax) for |m| <= l: stored entry, or (-1)^m conj for m < 0;
+// zero beyond lmax or |m| > l.
+__device__ __forceinline__ double2 tbl(const double2* a, int l, int m, int lmax) {
+  if (l > lmax || m > l || -m > l) return make_double2(0.0, 0.0);
+  if (m >= 0) return a[tix(l, m)];
+  const double2 v = a[tix(l, -m)];
+  return (m & 1) ? make_double2(-v.x, v.y) : make_double2(v.x, -v.y);
+}
+
+// LegendreP without the Condon-Shortley phase (:1196-1245).
+__device__ double legendre_p(int l, int m, double x) {
+  double pmm = 1.0, v;
+  if (m > 0) {
+    const double somx2 = sqrt((1.0 - x) * (1.0 + x));
+    double fact = 1.0;
+    for (int i = 1; i <= m; ++i) {
+      pmm = -pmm * fact * somx2;
+      fact += 2.0;
+    }
+  }
+  if (l == m) {
+    v = pmm;
+  } else {
+    double pmmp1 = x * (2.0 * m + 1.0) * pmm;
+    if (l == m + 1) {
+      v = pmmp1;
+    } else {
+      double pll = 0.0;
+      for (int ll = m + 2; ll <= l; ++ll) {
+        pll = (x * (2.0 * ll - 1.0) * pmmp1 - (ll + m - 1.0) * pmm) / (double)(ll - m);
+        pmm = pmmp1;
+        pmmp1 = pll;
+      }
+      v = pll;
+    }
+  }
+  return (m & 1) ? -v : v;
+}
+
+struct Sph { double r, ct, ph; };
+
+__device__ __forceinline__ Sph to_sph(double x, double y, double z) {   // :1083-1103
+  Sph s;
+  s.r = sqrt(x * x + y * y + z * z);
+  s.ct = s.r == 0.0 ? 1.0 : z / s.r;
+  s.ph = (x == 0.0 && y == 0.0) ? 0.0 : atan2(y, x);
+  return s;
+}
+
+// OMultipole (:695-715) for 0 <= m <= l.
+__device__ double2 o_multipole(int l, int m, Sph s, const double* fact) {
+  const double rl = (s.r > 0.0 || l > 0) ? pow(s.r, (double)l) : 1.0;
+  const double a = rl * (legendre_p(l, m, s.ct) / fact[l + m]);
+  return make_double2(a * cos((double)m * s.ph), -a * sin((double)m * s.ph));
+}
+
+// MTaylor (:720-734) for 0 <= m <= l.
+__device__ double2 m_taylor(int l, int m, Sph s, const double* fact) {
+  const double a = 1.0 / pow(s.r, (double)(l + 1)) * (legendre_p(l, m, s.ct) * fact[l - m]);
+  return make_double2(a * cos((double)m * s.ph), a * sin((double)m * s.ph));
+}
+
+// ---------------------------------------------------------------------------
+// Lattice coefficients (calc_lattice_coefficients, :154-213).
+// One block per entry of M* (kNM blocks, MstarFunc :1005-1036) and of L*
+// (kNT blocks, LstarFunc :1042-1077): one term per thread, terms sorted by
+// magnitude in LDS (real and imaginary parts separately), then summed from
+// the smallest up, as the reference does to avoid cancellation.
+constexpr int kTermCap = 1024;
+
+__global__ __launch_bounds__(kTermCap) void lattice_terms_kernel(const double* __restrict__ vec,
+                                                                 int nb, double scale,
+                                                                 const double* __restrict__ fact,
+                                                                 double2* __restrict__ mstar,
+                                                                 double2* __restrict__ lstar) {
+  __shared__ double re[kTermCap], im[kTermCap];
+  const bool is_m = blockIdx.x < (unsigned)kNM;
+  const int t = is_m ? (int)blockIdx.x : (int)blockIdx.x - kNM;
+  int l, m;
+  tdecode(t, l, m);
+  const int nterm = is_m ? nb : (nb - 1) * nb;
+  const int i = threadIdx.x;
+  double2 v = make_double2(0.0, 0.0);
+  if (i < nterm) {
+    if (is_m) {
+      v = o_multipole(l, m, to_sph(-vec[3 * i], -vec[3 * i + 1], -vec[3 * i + 2]), fact);
+    } else {
+      const int a = i / nb, b = i - a * nb;   // a: outer box (central box excluded)
+      const double cx = scale * vec[3 * a] + vec[3 * b];
+      const double cy = scale * vec[3 * a + 1] + vec[3 * b + 1];
+      const double cz = scale * vec[3 * a + 2] + vec[3 * b + 2];
+      v = m_taylor(l, m, to_sph(cx, cy, cz), fact);
+    }
+  }
+  re[i] = v.x;
+  im[i] = v.y;
+  __syncthreads();
+  // bitonic sort of both arrays by magnitude (zeros of the padding first)
+  for (int k = 2; k <= kTermCap; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int p = i ^ j;
+      if (p > i) {
+        const bool up = (i & k) == 0;
+        double a = re[i], b = re[p];
+        if ((fabs(a) > fabs(b)) == up) { re[i] = b; re[p] = a; }
+        a = im[i]; b = im[p];
+        if ((fabs(a) > fabs(b)) == up) { im[i] = b; im[p] = a; }
+      }
+      __syncthreads();
+    }
+  }
+  if (i == 0) {
+    double rp = 0.0, ip = 0.0;
+    for (int k = 0; k < kTermCap; ++k) { rp += re[k]; ip += im[k]; }
+    if (is_m) mstar[t] = make_double2(rp, ip);
+    else lstar[t] = make_double2(rp, ip);
+  }
+}
+
+// ML = L*; then maxiter times ML = M2L(UL(ML), M*) + L* (UL divides order l by
+// scale^(l+1), :982-1000; M2L :912-938).  One block, one entry per thread.
+__global__ __launch_bounds__(1024) void lattice_iterate_kernel(const double2* __restrict__ mstar,
+                                                              const double2* __restrict__ lstar,
+                                                              double scale, int maxiter,
+                                                              double2* __restrict__ ml) {
+  __shared__ double2 ul[kNT];
+  __shared__ double2 ms[kNM];
+  const int t = threadIdx.x;
+  int l = 0, m = 0;
+  if (t < kNT) tdecode(t, l, m);
+  if (t < kNM) ms[t] = mstar[t];
+  const double2 ls = t < kNT ? lstar[t] : make_double2(0.0, 0.0);
+  const double div = pow(scale, (double)(l + 1));
+  double2 cur = ls;
+  for (int it = 0; it < maxiter; ++it) {
+    if (t < kNT) ul[t] = make_double2(cur.x / div, cur.y / div);
+    __syncthreads();
+    if (t < kNT) {
+      double2 acc = make_double2(0.0, 0.0);
+      const int jmax = kLM < kLT - l ? kLM : kLT - l;
+      for (int j = 0; j <= jmax; ++j) {
+        const double sj = (j & 1) ? -1.0 : 1.0;
+        for (int k = -j; k <= j; ++k)
+          acc = cadd(acc, cmul(cscale(sj, tbl(ul, j + l, k + m, kLT)), tbl(ms, j, k, kLM)));
+      }
+      cur = cadd(acc, ls);
+    }
+    __syncthreads();
+  }
+  if (t < kNT) ml[t] = cur;
+}
+
+// ---------------------------------------------------------------------------
+// Per-step moments of the central box about its centre (calc_box_dipole
+// :605-631, calc_omega_tilde :514-600).  Grid (bx, kLM + 1): row m
+// accumulates O_lm q for l = m..kLM from the Cartesian recurrences
+//   O_mm = (x - i y)^m / (2^m m!),
+//   O_lm = ((2l-1) z O_{l-1,m} - r^2 O_{l-2,m}) / ((l-m)(l+m)),
+// which equal the reference's r^l P_l^m / (l+m)! e^{-i m p}.  Row 0 also
+// sums the dipole q r.  Each block writes its partial sums; the mu kernel
+// adds them in block order (deterministic).
+constexpr int kMomT = 256;
+
+__global__ __launch_bounds__(kMomT) void lattice_moments_kernel(int64_t n, const double* __restrict__ x,
+                                                                const double* __restrict__ y,
+                                                                const double* __restrict__ z,
+                                                                const double* __restrict__ q,
+                                                                LatticeGeom g,
+                                                                double2* __restrict__ part,
+                                                                double* __restrict__ dpart) {
+  const int m = blockIdx.y;
+  __shared__ double inv[kLM + 1];
+  __shared__ double red[kMomT / kWave][2 * (kLM + 1) + 3];
+  if (threadIdx.x <= (unsigned)kLM) {
+    const int l = threadIdx.x;
+    inv[l] = l > m ? 1.0 / (double)((l - m) * (l + m)) : 0.0;
+  }
+  __syncthreads();
+  double ar[kLM + 1], ai[kLM + 1], dp[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int l = 0; l <= kLM; ++l) { ar[l] = 0.0; ai[l] = 0.0; }
+  for (int64_t i = blockIdx.x * (int64_t)kMomT + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kMomT) {
+    const double rx = x[i] - g.center[0], ry = y[i] - g.center[1], rz = z[i] - g.center[2];
+    const double qi = q[i];
+    const double r2 = rx * rx + ry * ry + rz * rz;
+    if (m == 0) { dp[0] += qi * rx; dp[1] += qi * ry; dp[2] += qi * rz; }
+    double wr = 1.0, wi = 0.0;            // O_mm
+    for (int k = 1; k <= m; ++k) {
+      const double h = 0.5 / (double)k;
+      const double nr = (wr * rx + wi * ry) * h, ni = (wi * rx - wr * ry) * h;
+      wr = nr; wi = ni;
+    }
+    double p1r = 0.0, p1i = 0.0, p2r = 0.0, p2i = 0.0;
+#pragma unroll
+    for (int l = 0; l <= kLM; ++l) {
+      double cr = 0.0, ci = 0.0;
+      if (l == m) { cr = wr; ci = wi; }
+      else if (l > m) {
+        const double a = (double)(2 * l - 1) * rz;
+        cr = (a * p1r - r2 * p2r) * inv[l];
+        ci = (a * p1i - r2 * p2i) * inv[l];
+      }
+      ar[l] += qi * cr;
+      ai[l] += qi * ci;
+      p2r = p1r; p2i = p1i; p1r = cr; p1i = ci;
+    }
+  }
+  const int w = threadIdx.x / kWave;
+#pragma unroll
+  for (int l = 0; l <= kLM; ++l) {
+    const double sr = wave_sum(ar[l]), si = wave_sum(ai[l]);
+    if (lane_id() == 0) { red[w][2 * l] = sr; red[w][2 * l + 1] = si; }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double s = wave_sum(dp[d]);
+    if (lane_id() == 0) red[w][2 * (kLM + 1) + d] = s;
+  }
+  __syncthreads();
+  const int k = threadIdx.x;
+  if (k < 2 * (kLM + 1) + 3) {
+    double s = 0.0;
+    for (int v = 0; v < kMomT / kWave; ++v) s += red[v][k];
+    if (k < 2 * (kLM + 1)) {
+      const int l = k >> 1;
+      if (l >= m) {
+        double* dst = reinterpret_cast<double*>(&part[(size_t)blockIdx.x * kNM + tix(l, m)]);
+        dst[k & 1] = s;
+      }
+    } else if (m == 0) {
+      dpart[(size_t)blockIdx.x * 3 + (k - 2 * (kLM + 1))] = s;
+    }
+  }
+}
+
+// Sums the partial moments, adds the fictitious charges (Kudin's extrinsic
+// correction, :535-554): for each periodic axis p a charge -D_p / |t_p| at
+// origin + t_p, and their negated sum at the origin; removes the monopole
+// (:570-573) and forms mu = M2L(ML, omega) for l <= kLM (the orders the
+// field needs).  One block.
+__global__ __launch_bounds__(256) void lattice_mu_kernel(const double2* __restrict__ part,
+                                                        const double* __restrict__ dpart, int nblk,
+                                                        LatticeGeom g, const double2* __restrict__ ml,
+                                                        const double* __restrict__ fact,
+                                                        double2* __restrict__ mu,
+                                                        double* __restrict__ fict) {
+  __shared__ double2 om[kNM];
+  __shared__ double dip[3];
+  __shared__ double fq[4], fpos[4][3];
+  __shared__ int nf;
+  const int t = threadIdx.x;
+  if (t < kNM) {
+    double sr = 0.0, si = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+      const double2 v = part[(size_t)b * kNM + t];
+      sr += v.x; si += v.y;
+    }
+    om[t] = make_double2(sr, si);
+  }
+  if (t < 3) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += dpart[(size_t)b * 3 + t];
+    dip[t] = s;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int k = 0;
+    double sum = 0.0;
+    for (int p = 0; p < 3; ++p) {
+      if (!g.per[p]) continue;
+      const double* tv = g.lat + 3 * p;
+      for (int d = 0; d < 3; ++d) fpos[k][d] = g.origin[d] + tv[d];
+      fq[k] = -dip[p] / sqrt(tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2]);
+      sum += fq[k];
+      ++k;
+    }
+    for (int d = 0; d < 3; ++d) fpos[k][d] = g.origin[d];
+    fq[k] = -sum;
+    nf = k + 1;
+  }
+  __syncthreads();
+  if (t < kNM) {
+    int l, m;
+    tdecode(t, l, m);
+    double2 v = om[t];
+    for (int f = 0; f < nf; ++f) {
+      const Sph s = to_sph(fpos[f][0] - g.center[0], fpos[f][1] - g.center[1], fpos[f][2] - g.center[2]);
+      v = cadd(v, cscale(fq[f], o_multipole(l, m, s, fact)));
+    }
+    if (t == 0) v = make_double2(0.0, 0.0);
+    om[t] = v;
+  }
+  __syncthreads();
+  if (t < kNM) {
+    int l, m;
+    tdecode(t, l, m);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int j = 0; j <= kLM; ++j) {
+      const double sj = (j & 1) ? -1.0 : 1.0;
+      for (int k = -j; k <= j; ++k)
+        acc = cadd(acc, cmul(cscale(sj, tbl(ml, j + l, k + m, kLT)), tbl(om, j, k, kLM)));
+    }
+    mu[t] = acc;
+  }
+  if (t < 4 * 4) fict[t] = (t / 4) < nf ? (t % 4 < 3 ? fpos[t / 4][t % 4] : fq[t / 4]) : 0.0;
+  if (t == 0) fict[16] = (double)nf;
+}
+
+// Lattice field per particle (fmm_sum_lattice_force :766-833):
+//   phi = Re L00,  E = -(Re L11, Im L11, Re L10),  L = L2L(O(r - centre), mu)
+// accumulated column by column (order m, l = m..kLM by the recurrence), using
+//   L00 = sum_l O_l0 mu_l0 + 2 Re sum_{m>0} O_lm mu_lm
+//   L10 = sum_l O_l0 mu_{l+1,0} + 2 Re sum_{m>0} O_lm mu_{l+1,m}
+//   L11 = sum_{m>=0} O_lm mu_{l+1,m+1} - conj(O_l1) mu_{l+1,0}
+//         - sum_{m>=2} conj(O_lm mu_{l+1,m-1}),
+// then the fictitious charges of every neighbour box (plain Coulomb pair, no
+// softening).  Adds sign * (E, phi) to the walk's results.
+constexpr int kFieldT = 256;
+
+__global__ __launch_bounds__(kFieldT) void lattice_field_kernel(int64_t n, const double* __restrict__ x,
+                                                               const double* __restrict__ y,
+                                                               const double* __restrict__ z,
+                                                               LatticeGeom g,
+                                                               const double2* __restrict__ mu_g,
+                                                               const double* __restrict__ fict_g,
+                                                               const double* __restrict__ vbox, int nb,
+                                                               double sign, double* __restrict__ ex,
+                                                               double* __restrict__ ey,
+                                                               double* __restrict__ ez,
+                                                               double* __restrict__ pot) {
+  __shared__ double2 mu[kNM];
+  __shared__ double inv[kNM];
+  __shared__ double fict[17];
+  for (int t = threadIdx.x; t < kNM; t += kFieldT) {
+    mu[t] = mu_g[t];
+    int l, m;
+    tdecode(t, l, m);
+    inv[t] = l > m ? 1.0 / (double)((l - m) * (l + m)) : 0.0;
+  }
+  if (threadIdx.x < 17) fict[threadIdx.x] = fict_g[threadIdx.x];
+  __syncthreads();
+  const int64_t i = blockIdx.x * (int64_t)kFieldT + threadIdx.x;
+  if (i >= n) return;
+  const double px = x[i], py = y[i], pz = z[i];
+  const double rx = px - g.center[0], ry = py - g.center[1], rz = pz - g.center[2];
+  const double r2 = rx * rx + ry * ry + rz * rz;
+  double phi = 0.0, l10 = 0.0, l11r = 0.0, l11i = 0.0;
+  double wr = 1.0, wi = 0.0;     // O_mm
+  for (int m = 0; m <= kLM; ++m) {
+    if (m > 0) {
+      const double h = 0.5 / (double)m;
+      const double nr = (wr * rx + wi * ry) * h, ni = (wi * rx - wr * ry) * h;
+      wr = nr; wi = ni;
+    }
+    const double f = m == 0 ? 1.0 : 2.0;
+    double p1r = 0.0, p1i = 0.0, p2r = 0.0, p2i = 0.0;
+    for (int l = m; l <= kLM; ++l) {
+      double cr, ci;
+      if (l == m) { cr = wr; ci = wi; }
+      else {
+        const double a = (double)(2 * l - 1) * rz, s = inv[tix(l, m)];
+        cr = (a * p1r - r2 * p2r) * s;
+        ci = (a * p1i - r2 * p2i) * s;
+      }
+      const double2 o = make_double2(cr, ci);
+      phi += f * cmul(o, mu[tix(l, m)]).x;
+      if (l < kLM) {
+        l10 += f * cmul(o, mu[tix(l + 1, m)]).x;
+        const double2 a = cmul(o, mu[tix(l + 1, m + 1)]);
+        l11r += a.x; l11i += a.y;
+        if (m == 1) {
+          const double2 b = cmul(conj2(o), mu[tix(l + 1, 0)]);
+          l11r -= b.x; l11i -= b.y;
+        } else if (m >= 2) {
+          const double2 b = cmul(o, mu[tix(l + 1, m - 1)]);
+          l11r -= b.x; l11i += b.y;
+        }
+      }
+      p2r = p1r; p2i = p1i; p1r = cr; p1i = ci;
+    }
+  }
+  double e0 = -l11r, e1 = -l11i, e2 = -l10;
+  const int nf = (int)fict[16];
+  for (int fi = 0; fi < nf; ++fi) {
+    const double fx = fict[4 * fi], fy = fict[4 * fi + 1], fz = fict[4 * fi + 2], fq = fict[4 * fi + 3];
+    for (int b = 0; b < nb; ++b) {
+      const double dx = (px - vbox[3 * b]) - fx;
+      const double dy = (py - vbox[3 * b + 1]) - fy;
+      const double dz = (pz - vbox[3 * b + 2]) - fz;
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      const double rd = 1.0 / r;
+      const double rd3q = fq * rd * rd * rd;
+      phi += fq * rd;
+      e0 += rd3q * dx; e1 += rd3q * dy; e2 += rd3q * dz;
+    }
+  }
+  ex[i] += sign * e0;
+  ey[i] += sign * e1;
+  ez[i] += sign * e2;
+  pot[i] += sign * phi;
+}
+
+// constrain_periodic (module_mirror_boxes.f90:174-210): Fortran MODULO,
+// a - floor(a / p) p, per periodic axis for a principal-axes lattice; else in
+// lattice coordinates (x - origin) Lattice^-1, wrapped to [0, 1) when any
+// leaves [0, 1], mapped back, periodic axes replaced.
+__global__ void constrain_kernel(int64_t n, double* __restrict__ x, double* __restrict__ y,
+                                 double* __restrict__ z, LatticeGeom g) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double p[3] = {x[i], y[i], z[i]};
+    if (g.simple) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (!g.per[d]) continue;
+        const double w = g.lat[4 * d];
+        const double a = p[d] - g.origin[d];
+        p[d] = (a - floor(a / w) * w) + g.origin[d];
+      }
+    } else {
+      const double v[3] = {p[0] - g.origin[0], p[1] - g.origin[1], p[2] - g.origin[2]};
+      double lc[3];
+      bool out = false;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        lc[j] = v[0] * g.latinv[j] + v[1] * g.latinv[3 + j] + v[2] * g.latinv[6 + j];
+        out = out || lc[j] > 1.0 || lc[j] < 0.0;
+      }
+      if (out) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) lc[j] = lc[j] - floor(lc[j]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double r = lc[0] * g.lat[d] + lc[1] * g.lat[3 + d] + lc[2] * g.lat[6 + d] + g.origin[d];
+          if (g.per[d]) p[d] = r;
+        }
+      }
+    }
+    x[i] = p[0]; y[i] = p[1]; z[i] = p[2];
+  }
+}
+
+}  // namespace
+
+void launch_constrain_periodic(hipStream_t s, int64_t n, double* x, double* y, double* z,
+                               const LatticeGeom& g) {
+  int64_t b = (n + 255) / 256;
+  if (b < 1) b = 1;
+  if (b > 8192) b = 8192;
+  constrain_kernel<<<(unsigned)b, 256, 0, s>>>(n, x, y, z, g);
+}
+
+void launch_lattice_coefficients(hipStream_t s, const double* vec, int nb, double scale,
+                                 const double* fact, double2* mstar, double2* lstar,
+                                 int maxiter, double2* ml) {
+  lattice_terms_kernel<<<kNM + kNT, kTermCap, 0, s>>>(vec, nb, scale, fact, mstar, lstar);
+  lattice_iterate_kernel<<<1, 1024, 0, s>>>(mstar, lstar, scale, maxiter, ml);
+}
+
+int lattice_moment_blocks(int64_t n) {
+  int64_t b = (n + kMomT - 1) / kMomT;
+  if (b < 1) b = 1;
+  if (b > kLatticeMaxBlocks) b = kLatticeMaxBlocks;
+  return (int)b;
+}
+
+void launch_lattice_far(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
+                        const double* q, const LatticeGeom& g, const double2* ml, const double* fact,
+                        const double* vbox, int nb, double sign, double2* part, double* dpart,
+                        double2* mu, double* fict, double* ex, double* ey, double* ez,
+                        double* pot) {
+  const int nblk = lattice_moment_blocks(n);
+  lattice_moments_kernel<<<dim3(nblk, kLM + 1), kMomT, 0, s>>>(n, x, y, z, q, g, part, dpart);
+  lattice_mu_kernel<<<1, 256, 0, s>>>(part, dpart, nblk, g, ml, fact, mu, fict);
+  const unsigned fb = (unsigned)((n + kFieldT - 1) / kFieldT);
+  lattice_field_kernel<<<fb, kFieldT, 0, s>>>(n, x, y, z, g, mu, fict, vbox, nb, sign, ex, ey, ez, pot);
+}
+
+}  // namespace pepc

@@ -7,6 +7,7 @@
 // sequences of kernel launches on the context's stream with no host round
 // trip; the host reads the scalars once, in pepc_gpu_sync().
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -14,6 +15,7 @@
 
 #include "../../include/pepc_gpu.h"
 #include "kernels.h"
+#include "lattice_table.h"
 
 using namespace pepc;
 
@@ -90,6 +92,15 @@ struct Ctx {
   unsigned long long* ltotal = nullptr;
   int64_t list_ncap = 0, lent_cap = 0, frozen_entries = 0;
   bool frozen = false;
+  // periodic boundaries
+  bool periodic = false, far_field = false;
+  pepc_gpu_periodic per;
+  LatticeGeom lgeom;
+  int nbox = 1;
+  double* vbox = nullptr;           // 3 * nbox lattice vectors, central box last
+  double* fact = nullptr;           // factorials 0..80
+  double2 *ml = nullptr, *mstar = nullptr, *lstar = nullptr, *mu = nullptr, *lpart = nullptr;
+  double *ldpart = nullptr, *fict = nullptr;
 };
 
 int fail(Ctx* c, int code, const std::string& msg) {
@@ -139,7 +150,9 @@ void free_all(Ctx* c) {
                   c->mom64, c->wm32, c->wm64, c->hist, c->bsum, c->partial, c->box, c->sc,
                   c->sys, c->soff, c->sysroot, c->label, c->tlabel, c->ehist, c->ecount,
                   c->nadj, c->parent, c->leafof, c->aqueue, c->lcount, c->loff, c->lent,
-                  c->ltotal, c->nparent, c->echunk, c->esyschunk, c->epartial, c->prec4};
+                  c->ltotal, c->nparent, c->echunk, c->esyschunk, c->epartial, c->prec4,
+                  c->vbox, c->fact, c->ml, c->mstar, c->lstar, c->mu, c->lpart, c->ldpart,
+                  c->fict};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (double* p : c->st)
@@ -342,9 +355,16 @@ int traverse_device_impl(Ctx* c, double* ex, double* ey, double* ez, double* pot
   a.np2p = c->np2p; a.nm2p = c->nm2p; a.gstat = c->gstat; a.queue = c->queue;
   a.sys = c->nsys > 1 ? c->sys : nullptr;
   a.sysroot = c->sysroot;
+  if (c->periodic && c->nsys > 1)
+    return fail(c, PEPC_GPU_EINVAL, "traverse: periodic boundaries need a single system");
+  a.nbox = c->periodic ? c->nbox : 1;
+  a.vbox = c->periodic ? c->vbox : nullptr;
   launch_walk_reset(s, a);
   RECORD(c, 2);
   launch_walk(s, a, c->cfg.precision, c->walk_blocks, c->cfg.list_tile);
+  if (c->periodic && c->far_field)
+    launch_lattice_far(s, c->n, c->xs, c->ys, c->zs, c->qs, c->lgeom, c->ml, c->fact, c->vbox,
+                       c->nbox, c->cfg.sign, c->lpart, c->ldpart, c->mu, c->fict, ex, ey, ez, pot);
   RECORD(c, 3);
   launch_walk_stats(s, a);
   HIPCHECK(c, hipGetLastError());
@@ -715,6 +735,7 @@ int step_impl(Ctx* c, double dt) {
   const StepState st = step_state(c);
   hipStream_t s = c->stream;
   if (dt != 0.0) launch_kick_drift(s, n, dt, st);
+  if (dt != 0.0 && c->periodic) launch_constrain_periodic(s, n, st.x, st.y, st.z, c->lgeom);
   int r = grow_device_impl(c, n, st.x, st.y, st.z, st.q, c->nsys,
                            c->nsys > 1 ? c->hoff.data() : nullptr);
   if (r) return r;
@@ -919,6 +940,7 @@ int pepc_gpu_freeze(void* p, int64_t* nentries) {
   Ctx* c = as_ctx(p);
   if (!c) return PEPC_GPU_EINVAL;
   if (!c->grown) return fail(c, PEPC_GPU_EINVAL, "freeze: no tree (call grow first)");
+  if (c->periodic) return fail(c, PEPC_GPU_EINVAL, "freeze: not available with periodic boundaries");
   HIPCHECK(c, hipSetDevice(c->cfg.device));
   int r = ensure_adjoint(c);
   if (r) return r;
@@ -989,6 +1011,7 @@ int pepc_gpu_adjoint_device(void* p, const double* gex, const double* gey, const
   if (!gex || !gey || !gez || !gpot || !gx || !gy || !gz || !gq)
     return fail(c, PEPC_GPU_EINVAL, "adjoint: null argument");
   if (!c->grown) return fail(c, PEPC_GPU_EINVAL, "adjoint: no tree (call grow first)");
+  if (c->periodic) return fail(c, PEPC_GPU_EINVAL, "adjoint: not available with periodic boundaries");
   HIPCHECK(c, hipSetDevice(c->cfg.device));
   int r = ensure_adjoint(c);
   if (r) return r;
@@ -1112,6 +1135,175 @@ int pepc_gpu_read_energies(void* p, int64_t cap, double* out, int64_t* nrec) {
     const int64_t m = (k < cap ? k : cap) * c->nsys * 2;
     HIPCHECK(c, hipMemcpy(out, c->ehist, sizeof(double) * m, hipMemcpyDeviceToHost));
   }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Periodic boundaries
+}  // extern "C"
+
+namespace {
+
+// factorial of module_fmm_framework.f90:1247-1319: the exact integers up to
+// 24! (rounded to double), gamma(n + 1) beyond.
+double ref_factorial(int n) {
+  if (n <= 24) {
+    unsigned __int128 f = 1;
+    for (int k = 2; k <= n; ++k) f *= (unsigned)k;
+    return (double)f;
+  }
+  return std::tgamma((double)n + 1.0);
+}
+
+bool invert3(const double a[9], double out[9]) {
+  const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8],
+               c02 = a[3] * a[7] - a[4] * a[6];
+  const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+  if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) return false;
+  const double id = 1.0 / det;
+  out[0] = c00 * id; out[1] = (a[2] * a[7] - a[1] * a[8]) * id; out[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+  out[3] = c01 * id; out[4] = (a[0] * a[8] - a[2] * a[6]) * id; out[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+  out[6] = c02 * id; out[7] = (a[1] * a[6] - a[0] * a[7]) * id; out[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+  return true;
+}
+
+// Lattice indices of the neighbour boxes in the order of
+// neighbour_boxes_prepare (module_mirror_boxes.f90:119-149): a box and its
+// central-symmetric partner together, the central box last.
+std::vector<int> neighbour_boxes(const int per[3], int layers) {
+  int sw[3];
+  for (int d = 0; d < 3; ++d) sw[d] = per[d] ? layers : 0;
+  std::vector<int> b;
+  auto put = [&](int i, int j, int k) { b.push_back(i); b.push_back(j); b.push_back(k); };
+  for (int i = -sw[0]; i <= -1; ++i)
+    for (int j = -sw[1]; j <= sw[1]; ++j)
+      for (int k = -sw[2]; k <= sw[2]; ++k) { put(i, j, k); put(-i, -j, -k); }
+  for (int j = -sw[1]; j <= -1; ++j)
+    for (int k = -sw[2]; k <= sw[2]; ++k) { put(0, j, k); put(0, -j, -k); }
+  for (int k = -sw[2]; k <= -1; ++k) { put(0, 0, k); put(0, 0, -k); }
+  put(0, 0, 0);
+  return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+void pepc_gpu_periodic_default(pepc_gpu_periodic* p) {
+  if (!p) return;
+  memset(p, 0, sizeof(*p));
+  p->mirror_box_layers = 1;
+  p->t_lattice[0] = p->t_lattice[4] = p->t_lattice[8] = 1.0;
+  p->far_field = 1;
+}
+
+int pepc_gpu_set_periodic(void* ctx, const pepc_gpu_periodic* p) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return PEPC_GPU_EINVAL;
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  if (c->capturing) return fail(c, PEPC_GPU_EINVAL, "set_periodic: not during graph capture");
+  drop_graph(c);
+  if (!p || !(p->periodicity[0] || p->periodicity[1] || p->periodicity[2])) {
+    c->periodic = false;
+    c->far_field = false;
+    c->nbox = 1;
+    return 0;
+  }
+  const int layers = p->mirror_box_layers;
+  if (layers < 1 || layers > 3)
+    return fail(c, PEPC_GPU_EINVAL, "set_periodic: mirror_box_layers must be in [1, 3]");
+  for (int k = 0; k < 9; ++k)
+    if (!std::isfinite(p->t_lattice[k])) return fail(c, PEPC_GPU_EINVAL, "set_periodic: non-finite lattice");
+  for (int k = 0; k < 3; ++k)
+    if (!std::isfinite(p->origin[k])) return fail(c, PEPC_GPU_EINVAL, "set_periodic: non-finite origin");
+  LatticeGeom g;
+  memcpy(g.lat, p->t_lattice, sizeof(g.lat));
+  if (!invert3(g.lat, g.latinv)) return fail(c, PEPC_GPU_EINVAL, "set_periodic: degenerate lattice");
+  const bool all3 = p->periodicity[0] && p->periodicity[1] && p->periodicity[2];
+  const bool far = p->far_field != 0;
+  if (far && !all3)
+    return fail(c, PEPC_GPU_EINVAL, "set_periodic: the far field needs all three axes periodic");
+  if (far && layers != 1)
+    return fail(c, PEPC_GPU_EINVAL, "set_periodic: the far field supports mirror_box_layers = 1");
+  for (int d = 0; d < 3; ++d) {
+    g.origin[d] = p->origin[d];
+    g.center[d] = 0.5 * (g.lat[d] + g.lat[3 + d] + g.lat[6 + d]) + g.origin[d];
+    g.per[d] = p->periodicity[d] ? 1 : 0;
+  }
+  const double* t = g.lat;   // system_uses_principal_axes, module_mirror_boxes.f90:224-236
+  g.simple = (t[0] > 0.0 && t[1] == 0.0 && t[2] == 0.0 && t[3] == 0.0 && t[4] > 0.0 && t[5] == 0.0 &&
+              t[6] == 0.0 && t[7] == 0.0 && t[8] > 0.0) ? 1 : 0;
+  const std::vector<int> bx = neighbour_boxes(g.per, layers);
+  const int nb = (int)bx.size() / 3;
+  std::vector<double> vec(3 * (size_t)nb);
+  for (int b = 0; b < nb; ++b)   // lattice_vect, module_mirror_boxes.f90:73-79
+    for (int d = 0; d < 3; ++d)
+      vec[3 * b + d] = (double)bx[3 * b] * t[d] + (double)bx[3 * b + 1] * t[3 + d] +
+                       (double)bx[3 * b + 2] * t[6 + d];
+  c->periodic = false;              // until the new lattice is complete
+  c->far_field = false;
+  hipStream_t s = c->stream;
+  std::vector<double> f;            // host sources stay alive until the final sync
+  std::vector<double2> mlh;
+  ALLOC(c, c->vbox, 3 * (int64_t)nb);
+  HIPCHECK(c, hipMemcpyAsync(c->vbox, vec.data(), sizeof(double) * vec.size(), hipMemcpyHostToDevice, s));
+  if (far) {
+    ALLOC(c, c->fact, kLatticeFact);
+    ALLOC(c, c->ml, kLatticeNT); ALLOC(c, c->mstar, kLatticeNM); ALLOC(c, c->lstar, kLatticeNT);
+    ALLOC(c, c->mu, kLatticeNM); ALLOC(c, c->lpart, (int64_t)kLatticeMaxBlocks * kLatticeNM);
+    ALLOC(c, c->ldpart, (int64_t)kLatticeMaxBlocks * 3); ALLOC(c, c->fict, 17);
+    f.resize(kLatticeFact);
+    for (int k = 0; k < kLatticeFact; ++k) f[k] = ref_factorial(k);
+    HIPCHECK(c, hipMemcpyAsync(c->fact, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice, s));
+    auto nsq = [](const double* v) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; };
+    // system_is_unit_cube (module_mirror_boxes.f90:248-255): the reference
+    // then loads its pretabulated coefficients (module_fmm_framework.f90:114-121)
+    const bool unit = g.simple && nsq(t) == 1.0 && nsq(t + 3) == 1.0 && nsq(t + 6) == 1.0;
+    if (unit) {
+      mlh.assign(kLatticeNT, make_double2(0.0, 0.0));
+      for (int k = 0; k < kUnitCubeEntries; ++k) {
+        const UnitCubeEntry& e = kUnitCubeML[k];
+        mlh[e.l * (e.l + 1) / 2 + e.m] = make_double2(e.v, 0.0);
+      }
+      HIPCHECK(c, hipMemcpyAsync(c->ml, mlh.data(), sizeof(double2) * mlh.size(), hipMemcpyHostToDevice, s));
+    } else {
+      launch_lattice_coefficients(s, c->vbox, nb, (double)(2 * layers + 1), c->fact, c->mstar,
+                                  c->lstar, 32, c->ml);
+      HIPCHECK(c, hipGetLastError());
+    }
+  }
+  HIPCHECK(c, hipStreamSynchronize(s));
+  c->per = *p;
+  c->lgeom = g;
+  c->nbox = nb;
+  c->periodic = true;
+  c->far_field = far;
+  return 0;
+}
+
+int pepc_gpu_lattice_get(void* ctx, double* ml, double* mu, double* fict, int32_t* nfict) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return PEPC_GPU_EINVAL;
+  if (!c->periodic || !c->far_field) return fail(c, PEPC_GPU_EINVAL, "lattice_get: no far field set");
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (ml) HIPCHECK(c, hipMemcpy(ml, c->ml, sizeof(double2) * kLatticeNT, hipMemcpyDeviceToHost));
+  if (mu) HIPCHECK(c, hipMemcpy(mu, c->mu, sizeof(double2) * kLatticeNM, hipMemcpyDeviceToHost));
+  double f[17];
+  if (fict || nfict) HIPCHECK(c, hipMemcpy(f, c->fict, sizeof(f), hipMemcpyDeviceToHost));
+  if (fict) memcpy(fict, f, sizeof(double) * 16);
+  if (nfict) *nfict = (int32_t)f[16];
+  return 0;
+}
+
+int pepc_gpu_constrain_periodic_device(void* ctx, int64_t n, double* x, double* y, double* z) {
+  Ctx* c = as_ctx(ctx);
+  if (!c) return PEPC_GPU_EINVAL;
+  if (n < 0 || (n > 0 && (!x || !y || !z))) return fail(c, PEPC_GPU_EINVAL, "constrain_periodic: bad argument");
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  if (!c->periodic || n == 0) return 0;
+  launch_constrain_periodic(c->stream, n, x, y, z, c->lgeom);
+  HIPCHECK(c, hipGetLastError());
   return 0;
 }
 

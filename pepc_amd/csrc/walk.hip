@@ -25,6 +25,14 @@
 // batch's partial sums.  FP64 path: absolute coordinates, FP64 kernels,
 // plain FP64 accumulation, as the reference.
 //
+// Periodic systems (PER): the group walks the tree once per lattice vector
+// vbox of the mirror boxes, with its targets shifted to x - vbox
+// (module_libpepc_main.f90:184-187, module_tree_walk_simple.f90:328,345);
+// sums run on across the boxes.  Self is excluded in the central box only.
+// FP64 keeps separations as (x - vbox) - coc and flushes its lists at the end
+// of each box; FP32 puts the shift into the node frame instead (node entries
+// relative to first particle - vbox), so its lists carry over.
+//
 // The M2P evaluation regroups the reference's expression
 // (module_coulomb_kernels.f90:60-145) around the quadratic form
 // S = d.A.d of the quadrupole tensor; it is the same polynomial in d and the
@@ -137,7 +145,7 @@ __device__ __forceinline__ bool mac_fails(double xt, double yt, double zt, doubl
 
 // LM: capacity of the M2P list staged in LDS (the list tile); the P2P list
 // holds kListP entries.
-template <typename R, int LM>
+template <typename R, int LM, bool PER>
 __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
   using P = Prec<R>;
   using V4 = typename P::V4;
@@ -179,23 +187,20 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
     const double ox = a.x[a0], oy = a.y[a0], oz = a.z[a0];
     const double lox = wave_min(xt), loy = wave_min(yt), loz = wave_min(zt);
     const double hix = wave_max(xt), hiy = wave_max(yt), hiz = wave_max(zt);
-    // target position in the kernel's frame
-    R tx, ty, tz;
+    // target position in the kernel's frame (FP64: set per box below)
+    R tx = R(0), ty = R(0), tz = R(0);
     if constexpr (sizeof(R) == 4) {
       tx = (R)(xt - ox); ty = (R)(yt - oy); tz = (R)(zt - oz);
-    } else {
-      tx = xt; ty = yt; tz = zt;
     }
 
     Acc<R> acc;
 #pragma unroll
     for (int k = 0; k < 4; ++k) { acc.s[k] = R(0); acc.c[k] = R(0); acc.p[k] = R(0); }
 
-    uint32_t sp = 1, nm = 0, np = 0, cnt_p2p = 0, cnt_m2p = 0, namb = 0, nsteps = 0;
+    uint32_t sp = 0, nm = 0, np = 0, cnt_p2p = 0, cnt_m2p = 0, namb = 0, nsteps = 0;
     unsigned long long nmac = 0;
     bool overflow = false;
-    if (lane == 0) stack[0] = a.sys ? a.sysroot[a.sys[a0]] : 0u;   // the group's own root
-    __syncthreads();
+    const uint32_t root = a.sys ? a.sysroot[a.sys[a0]] : 0u;   // the group's own root
 
     // Batches: entry k goes to slice k mod S; the loop count is uniform.
     // FP32: two entries per pass (their LDS reads issued together), then the
@@ -228,13 +233,28 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
         const uint32_t k = kk + s;
         if (valid && k < np) {
           const V4 e = lp[k];
-          const bool self = lpi[k] == tid;
+          const bool self = lpi[k] == tid;   // mirror-box entries hold ~0u
           p2p<R>(tx - (R)e.x, ty - (R)e.y, tz - (R)e.z, (R)e.w, eps2, self, acc.p);
         }
       }
       fold(acc);
     };
 
+    const int nbox = PER ? a.nbox : 1;
+    for (int box = 0; box < nbox && !overflow; ++box) {
+    double vx = 0.0, vy = 0.0, vz = 0.0;
+    if constexpr (PER) { vx = a.vbox[3 * box]; vy = a.vbox[3 * box + 1]; vz = a.vbox[3 * box + 2]; }
+    const bool central = !PER || (vx == 0.0 && vy == 0.0 && vz == 0.0);
+    // targets of this box: x - vbox; group box likewise (rounding is monotone,
+    // so the shifted extremes are the extremes of the shifted targets)
+    const double sx = xt - vx, sy = yt - vy, sz = zt - vz;
+    const double blox = lox - vx, bloy = loy - vy, bloz = loz - vz;
+    const double bhix = hix - vx, bhiy = hiy - vy, bhiz = hiz - vz;
+    const double fox = ox - vx, foy = oy - vy, foz = oz - vz;   // FP32 node frame
+    if constexpr (sizeof(R) == 8) { tx = sx; ty = sy; tz = sz; }
+    if (lane == 0) stack[0] = root;
+    sp = 1;
+    __syncthreads();
     while (sp > 0) {
       int room = ((int)kStackCap - (int)sp) / 7;
       room = room < 1 ? 1 : room;
@@ -274,12 +294,12 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
       bool acc_ok = false, rej = false;
       uint32_t macc = 0;
       if (twig) {
-        const double ex = fmax(fmax(lox - gm.cx, gm.cx - hix), 0.0);
-        const double ey = fmax(fmax(loy - gm.cy, gm.cy - hiy), 0.0);
-        const double ez = fmax(fmax(loz - gm.cz, gm.cz - hiz), 0.0);
-        const double fx = fmax(fabs(gm.cx - lox), fabs(gm.cx - hix));
-        const double fy = fmax(fabs(gm.cy - loy), fabs(gm.cy - hiy));
-        const double fz = fmax(fabs(gm.cz - loz), fabs(gm.cz - hiz));
+        const double ex = fmax(fmax(blox - gm.cx, gm.cx - bhix), 0.0);
+        const double ey = fmax(fmax(bloy - gm.cy, gm.cy - bhiy), 0.0);
+        const double ez = fmax(fmax(bloz - gm.cz, gm.cz - bhiz), 0.0);
+        const double fx = fmax(fabs(gm.cx - blox), fabs(gm.cx - bhix));
+        const double fy = fmax(fabs(gm.cy - bloy), fabs(gm.cy - bhiy));
+        const double fz = fmax(fabs(gm.cz - bloz), fabs(gm.cz - bhiz));
         const double dmin2 = ex * ex + ey * ey + ez * ez;
         const double dmax2 = fx * fx + fy * fy + fz * fz;
         acc_ok = (theta2 * dmin2) * (1.0 - 1e-12) > gm.b2;
@@ -301,7 +321,7 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
         const int j = active ? __ffsll((long long)mk) - 1 : lane;
         const double cx = __shfl(gm.cx, j), cy = __shfl(gm.cy, j), cz = __shfl(gm.cz, j);
         const double b2 = __shfl(gm.b2, j);
-        const bool fail = active && mac_fails(xt, yt, zt, cx, cy, cz, b2, theta2);
+        const bool fail = active && mac_fails(sx, sy, sz, cx, cy, cz, b2, theta2);
         const unsigned long long ball = __ballot(fail && valid);
         if (amb_me && my_rank >= base && my_rank < base + S) {
           const uint32_t r = my_rank - base;
@@ -321,7 +341,7 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
         const uint32_t pos = nm + (uint32_t)popc64(am & lt);
         V4 e0;
         if constexpr (sizeof(R) == 4)
-          e0 = make_float4((float)(gm.cx - ox), (float)(gm.cy - oy), (float)(gm.cz - oz), 0.0f);
+          e0 = make_float4((float)(gm.cx - fox), (float)(gm.cy - foy), (float)(gm.cz - foz), 0.0f);
         else
           e0 = make_double4(gm.cx, gm.cy, gm.cz, 0.0);
         // w0 = {q, px, py, pz}, w1 = {2qx, 2qy, 2qz, xy}, w2 = {yz, zx, tr, -}
@@ -336,13 +356,13 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
         const uint32_t pos = np + (uint32_t)popc64(pm & lt);
         V4 e;
         if constexpr (sizeof(R) == 4) {
-          e = make_float4((float)(gm.cx - ox), (float)(gm.cy - oy), (float)(gm.cz - oz),
+          e = make_float4((float)(gm.cx - fox), (float)(gm.cy - foy), (float)(gm.cz - foz),
                           (float)gm.b2);
         } else {
           e = make_double4(gm.cx, gm.cy, gm.cz, gm.b2);
         }
         lp[pos] = e;
-        lpi[pos] = lk.first;
+        lpi[pos] = central ? lk.first : ~0u;
       }
       const uint32_t nma = (uint32_t)popc64(am), npa = (uint32_t)popc64(pm);
       nm += nma; np += npa; cnt_m2p += nma; cnt_p2p += npa;
@@ -355,6 +375,13 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
       sp += tot;
       __syncthreads();
     }
+    if constexpr (PER && sizeof(R) == 8) {   // next box shifts the targets
+      __syncthreads();
+      if (nm) flush_m2p();
+      if (np) flush_p2p();
+      nm = 0; np = 0;
+    }
+    }   // boxes
     __syncthreads();
     if (nm) flush_m2p();
     if (np) flush_p2p();
@@ -500,12 +527,12 @@ int walk_default_blocks(int precision, int list_tile) {
   hipError_t e;
   if (precision == kWalkFP64)
     e = list_tile == 32
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<double, 32>, kWave, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<double, 64>, kWave, 0);
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<double, 32, false>, kWave, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<double, 64, false>, kWave, 0);
   else
     e = list_tile == 64
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<float, 64>, kWave, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<float, 128>, kWave, 0);
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<float, 64, false>, kWave, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<float, 128, false>, kWave, 0);
   if (e != hipSuccess || per < 1) per = 8;
   return cus * per;
 }
@@ -520,18 +547,27 @@ bool walk_list_tile_ok(int precision, int list_tile) {
                                 : (list_tile == 64 || list_tile == 128);
 }
 
-void launch_walk(hipStream_t s, const WalkArgs& a, int precision, int blocks, int list_tile) {
+template <bool PER>
+static void launch_walk_t(hipStream_t s, const WalkArgs& a, int precision, int blocks,
+                          int list_tile) {
   if (precision == kWalkFP64) {
     if (list_tile == 32)
-      walk_kernel<double, 32><<<blocks, kWave, 0, s>>>(a);
+      walk_kernel<double, 32, PER><<<blocks, kWave, 0, s>>>(a);
     else
-      walk_kernel<double, 64><<<blocks, kWave, 0, s>>>(a);
+      walk_kernel<double, 64, PER><<<blocks, kWave, 0, s>>>(a);
   } else {
     if (list_tile == 64)
-      walk_kernel<float, 64><<<blocks, kWave, 0, s>>>(a);
+      walk_kernel<float, 64, PER><<<blocks, kWave, 0, s>>>(a);
     else
-      walk_kernel<float, 128><<<blocks, kWave, 0, s>>>(a);
+      walk_kernel<float, 128, PER><<<blocks, kWave, 0, s>>>(a);
   }
+}
+
+void launch_walk(hipStream_t s, const WalkArgs& a, int precision, int blocks, int list_tile) {
+  if (a.nbox > 1)
+    launch_walk_t<true>(s, a, precision, blocks, list_tile);
+  else
+    launch_walk_t<false>(s, a, precision, blocks, list_tile);
 }
 
 void launch_walk_stats(hipStream_t s, const WalkArgs& a) {

@@ -67,6 +67,12 @@ module pepc_gpu_c
       integer(c_int32_t) :: walk_blocks, device, group_pack, list_tile
    end type pepc_gpu_config
 
+   type, bind(C) :: pepc_gpu_periodic
+      integer(c_int32_t) :: periodicity(3), mirror_box_layers
+      real(c_double) :: t_lattice(9), origin(3)
+      integer(c_int32_t) :: far_field, pad
+   end type pepc_gpu_periodic
+
    type, bind(C) :: pepc_gpu_stats
       integer(c_int64_t) :: n, nnodes, nleaf, ntwig, ngroups, dup_fixed
       integer(c_int64_t) :: interactions, mac_evaluations, p2p, m2p, amb_nodes, walk_steps
@@ -131,6 +137,17 @@ module pepc_gpu_c
          type(c_ptr), value :: ctx
          real(c_double), intent(out) :: tg, tw
       end function
+
+      subroutine pepc_gpu_periodic_default(p) bind(C, name="pepc_gpu_periodic_default")
+         import :: pepc_gpu_periodic
+         type(pepc_gpu_periodic), intent(out) :: p
+      end subroutine
+
+      integer(c_int) function pepc_gpu_set_periodic(ctx, p) bind(C, name="pepc_gpu_set_periodic")
+         import :: c_int, c_ptr, pepc_gpu_periodic
+         type(c_ptr), value :: ctx
+         type(pepc_gpu_periodic), intent(in) :: p
+      end function
    end interface
 end module pepc_gpu_c
 
@@ -151,6 +168,7 @@ module module_pepc
    public :: pepc_statistics, pepc_last_stats, pepc_last_timings
 
    ! &calc_force_coulomb (coulomb/module_interaction_specific.f90:35-43)
+   logical, public :: include_far_field_if_periodic = .true.
    integer, public :: force_law = 3
    integer, public :: mac_select = 0
    real(kind_physics), public :: theta2 = 0.36_kind_physics
@@ -165,9 +183,17 @@ module module_pepc
    character(len=8), public :: precision = 'fp32'
    real(kind_physics), public :: interaction_sign = 1.0_kind_physics
    logical, public :: print_stats = .false.   ! the reference's DBG_STATS
+   ! periodic boundaries: &libpepc periodicity, mirror_box_layers and the
+   ! lattice of module_mirror_boxes.f90:30-36 (set before pepc_prepare)
+   logical, public :: periodicity(3) = [.false., .false., .false.]
+   integer, public :: mirror_box_layers = 1
+   real(kind_physics), public :: t_lattice_1(3) = [1, 0, 0]
+   real(kind_physics), public :: t_lattice_2(3) = [0, 1, 0]
+   real(kind_physics), public :: t_lattice_3(3) = [0, 0, 1]
+   real(kind_physics), public :: LatticeOrigin(3) = [0, 0, 0]
 
-   namelist /calc_force_coulomb/ force_law, mac_select, theta2, eps2
-   namelist /libpepc/ force_cubic_domain, np_mult, print_stats
+   namelist /calc_force_coulomb/ force_law, mac_select, include_far_field_if_periodic, theta2, eps2
+   namelist /libpepc/ periodicity, force_cubic_domain, np_mult, mirror_box_layers, print_stats
    namelist /walk_para_gpu/ group_size, group_pack, list_tile, precision, interaction_sign
 
    type(c_ptr) :: ctx = c_null_ptr
@@ -244,6 +270,7 @@ contains
    subroutine pepc_prepare(idim)
       integer(kind_dim), optional, intent(in) :: idim
       type(pepc_gpu_config) :: cfg
+      type(pepc_gpu_periodic) :: per
       if (present(idim)) then
          if (idim /= 3) then
             write (*, *) 'PEPC-GPU: only idim = 3 is supported'
@@ -270,6 +297,17 @@ contains
          cfg%precision = 0
       end if
       call check(pepc_gpu_create(ctx, cfg), 'pepc_prepare')
+      if (any(periodicity)) then   ! neighbour_boxes_prepare + fmm_framework_prepare
+         call pepc_gpu_periodic_default(per)
+         per%periodicity = merge(1, 0, periodicity)
+         per%mirror_box_layers = mirror_box_layers
+         per%t_lattice(1:3) = t_lattice_1
+         per%t_lattice(4:6) = t_lattice_2
+         per%t_lattice(7:9) = t_lattice_3
+         per%origin = LatticeOrigin
+         per%far_field = merge(1, 0, include_far_field_if_periodic)
+         call check(pepc_gpu_set_periodic(ctx, per), 'pepc_prepare (periodic)')
+      end if
    end subroutine pepc_prepare
 
    subroutine pepc_finalize(comm)

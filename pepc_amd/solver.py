@@ -312,6 +312,48 @@ class Solver:
                                                 C.byref(ng)))
         return {"gfirst": gf, "gcount": gc, "gmac": gm, "p2p": p2p, "m2p": m2p, "stats": st}
 
+    # ------------------------------------------------------ periodic boundaries
+    def set_periodic(self, periodicity=(1, 1, 1), lattice=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0),
+                     layers=1, far_field=True):
+        """Periodic boundaries for the following traverses (periodicity=None
+        clears them).  lattice: three box lengths (principal axes) or a 3x3
+        array whose rows are t_lattice_1..3."""
+        if periodicity is None:
+            self._check(self.lib.pepc_gpu_set_periodic(self.h, None))
+            return
+        p = _lib.Periodic()
+        self.lib.pepc_gpu_periodic_default(C.byref(p))
+        lat = np.asarray(lattice, dtype=np.float64)
+        if lat.shape == (3,):
+            lat = np.diag(lat)
+        if lat.shape != (3, 3):
+            raise ValueError("set_periodic: lattice must be 3 lengths or a 3x3 array")
+        for k in range(3):
+            p.periodicity[k] = int(bool(periodicity[k]))
+            p.origin[k] = float(origin[k])
+        for k, v in enumerate(lat.reshape(9)):
+            p.t_lattice[k] = float(v)
+        p.mirror_box_layers = int(layers)
+        p.far_field = int(bool(far_field))
+        self._check(self.lib.pepc_gpu_set_periodic(self.h, C.byref(p)))
+
+    def lattice(self):
+        """(ml complex (861,), mu complex (231,), fict (nfict, 4)) of the far field."""
+        ml = np.empty(2 * 861)
+        mu = np.empty(2 * 231)
+        fict = np.empty(16)
+        nf = C.c_int32()
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)
+        self._check(self.lib.pepc_gpu_lattice_get(self.h, vp(ml), vp(mu), vp(fict), C.byref(nf)))
+        return ml.view(np.complex128), mu.view(np.complex128), fict[:4 * nf.value].reshape(-1, 4)
+
+    def constrain_periodic(self, x, y, z):
+        """Wraps device tensors x, y, z (float64, in place) into the central box."""
+        self._bind_stream()
+        n = x.numel()
+        self._check(self.lib.pepc_gpu_constrain_periodic_device(self.h, n, _ptr(x), _ptr(y), _ptr(z)))
+        self.torch.cuda.synchronize(self.device)
+
     def direct(self, x, y, z, q, testidx, eps2=None, sign=None):
         """FP64 direct sum at the given targets (device computation)."""
         t = self.torch

@@ -64,6 +64,18 @@ def mini(n=32768, seed=1):
                   nsteps=10, seed=seed)
 
 
+def periodic(n=32768, seed=11, box=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), eps2=1e-4,
+             theta2=0.36):
+    """Neutral plasma in a periodic box: uniform positions, alternating
+    +-1 charges (pepc-mini's charges), unit masses, at rest."""
+    r = _rng(seed)
+    p = np.asarray(origin, float) + r.random((n, 3)) * np.asarray(box, float)
+    q = -1.0 + 2.0 * np.mod(np.arange(1, n + 1), 2)
+    return System("periodic", p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), q,
+                  np.ones(n), np.zeros((n, 3)), eps2=eps2, theta2=theta2, dt=0.002,
+                  nsteps=10, seed=seed, extra={"box": tuple(box), "origin": tuple(origin)})
+
+
 def unit_ball(r, n):
     u = r.random(n)
     rad = np.cbrt(u)
