@@ -153,14 +153,17 @@ void launch_lattice_coefficients(hipStream_t s, const double* vec, int nb, doubl
                                  const double* fact, double2* mstar, double2* lstar,
                                  int maxiter, double2* ml);
 int lattice_moment_blocks(int64_t n);
+// Recurrence constants: inv[l(l+1)/2 + m] = 1 / ((l-m)(l+m)) for l > m,
+// inv[kLatticeNM + m] = 1 / (2m); kLatticeNM + 21 doubles.
+void launch_lattice_inv(hipStream_t s, double* inv);
 // Per-step moments, mu and the field added to ex..pot (sign applied).
-// part: lattice_moment_blocks(n) * kLatticeNM, dpart: 3 per block, mu:
-// kLatticeNM, fict: 17 doubles.
+// part: lattice_moment_blocks(n) * kLatticeNM, dpart: 3 per block, om and
+// mu: kLatticeNM, dip: 3, fict: 17 doubles, coef: 8 * kLatticeNM doubles.
 void launch_lattice_far(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
                         const double* q, const LatticeGeom& g, const double2* ml, const double* fact,
-                        const double* vbox, int nb, double sign, double2* part, double* dpart,
-                        double2* mu, double* fict, double* ex, double* ey, double* ez,
-                        double* pot);
+                        const double* inv, const double* vbox, int nb, double sign, double2* part,
+                        double* dpart, double2* om, double* dip, double2* mu, double* fict,
+                        double* coef, double* ex, double* ey, double* ez, double* pot);
 void launch_constrain_periodic(hipStream_t s, int64_t n, double* x, double* y, double* z,
                                const LatticeGeom& g);
 

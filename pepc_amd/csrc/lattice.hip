@@ -209,14 +209,98 @@ __global__ __launch_bounds__(1024) void lattice_iterate_kernel(const double2* __
 
 // ---------------------------------------------------------------------------
 // Per-step moments of the central box about its centre (calc_box_dipole
-// :605-631, calc_omega_tilde :514-600).  Grid (bx, kLM + 1): row m
-// accumulates O_lm q for l = m..kLM from the Cartesian recurrences
-//   O_mm = (x - i y)^m / (2^m m!),
-//   O_lm = ((2l-1) z O_{l-1,m} - r^2 O_{l-2,m}) / ((l-m)(l+m)),
-// which equal the reference's r^l P_l^m / (l+m)! e^{-i m p}.  Row 0 also
-// sums the dipole q r.  Each block writes its partial sums; the mu kernel
-// adds them in block order (deterministic).
+// :605-631, calc_omega_tilde :514-600).  Grid (bx, kLM + 1): row M (a
+// template parameter, so every loop below has a fixed trip count and the
+// recurrence constants fold) accumulates q O_lM for l = M..kLM with
+//   O_MM = (x - i y)^M / (2^M M!),
+//   O_lM = ((2l-1) z O_{l-1,M} - r^2 O_{l-2,M}) / ((l-M)(l+M)),
+// which equal the reference's r^l P_l^M / (l+M)! e^{-i M p}.  Row 0 also
+// sums the dipole q r.  Every thread handles a strided run of particles (the
+// next one's coordinates loaded while the current one is evaluated); each
+// block writes its partial sums, which lattice_omega_kernel adds in block
+// order (deterministic).
 constexpr int kMomT = 256;
+constexpr int kMomRed = 2 * (kLM + 1) + 3;
+
+template <int M>
+__device__ __forceinline__ void moments_row(int64_t n, const double* __restrict__ x,
+                                            const double* __restrict__ y,
+                                            const double* __restrict__ z,
+                                            const double* __restrict__ q, const LatticeGeom& g,
+                                            double2* __restrict__ part, double* __restrict__ dpart,
+                                            double (*red)[kMomRed]) {
+  constexpr int K = kLM + 1 - M;
+  double ar[K], ai[K], dp[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int j = 0; j < K; ++j) { ar[j] = 0.0; ai[j] = 0.0; }
+  const int64_t stride = (int64_t)gridDim.x * kMomT;
+  int64_t i = blockIdx.x * (int64_t)kMomT + threadIdx.x;
+  double nx = 0.0, ny = 0.0, nz = 0.0, nq = 0.0;
+  if (i < n) { nx = x[i]; ny = y[i]; nz = z[i]; nq = q[i]; }
+  while (i < n) {
+    const double rx = nx - g.center[0], ry = ny - g.center[1], rz = nz - g.center[2];
+    const double qi = nq;
+    const int64_t i2 = i + stride;
+    if (i2 < n) { nx = x[i2]; ny = y[i2]; nz = z[i2]; nq = q[i2]; }
+    i = i2;
+    const double r2 = rx * rx + ry * ry + rz * rz;
+    if constexpr (M == 0) { dp[0] += qi * rx; dp[1] += qi * ry; dp[2] += qi * rz; }
+    double wr = 1.0, wi = 0.0;
+#pragma unroll
+    for (int k = 1; k <= M; ++k) {
+      const double h = 0.5 / (double)k;
+      const double tr = (wr * rx + wi * ry) * h, ti = (wi * rx - wr * ry) * h;
+      wr = tr; wi = ti;
+    }
+    ar[0] += qi * wr;
+    ai[0] += qi * wi;
+    double p1r = wr, p1i = wi, p2r = 0.0, p2i = 0.0;
+#pragma unroll
+    for (int j = 1; j < K; ++j) {
+      const int l = M + j;
+      const double a = (double)(2 * l - 1) * rz;
+      const double inv = 1.0 / (double)((l - M) * (l + M));
+      const double cr = (a * p1r - r2 * p2r) * inv, ci = (a * p1i - r2 * p2i) * inv;
+      ar[j] += qi * cr;
+      ai[j] += qi * ci;
+      p2r = p1r; p2i = p1i; p1r = cr; p1i = ci;
+    }
+  }
+  const int w = threadIdx.x / kWave;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const double sr = wave_sum(ar[j]), si = wave_sum(ai[j]);
+    if (lane_id() == 0) { red[w][2 * j] = sr; red[w][2 * j + 1] = si; }
+  }
+  if constexpr (M == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double sd = wave_sum(dp[d]);
+      if (lane_id() == 0) red[w][2 * K + d] = sd;
+    }
+  }
+  __syncthreads();
+  const int k = threadIdx.x;
+  if (k < 2 * K + (M == 0 ? 3 : 0)) {
+    double sum = 0.0;
+    for (int v = 0; v < kMomT / kWave; ++v) sum += red[v][k];
+    if (k < 2 * K) {
+      double* dst = reinterpret_cast<double*>(&part[(size_t)blockIdx.x * kNM + tix(M + (k >> 1), M)]);
+      dst[k & 1] = sum;
+    } else {
+      dpart[(size_t)blockIdx.x * 3 + (k - 2 * K)] = sum;
+    }
+  }
+}
+
+template <int M>
+__device__ __forceinline__ void moments_dispatch(int row, int64_t n, const double* x, const double* y,
+                                                 const double* z, const double* q,
+                                                 const LatticeGeom& g, double2* part, double* dpart,
+                                                 double (*red)[kMomRed]) {
+  if (row == M) moments_row<M>(n, x, y, z, q, g, part, dpart, red);
+  else if constexpr (M < kLM) moments_dispatch<M + 1>(row, n, x, y, z, q, g, part, dpart, red);
+}
 
 __global__ __launch_bounds__(kMomT) void lattice_moments_kernel(int64_t n, const double* __restrict__ x,
                                                                 const double* __restrict__ y,
@@ -225,102 +309,57 @@ __global__ __launch_bounds__(kMomT) void lattice_moments_kernel(int64_t n, const
                                                                 LatticeGeom g,
                                                                 double2* __restrict__ part,
                                                                 double* __restrict__ dpart) {
-  const int m = blockIdx.y;
-  __shared__ double inv[kLM + 1];
-  __shared__ double red[kMomT / kWave][2 * (kLM + 1) + 3];
-  if (threadIdx.x <= (unsigned)kLM) {
-    const int l = threadIdx.x;
-    inv[l] = l > m ? 1.0 / (double)((l - m) * (l + m)) : 0.0;
-  }
-  __syncthreads();
-  double ar[kLM + 1], ai[kLM + 1], dp[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-  for (int l = 0; l <= kLM; ++l) { ar[l] = 0.0; ai[l] = 0.0; }
-  for (int64_t i = blockIdx.x * (int64_t)kMomT + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kMomT) {
-    const double rx = x[i] - g.center[0], ry = y[i] - g.center[1], rz = z[i] - g.center[2];
-    const double qi = q[i];
-    const double r2 = rx * rx + ry * ry + rz * rz;
-    if (m == 0) { dp[0] += qi * rx; dp[1] += qi * ry; dp[2] += qi * rz; }
-    double wr = 1.0, wi = 0.0;            // O_mm
-    for (int k = 1; k <= m; ++k) {
-      const double h = 0.5 / (double)k;
-      const double nr = (wr * rx + wi * ry) * h, ni = (wi * rx - wr * ry) * h;
-      wr = nr; wi = ni;
+  __shared__ double red[kMomT / kWave][kMomRed];
+  moments_dispatch<0>((int)blockIdx.y, n, x, y, z, q, g, part, dpart, red);
+}
+
+// omega (and the dipole) from the per-block partials: one wave per entry,
+// lane j adds blocks j, j+64, ... in order, then a fixed butterfly.
+__global__ __launch_bounds__(kWave) void lattice_omega_kernel(const double2* __restrict__ part,
+                                                             const double* __restrict__ dpart,
+                                                             int nblk, double2* __restrict__ om,
+                                                             double* __restrict__ dip) {
+  const int e = blockIdx.x;
+  const int lane = threadIdx.x;
+  double sr = 0.0, si = 0.0;
+  if (e < kNM) {
+    for (int b = lane; b < nblk; b += kWave) {
+      const double2 v = part[(size_t)b * kNM + e];
+      sr += v.x; si += v.y;
     }
-    double p1r = 0.0, p1i = 0.0, p2r = 0.0, p2i = 0.0;
-#pragma unroll
-    for (int l = 0; l <= kLM; ++l) {
-      double cr = 0.0, ci = 0.0;
-      if (l == m) { cr = wr; ci = wi; }
-      else if (l > m) {
-        const double a = (double)(2 * l - 1) * rz;
-        cr = (a * p1r - r2 * p2r) * inv[l];
-        ci = (a * p1i - r2 * p2i) * inv[l];
-      }
-      ar[l] += qi * cr;
-      ai[l] += qi * ci;
-      p2r = p1r; p2i = p1i; p1r = cr; p1i = ci;
-    }
+  } else {
+    for (int b = lane; b < nblk; b += kWave) sr += dpart[(size_t)b * 3 + (e - kNM)];
   }
-  const int w = threadIdx.x / kWave;
-#pragma unroll
-  for (int l = 0; l <= kLM; ++l) {
-    const double sr = wave_sum(ar[l]), si = wave_sum(ai[l]);
-    if (lane_id() == 0) { red[w][2 * l] = sr; red[w][2 * l + 1] = si; }
-  }
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double s = wave_sum(dp[d]);
-    if (lane_id() == 0) red[w][2 * (kLM + 1) + d] = s;
-  }
-  __syncthreads();
-  const int k = threadIdx.x;
-  if (k < 2 * (kLM + 1) + 3) {
-    double s = 0.0;
-    for (int v = 0; v < kMomT / kWave; ++v) s += red[v][k];
-    if (k < 2 * (kLM + 1)) {
-      const int l = k >> 1;
-      if (l >= m) {
-        double* dst = reinterpret_cast<double*>(&part[(size_t)blockIdx.x * kNM + tix(l, m)]);
-        dst[k & 1] = s;
-      }
-    } else if (m == 0) {
-      dpart[(size_t)blockIdx.x * 3 + (k - 2 * (kLM + 1))] = s;
-    }
+  sr = wave_sum(sr);
+  si = wave_sum(si);
+  if (lane == 0) {
+    if (e < kNM) om[e] = make_double2(sr, si);
+    else dip[e - kNM] = sr;
   }
 }
 
-// Sums the partial moments, adds the fictitious charges (Kudin's extrinsic
-// correction, :535-554): for each periodic axis p a charge -D_p / |t_p| at
-// origin + t_p, and their negated sum at the origin; removes the monopole
-// (:570-573) and forms mu = M2L(ML, omega) for l <= kLM (the orders the
-// field needs).  One block.
-__global__ __launch_bounds__(256) void lattice_mu_kernel(const double2* __restrict__ part,
-                                                        const double* __restrict__ dpart, int nblk,
-                                                        LatticeGeom g, const double2* __restrict__ ml,
-                                                        const double* __restrict__ fact,
-                                                        double2* __restrict__ mu,
-                                                        double* __restrict__ fict) {
+// Adds the fictitious charges (Kudin's extrinsic correction, :535-554): for
+// each periodic axis p a charge -D_p / |t_p| at origin + t_p, and their
+// negated sum at the origin; removes the monopole (:570-573) and forms
+// mu = M2L(ML, omega) for l <= kLM (the orders the field needs), four
+// threads per entry (j = part mod 4), their sums added in order.  One block.
+constexpr int kMuT = 1024;
+
+__global__ __launch_bounds__(kMuT) void lattice_mu_kernel(const double2* __restrict__ om_g,
+                                                         const double* __restrict__ dip_g,
+                                                         LatticeGeom g, const double2* __restrict__ ml_g,
+                                                         const double* __restrict__ fact,
+                                                         double2* __restrict__ mu,
+                                                         double* __restrict__ fict,
+                                                         double* __restrict__ coef) {
   __shared__ double2 om[kNM];
-  __shared__ double dip[3];
+  __shared__ double2 mus[kNM];
+  __shared__ double2 ml[kNT];
+  __shared__ double2 acc4[kNM][4];
   __shared__ double fq[4], fpos[4][3];
   __shared__ int nf;
   const int t = threadIdx.x;
-  if (t < kNM) {
-    double sr = 0.0, si = 0.0;
-    for (int b = 0; b < nblk; ++b) {
-      const double2 v = part[(size_t)b * kNM + t];
-      sr += v.x; si += v.y;
-    }
-    om[t] = make_double2(sr, si);
-  }
-  if (t < 3) {
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += dpart[(size_t)b * 3 + t];
-    dip[t] = s;
-  }
-  __syncthreads();
+  for (int k = t; k < kNT; k += kMuT) ml[k] = ml_g[k];
   if (t == 0) {
     int k = 0;
     double sum = 0.0;
@@ -328,7 +367,7 @@ __global__ __launch_bounds__(256) void lattice_mu_kernel(const double2* __restri
       if (!g.per[p]) continue;
       const double* tv = g.lat + 3 * p;
       for (int d = 0; d < 3; ++d) fpos[k][d] = g.origin[d] + tv[d];
-      fq[k] = -dip[p] / sqrt(tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2]);
+      fq[k] = -dip_g[p] / sqrt(tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2]);
       sum += fq[k];
       ++k;
     }
@@ -340,7 +379,7 @@ __global__ __launch_bounds__(256) void lattice_mu_kernel(const double2* __restri
   if (t < kNM) {
     int l, m;
     tdecode(t, l, m);
-    double2 v = om[t];
+    double2 v = om_g[t];
     for (int f = 0; f < nf; ++f) {
       const Sph s = to_sph(fpos[f][0] - g.center[0], fpos[f][1] - g.center[1], fpos[f][2] - g.center[2]);
       v = cadd(v, cscale(fq[f], o_multipole(l, m, s, fact)));
@@ -349,54 +388,82 @@ __global__ __launch_bounds__(256) void lattice_mu_kernel(const double2* __restri
     om[t] = v;
   }
   __syncthreads();
-  if (t < kNM) {
+  const int e = t >> 2, part = t & 3;
+  if (e < kNM) {
     int l, m;
-    tdecode(t, l, m);
+    tdecode(e, l, m);
     double2 acc = make_double2(0.0, 0.0);
-    for (int j = 0; j <= kLM; ++j) {
+    for (int j = part; j <= kLM; j += 4) {
       const double sj = (j & 1) ? -1.0 : 1.0;
       for (int k = -j; k <= j; ++k)
         acc = cadd(acc, cmul(cscale(sj, tbl(ml, j + l, k + m, kLT)), tbl(om, j, k, kLM)));
     }
-    mu[t] = acc;
+    acc4[e][part] = acc;
+  }
+  __syncthreads();
+  if (t < kNM) {
+    const double2 v = cadd(cadd(cadd(acc4[t][0], acc4[t][1]), acc4[t][2]), acc4[t][3]);
+    mu[t] = v;
+    mus[t] = v;
   }
   if (t < 4 * 4) fict[t] = (t / 4) < nf ? (t % 4 < 3 ? fpos[t / 4][t % 4] : fq[t / 4]) : 0.0;
   if (t == 0) fict[16] = (double)nf;
+  __syncthreads();
+  // Field coefficients per entry (l, m): the L00, L10 and L11 sums of the
+  // field kernel are linear in O_lm = (c, s):
+  //   phi += c k0 + s k1,  L10 += c k2 + s k3,  L11 += (c k4 + s k5, c k6 + s k7)
+  // from f = 1 (m = 0) or 2, A = mu_{l+1,m+1}, B = mu_{l+1,m-1} (m >= 1).
+  if (t < kNM) {
+    int l, m;
+    tdecode(t, l, m);
+    const double f = m == 0 ? 1.0 : 2.0;
+    const double2 u = mus[t];
+    double k[8] = {f * u.x, -f * u.y, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (l < kLM) {
+      const double2 a = mus[tix(l + 1, m)];
+      k[2] = f * a.x;
+      k[3] = -f * a.y;
+      const double2 A = mus[tix(l + 1, m + 1)];
+      const double2 B = m >= 1 ? mus[tix(l + 1, m - 1)] : make_double2(0.0, 0.0);
+      const double sb = m == 0 ? 0.0 : (m == 1 ? -1.0 : 1.0);
+      k[4] = A.x - B.x;
+      k[5] = -A.y + sb * B.y;
+      k[6] = A.y + sb * B.y;
+      k[7] = A.x + B.x;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) coef[8 * t + j] = k[j];
+  }
 }
 
 // Lattice field per particle (fmm_sum_lattice_force :766-833):
 //   phi = Re L00,  E = -(Re L11, Im L11, Re L10),  L = L2L(O(r - centre), mu)
-// accumulated column by column (order m, l = m..kLM by the recurrence), using
+// with
 //   L00 = sum_l O_l0 mu_l0 + 2 Re sum_{m>0} O_lm mu_lm
 //   L10 = sum_l O_l0 mu_{l+1,0} + 2 Re sum_{m>0} O_lm mu_{l+1,m}
 //   L11 = sum_{m>=0} O_lm mu_{l+1,m+1} - conj(O_l1) mu_{l+1,0}
 //         - sum_{m>=2} conj(O_lm mu_{l+1,m-1}),
-// then the fictitious charges of every neighbour box (plain Coulomb pair, no
-// softening).  Adds sign * (E, phi) to the walk's results.
+// all linear in (Re, Im) O_lm, so each entry is eight multiply-adds against
+// the coefficients lattice_mu_kernel tabulates; O_lm column by column (order
+// m, l = m..kLM by the recurrence).  Then the fictitious charges of every
+// neighbour box (plain Coulomb pair, no softening; 1/r by reciprocal square
+// root).  Coefficients, recurrence constants, box vectors and charges are
+// wave-uniform and come through the scalar cache.  Adds sign * (E, phi) to
+// the walk's results.
 constexpr int kFieldT = 256;
 
 __global__ __launch_bounds__(kFieldT) void lattice_field_kernel(int64_t n, const double* __restrict__ x,
                                                                const double* __restrict__ y,
                                                                const double* __restrict__ z,
                                                                LatticeGeom g,
-                                                               const double2* __restrict__ mu_g,
-                                                               const double* __restrict__ fict_g,
+                                                               const double* __restrict__ coef,
+                                                               const double* __restrict__ inv,
+                                                               const double* __restrict__ fict,
                                                                const double* __restrict__ vbox, int nb,
                                                                double sign, double* __restrict__ ex,
                                                                double* __restrict__ ey,
                                                                double* __restrict__ ez,
                                                                double* __restrict__ pot) {
-  __shared__ double2 mu[kNM];
-  __shared__ double inv[kNM];
-  __shared__ double fict[17];
-  for (int t = threadIdx.x; t < kNM; t += kFieldT) {
-    mu[t] = mu_g[t];
-    int l, m;
-    tdecode(t, l, m);
-    inv[t] = l > m ? 1.0 / (double)((l - m) * (l + m)) : 0.0;
-  }
-  if (threadIdx.x < 17) fict[threadIdx.x] = fict_g[threadIdx.x];
-  __syncthreads();
   const int64_t i = blockIdx.x * (int64_t)kFieldT + threadIdx.x;
   if (i >= n) return;
   const double px = x[i], py = y[i], pz = z[i];
@@ -406,35 +473,25 @@ __global__ __launch_bounds__(kFieldT) void lattice_field_kernel(int64_t n, const
   double wr = 1.0, wi = 0.0;     // O_mm
   for (int m = 0; m <= kLM; ++m) {
     if (m > 0) {
-      const double h = 0.5 / (double)m;
-      const double nr = (wr * rx + wi * ry) * h, ni = (wi * rx - wr * ry) * h;
-      wr = nr; wi = ni;
+      const double h = inv[kNM + m];          // 1 / (2m)
+      const double tr = (wr * rx + wi * ry) * h, ti = (wi * rx - wr * ry) * h;
+      wr = tr; wi = ti;
     }
-    const double f = m == 0 ? 1.0 : 2.0;
-    double p1r = 0.0, p1i = 0.0, p2r = 0.0, p2i = 0.0;
+    double cr = wr, ci = wi, p1r = 0.0, p1i = 0.0;
     for (int l = m; l <= kLM; ++l) {
-      double cr, ci;
-      if (l == m) { cr = wr; ci = wi; }
-      else {
-        const double a = (double)(2 * l - 1) * rz, s = inv[tix(l, m)];
-        cr = (a * p1r - r2 * p2r) * s;
-        ci = (a * p1i - r2 * p2i) * s;
+      if (l > m) {
+        const double a = (double)(2 * l - 1) * rz, sc = inv[tix(l, m)];
+        const double nr = (a * p1r - r2 * cr) * sc, ni = (a * p1i - r2 * ci) * sc;
+        cr = p1r; ci = p1i;           // O_{l-2}
+        p1r = nr; p1i = ni;           // O_l
+      } else {
+        p1r = wr; p1i = wi; cr = 0.0; ci = 0.0;
       }
-      const double2 o = make_double2(cr, ci);
-      phi += f * cmul(o, mu[tix(l, m)]).x;
-      if (l < kLM) {
-        l10 += f * cmul(o, mu[tix(l + 1, m)]).x;
-        const double2 a = cmul(o, mu[tix(l + 1, m + 1)]);
-        l11r += a.x; l11i += a.y;
-        if (m == 1) {
-          const double2 b = cmul(conj2(o), mu[tix(l + 1, 0)]);
-          l11r -= b.x; l11i -= b.y;
-        } else if (m >= 2) {
-          const double2 b = cmul(o, mu[tix(l + 1, m - 1)]);
-          l11r -= b.x; l11i += b.y;
-        }
-      }
-      p2r = p1r; p2i = p1i; p1r = cr; p1i = ci;
+      const double* k = coef + 8 * tix(l, m);
+      phi += p1r * k[0] + p1i * k[1];
+      l10 += p1r * k[2] + p1i * k[3];
+      l11r += p1r * k[4] + p1i * k[5];
+      l11i += p1r * k[6] + p1i * k[7];
     }
   }
   double e0 = -l11r, e1 = -l11i, e2 = -l10;
@@ -445,10 +502,10 @@ __global__ __launch_bounds__(kFieldT) void lattice_field_kernel(int64_t n, const
       const double dx = (px - vbox[3 * b]) - fx;
       const double dy = (py - vbox[3 * b + 1]) - fy;
       const double dz = (pz - vbox[3 * b + 2]) - fz;
-      const double r = sqrt(dx * dx + dy * dy + dz * dz);
-      const double rd = 1.0 / r;
-      const double rd3q = fq * rd * rd * rd;
-      phi += fq * rd;
+      const double rd = rsqrt(dx * dx + dy * dy + dz * dz);
+      const double qrd = fq * rd;
+      const double rd3q = qrd * rd * rd;
+      phi += qrd;
       e0 += rd3q * dx; e1 += rd3q * dy; e2 += rd3q * dz;
     }
   }
@@ -456,6 +513,20 @@ __global__ __launch_bounds__(kFieldT) void lattice_field_kernel(int64_t n, const
   ey[i] += sign * e1;
   ez[i] += sign * e2;
   pot[i] += sign * phi;
+}
+
+// inv[l(l+1)/2 + m] = 1 / ((l-m)(l+m)) for l > m; inv[kNM + m] = 1 / (2m).
+__global__ void lattice_inv_kernel(double* __restrict__ inv) {
+  for (int t = threadIdx.x; t < kNM + kLM + 1; t += blockDim.x) {
+    if (t < kNM) {
+      int l, m;
+      tdecode(t, l, m);
+      inv[t] = l > m ? 1.0 / (double)((l - m) * (l + m)) : 0.0;
+    } else {
+      const int m = t - kNM;
+      inv[t] = m > 0 ? 0.5 / (double)m : 0.0;
+    }
+  }
 }
 
 // constrain_periodic (module_mirror_boxes.f90:174-210): Fortran MODULO,
@@ -516,22 +587,27 @@ void launch_lattice_coefficients(hipStream_t s, const double* vec, int nb, doubl
 }
 
 int lattice_moment_blocks(int64_t n) {
-  int64_t b = (n + kMomT - 1) / kMomT;
+  // at least 16 particles per thread, at most 256 blocks per row
+  int64_t b = (n + kMomT * 16 - 1) / (kMomT * 16);
   if (b < 1) b = 1;
-  if (b > kLatticeMaxBlocks) b = kLatticeMaxBlocks;
+  if (b > 256) b = 256;
   return (int)b;
 }
 
+void launch_lattice_inv(hipStream_t s, double* inv) { lattice_inv_kernel<<<1, 256, 0, s>>>(inv); }
+
 void launch_lattice_far(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
                         const double* q, const LatticeGeom& g, const double2* ml, const double* fact,
-                        const double* vbox, int nb, double sign, double2* part, double* dpart,
-                        double2* mu, double* fict, double* ex, double* ey, double* ez,
-                        double* pot) {
+                        const double* inv, const double* vbox, int nb, double sign, double2* part,
+                        double* dpart, double2* om, double* dip, double2* mu, double* fict,
+                        double* coef, double* ex, double* ey, double* ez, double* pot) {
   const int nblk = lattice_moment_blocks(n);
   lattice_moments_kernel<<<dim3(nblk, kLM + 1), kMomT, 0, s>>>(n, x, y, z, q, g, part, dpart);
-  lattice_mu_kernel<<<1, 256, 0, s>>>(part, dpart, nblk, g, ml, fact, mu, fict);
+  lattice_omega_kernel<<<kNM + 3, kWave, 0, s>>>(part, dpart, nblk, om, dip);
+  lattice_mu_kernel<<<1, kMuT, 0, s>>>(om, dip, g, ml, fact, mu, fict, coef);
   const unsigned fb = (unsigned)((n + kFieldT - 1) / kFieldT);
-  lattice_field_kernel<<<fb, kFieldT, 0, s>>>(n, x, y, z, g, mu, fict, vbox, nb, sign, ex, ey, ez, pot);
+  lattice_field_kernel<<<fb, kFieldT, 0, s>>>(n, x, y, z, g, coef, inv, fict, vbox, nb, sign, ex, ey,
+                                              ez, pot);
 }
 
 }  // namespace pepc

@@ -100,7 +100,8 @@ struct Ctx {
   double* vbox = nullptr;           // 3 * nbox lattice vectors, central box last
   double* fact = nullptr;           // factorials 0..80
   double2 *ml = nullptr, *mstar = nullptr, *lstar = nullptr, *mu = nullptr, *lpart = nullptr;
-  double *ldpart = nullptr, *fict = nullptr;
+  double *ldpart = nullptr, *fict = nullptr, *linv = nullptr, *ldip = nullptr, *lcoef = nullptr;
+  double2* lom = nullptr;
 };
 
 int fail(Ctx* c, int code, const std::string& msg) {
@@ -152,7 +153,7 @@ void free_all(Ctx* c) {
                   c->nadj, c->parent, c->leafof, c->aqueue, c->lcount, c->loff, c->lent,
                   c->ltotal, c->nparent, c->echunk, c->esyschunk, c->epartial, c->prec4,
                   c->vbox, c->fact, c->ml, c->mstar, c->lstar, c->mu, c->lpart, c->ldpart,
-                  c->fict};
+                  c->fict, c->linv, c->ldip, c->lom, c->lcoef};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (double* p : c->st)
@@ -363,8 +364,9 @@ int traverse_device_impl(Ctx* c, double* ex, double* ey, double* ez, double* pot
   RECORD(c, 2);
   launch_walk(s, a, c->cfg.precision, c->walk_blocks, c->cfg.list_tile);
   if (c->periodic && c->far_field)
-    launch_lattice_far(s, c->n, c->xs, c->ys, c->zs, c->qs, c->lgeom, c->ml, c->fact, c->vbox,
-                       c->nbox, c->cfg.sign, c->lpart, c->ldpart, c->mu, c->fict, ex, ey, ez, pot);
+    launch_lattice_far(s, c->n, c->xs, c->ys, c->zs, c->qs, c->lgeom, c->ml, c->fact, c->linv,
+                       c->vbox, c->nbox, c->cfg.sign, c->lpart, c->ldpart, c->lom, c->ldip, c->mu,
+                       c->fict, c->lcoef, ex, ey, ez, pot);
   RECORD(c, 3);
   launch_walk_stats(s, a);
   HIPCHECK(c, hipGetLastError());
@@ -1252,6 +1254,9 @@ int pepc_gpu_set_periodic(void* ctx, const pepc_gpu_periodic* p) {
     ALLOC(c, c->ml, kLatticeNT); ALLOC(c, c->mstar, kLatticeNM); ALLOC(c, c->lstar, kLatticeNT);
     ALLOC(c, c->mu, kLatticeNM); ALLOC(c, c->lpart, (int64_t)kLatticeMaxBlocks * kLatticeNM);
     ALLOC(c, c->ldpart, (int64_t)kLatticeMaxBlocks * 3); ALLOC(c, c->fict, 17);
+    ALLOC(c, c->linv, kLatticeNM + kLatticeLM + 1); ALLOC(c, c->ldip, 3);
+    ALLOC(c, c->lom, kLatticeNM); ALLOC(c, c->lcoef, 8 * kLatticeNM);
+    launch_lattice_inv(s, c->linv);
     f.resize(kLatticeFact);
     for (int k = 0; k < kLatticeFact; ++k) f[k] = ref_factorial(k);
     HIPCHECK(c, hipMemcpyAsync(c->fact, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice, s));
