@@ -123,6 +123,30 @@ int pepc_gpu_grow_device(void *ctx, int64_t n, const double *x, const double *y,
 int pepc_gpu_traverse_device(void *ctx, double *ex, double *ey, double *ez, double *pot,
                              double *work);
 
+/* Traversal of other particles: replaces pepc_traverse_tree(particles) for
+ * a particle array that is not the grown one (pepc_grow_and_traverse_for_others,
+ * module_pepc.f90:394-433; libpepc_traverse_tree :161-200).  Fields and
+ * potentials at m sink positions x,y,z (device arrays, any order; outputs in
+ * the same order) from the grown tree, periodic boundaries included.
+ * PEPC_GPU_SINKS_PER_PARTICLE: every sink walks alone, the reference's rule
+ * for particles without keys (module_tree_walk_simple.f90:155-159), so
+ * counts equal the reference's.  PEPC_GPU_SINKS_GROUPED (no reference
+ * counterpart): sinks sorted along the source box's Hilbert curve (clamped
+ * into the box) walk in runs of group_size; same physics, fewer traversals,
+ * other counts.  A sink at exactly a source's position skips that source
+ * (zero separation is self, :331-340).  work[i] = interactions of sink i
+ * (the reference adds them to %work).  Not with batched systems.
+ * The _device form is asynchronous; the host form copies and synchronises. */
+enum { PEPC_GPU_SINKS_PER_PARTICLE = 0, PEPC_GPU_SINKS_GROUPED = 1 };
+int pepc_gpu_traverse_sinks_device(void *ctx, int64_t m, const double *x, const double *y,
+                                   const double *z, int32_t mode, double *ex, double *ey,
+                                   double *ez, double *pot, double *work);
+int pepc_gpu_traverse_sinks(void *ctx, int64_t m, const double *x, const double *y,
+                            const double *z, int32_t mode, double *ex, double *ey, double *ez,
+                            double *pot, double *work);
+/* Per-sink P2P and M2P counts of the last sink traversal (sinks' order). */
+int pepc_gpu_read_sink_counts(void *ctx, int64_t cap, uint32_t *p2p, uint32_t *m2p);
+
 /* Waits for the context's work, reads its counters into *st (may be NULL)
  * and reports errors raised by the kernels. */
 int pepc_gpu_sync(void *ctx, pepc_gpu_stats *st);

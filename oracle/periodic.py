@@ -178,6 +178,35 @@ def walk_group_boxes(x, y, z, tree, theta2, eps2, vbox, tile_size=16, sign=1.0, 
                          int(tot[0]), int(tot[1]), int(tot[2]))
 
 
+def walk_sinks(sx, sy, sz, tree, nsources, theta2, eps2, vbox=((0.0, 0.0, 0.0),), sign=1.0,
+               nthreads=0, tiles=None):
+    """Walk of sink points through a tree of nsources particles: every sink
+    its own tile (module_tree_walk_simple.f90:155-159) unless tiles are
+    given; self = zero separation from a leaf's particle (:331-340)."""
+    m = len(sx)
+    if tiles is None:
+        tiles = (np.arange(m, dtype=np.int64), np.ones(m, dtype=np.int64))
+    tf, tc = tiles
+    ex, ey, ez, pot, work = (np.zeros(m) for _ in range(5))
+    p2p = np.zeros(m, np.int32)
+    m2p = np.zeros(m, np.int32)
+    tmac = np.zeros(len(tf), np.int64)
+    tot = np.zeros(3, np.int64)
+    vb = np.ascontiguousarray(np.asarray(vbox, np.float64).reshape(-1))
+    c = _o._c
+    rc = _lib().or_walk_group_boxes(nsources, c(sx, np.float64), c(sy, np.float64),
+                                    c(sz, np.float64), tree.nnodes, tree.level, tree.leaves,
+                                    tree.first_child, tree.next_sibling,
+                                    np.ascontiguousarray(tree.mom.ravel()), tree.b2, float(theta2),
+                                    float(eps2), float(sign), len(tf), c(tf, np.int64),
+                                    c(tc, np.int64), 1, int(nthreads), len(vb) // 3, vb, ex, ey, ez,
+                                    pot, work, p2p, m2p, tmac, tot)
+    if rc:
+        raise ValueError("walk did not reach every leaf")
+    return _o.WalkResult(np.stack([ex, ey, ez], 1), pot, work, p2p, m2p, tf, tc, tmac,
+                         int(tot[0]), int(tot[1]), int(tot[2]))
+
+
 def solve_periodic(x, y, z, q, theta2, eps2, ml, lattice=(1.0, 1.0, 1.0),
                    origin=(0.0, 0.0, 0.0), layers=1, far_field=True, tile_size=16, sign=1.0,
                    nthreads=0):

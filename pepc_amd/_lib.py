@@ -29,7 +29,11 @@ EXPORTS = [
     "pepc_gpu_adjoint_device", "pepc_gpu_autotune", "pepc_gpu_get_config", "pepc_gpu_freeze",
     "pepc_gpu_replay_device", "pepc_gpu_restore_device", "pepc_gpu_periodic_default",
     "pepc_gpu_set_periodic", "pepc_gpu_lattice_get", "pepc_gpu_constrain_periodic_device",
+    "pepc_gpu_traverse_sinks_device", "pepc_gpu_traverse_sinks", "pepc_gpu_read_sink_counts",
 ]
+
+SINKS_PER_PARTICLE = 0
+SINKS_GROUPED = 1
 
 RUN_GRAPH = 1
 
@@ -128,6 +132,9 @@ def load(path: str = LIB_PATH):
     L.pepc_gpu_set_periodic.argtypes = [_vp, C.POINTER(Periodic)]
     L.pepc_gpu_lattice_get.argtypes = [_vp, _vp, _vp, _vp, C.POINTER(C.c_int32)]
     L.pepc_gpu_constrain_periodic_device.argtypes = [_vp, _i64, _vp, _vp, _vp]
+    L.pepc_gpu_traverse_sinks_device.argtypes = [_vp, _i64, _vp, _vp, _vp, C.c_int32] + [_vp] * 5
+    L.pepc_gpu_traverse_sinks.argtypes = [_vp, _i64, _vp, _vp, _vp, C.c_int32] + [_vp] * 5
+    L.pepc_gpu_read_sink_counts.argtypes = [_vp, _i64, _vp, _vp]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is not None and name not in ("pepc_gpu_config_default",

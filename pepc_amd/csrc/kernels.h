@@ -19,6 +19,12 @@ void launch_box_batch(hipStream_t s, int nsys, const int64_t* soff, const double
 void launch_keys(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
                  const double* q, const DevBox* boxes, const uint32_t* sys,
                  unsigned long long* keys, uint32_t* idx, double4* rec);
+// Keys of points that need not lie in the box (sinks): coordinates clamped
+// into the box first; idx[i] = i.
+void launch_sink_keys(hipStream_t s, int64_t m, const double* x, const double* y, const double* z,
+                      const DevBox* box, unsigned long long* keys, uint32_t* idx);
+// Runs of g consecutive sinks: gfirst[k] = k g, gcount[k] = min(g, m - k g).
+void launch_sink_groups(hipStream_t s, int64_t m, int g, uint32_t* gfirst, uint32_t* gcount);
 
 // sort.hip
 int64_t scan_blocks(int64_t n);
@@ -123,8 +129,14 @@ struct WalkArgs {
   const uint32_t* sysroot;           // batched systems: root node per system
   int nbox;                          // periodic: lattice vectors walked per group (else 1)
   const double* vbox;                // periodic: 3 * nbox lattice vectors, (0,0,0) last
+  // sinks (targets other than the grown particles; tx == nullptr otherwise):
+  // target coordinates, group count, and gfirst == nullptr for one sink per
+  // group.  work = interactions (no +1); gstat is not written.
+  const double *tx = nullptr, *ty = nullptr, *tz = nullptr;
+  uint32_t nsink_groups = 0;
 };
 void launch_walk_reset(hipStream_t s, const WalkArgs& a);
+void launch_queue_reset(hipStream_t s, uint32_t* queue);   // work-queue head only
 // list_tile: M2P list capacity in LDS (0 = default: 128 FP32, 64 FP64;
 // FP32 takes 64 or 128, FP64 32 or 64).
 bool walk_list_tile_ok(int precision, int list_tile);
@@ -164,6 +176,16 @@ void launch_lattice_far(hipStream_t s, int64_t n, const double* x, const double*
                         const double* inv, const double* vbox, int nb, double sign, double2* part,
                         double* dpart, double2* om, double* dip, double2* mu, double* fict,
                         double* coef, double* ex, double* ey, double* ez, double* pot);
+// The field part alone at m other points (mu, fict and coef of the sources
+// already computed by launch_lattice_far or launch_lattice_sources).
+void launch_lattice_sources(hipStream_t s, int64_t n, const double* x, const double* y,
+                            const double* z, const double* q, const LatticeGeom& g,
+                            const double2* ml, const double* fact, double2* part, double* dpart,
+                            double2* om, double* dip, double2* mu, double* fict, double* coef);
+void launch_lattice_field(hipStream_t s, int64_t m, const double* x, const double* y, const double* z,
+                          const LatticeGeom& g, const double* coef, const double* inv,
+                          const double* fict, const double* vbox, int nb, double sign, double* ex,
+                          double* ey, double* ez, double* pot);
 void launch_constrain_periodic(hipStream_t s, int64_t n, double* x, double* y, double* z,
                                const LatticeGeom& g);
 

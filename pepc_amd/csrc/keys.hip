@@ -190,7 +190,54 @@ __global__ __launch_bounds__(256) void keys_kernel(int64_t n, const double* __re
   idx[i] = (uint32_t)i;
 }
 
+// Points that need not lie inside the box (sinks): each coordinate clamped to
+// [boxmin, boxmax] and the cell index to 2^20 - 1 before the same key map.
+__global__ __launch_bounds__(256) void sink_keys_kernel(int64_t m, const double* __restrict__ x,
+                                                        const double* __restrict__ y,
+                                                        const double* __restrict__ z,
+                                                        const DevBox* __restrict__ box,
+                                                        unsigned long long* __restrict__ keys,
+                                                        uint32_t* __restrict__ idx) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double p[3] = {x[i], y[i], z[i]};
+  uint32_t c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double v = fmin(fmax(p[d], box->boxmin[d]), box->boxmax[d]);
+    long long k = (long long)((v - box->boxmin[d]) / box->cell[d]);
+    k = k < 0 ? 0 : (k > (1ll << kMaxLevel) - 1 ? (1ll << kMaxLevel) - 1 : k);
+    c[d] = (uint32_t)k;
+  }
+  keys[i] = hilbert3d(c[0], c[1], c[2]);
+  idx[i] = (uint32_t)i;
+}
+
+__global__ void sink_groups_kernel(int64_t m, int g, uint32_t* __restrict__ gfirst,
+                                   uint32_t* __restrict__ gcount) {
+  const int64_t ng = (m + g - 1) / g;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ng;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t f = k * g;
+    gfirst[k] = (uint32_t)f;
+    gcount[k] = (uint32_t)(m - f < g ? m - f : g);
+  }
+}
+
 }  // namespace
+
+void launch_sink_keys(hipStream_t s, int64_t m, const double* x, const double* y, const double* z,
+                      const DevBox* box, unsigned long long* keys, uint32_t* idx) {
+  if (m <= 0) return;
+  sink_keys_kernel<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(m, x, y, z, box, keys, idx);
+}
+
+void launch_sink_groups(hipStream_t s, int64_t m, int g, uint32_t* gfirst, uint32_t* gcount) {
+  const int64_t ng = (m + g - 1) / g;
+  int64_t b = (ng + 255) / 256;
+  b = b < 1 ? 1 : (b > 4096 ? 4096 : b);
+  sink_groups_kernel<<<(unsigned)b, 256, 0, s>>>(m, g, gfirst, gcount);
+}
 
 int box_partial_blocks(int64_t n) {
   int64_t b = (n + kBoxThreads * 16 - 1) / (kBoxThreads * 16);

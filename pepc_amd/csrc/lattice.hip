@@ -596,18 +596,33 @@ int lattice_moment_blocks(int64_t n) {
 
 void launch_lattice_inv(hipStream_t s, double* inv) { lattice_inv_kernel<<<1, 256, 0, s>>>(inv); }
 
+void launch_lattice_sources(hipStream_t s, int64_t n, const double* x, const double* y,
+                            const double* z, const double* q, const LatticeGeom& g,
+                            const double2* ml, const double* fact, double2* part, double* dpart,
+                            double2* om, double* dip, double2* mu, double* fict, double* coef) {
+  const int nblk = lattice_moment_blocks(n);
+  lattice_moments_kernel<<<dim3(nblk, kLM + 1), kMomT, 0, s>>>(n, x, y, z, q, g, part, dpart);
+  lattice_omega_kernel<<<kNM + 3, kWave, 0, s>>>(part, dpart, nblk, om, dip);
+  lattice_mu_kernel<<<1, kMuT, 0, s>>>(om, dip, g, ml, fact, mu, fict, coef);
+}
+
+void launch_lattice_field(hipStream_t s, int64_t m, const double* x, const double* y, const double* z,
+                          const LatticeGeom& g, const double* coef, const double* inv,
+                          const double* fict, const double* vbox, int nb, double sign, double* ex,
+                          double* ey, double* ez, double* pot) {
+  if (m <= 0) return;
+  const unsigned fb = (unsigned)((m + kFieldT - 1) / kFieldT);
+  lattice_field_kernel<<<fb, kFieldT, 0, s>>>(m, x, y, z, g, coef, inv, fict, vbox, nb, sign, ex, ey,
+                                              ez, pot);
+}
+
 void launch_lattice_far(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
                         const double* q, const LatticeGeom& g, const double2* ml, const double* fact,
                         const double* inv, const double* vbox, int nb, double sign, double2* part,
                         double* dpart, double2* om, double* dip, double2* mu, double* fict,
                         double* coef, double* ex, double* ey, double* ez, double* pot) {
-  const int nblk = lattice_moment_blocks(n);
-  lattice_moments_kernel<<<dim3(nblk, kLM + 1), kMomT, 0, s>>>(n, x, y, z, q, g, part, dpart);
-  lattice_omega_kernel<<<kNM + 3, kWave, 0, s>>>(part, dpart, nblk, om, dip);
-  lattice_mu_kernel<<<1, kMuT, 0, s>>>(om, dip, g, ml, fact, mu, fict, coef);
-  const unsigned fb = (unsigned)((n + kFieldT - 1) / kFieldT);
-  lattice_field_kernel<<<fb, kFieldT, 0, s>>>(n, x, y, z, g, coef, inv, fict, vbox, nb, sign, ex, ey,
-                                              ez, pot);
+  launch_lattice_sources(s, n, x, y, z, q, g, ml, fact, part, dpart, om, dip, mu, fict, coef);
+  launch_lattice_field(s, n, x, y, z, g, coef, inv, fict, vbox, nb, sign, ex, ey, ez, pot);
 }
 
 }  // namespace pepc

@@ -102,6 +102,15 @@ struct Ctx {
   double2 *ml = nullptr, *mstar = nullptr, *lstar = nullptr, *mu = nullptr, *lpart = nullptr;
   double *ldpart = nullptr, *fict = nullptr, *linv = nullptr, *ldip = nullptr, *lcoef = nullptr;
   double2* lom = nullptr;
+  // sinks (traversal of other particles)
+  int64_t scap = 0, sk_m = 0;
+  int sk_mode = 0;
+  double *sk_x = nullptr, *sk_y = nullptr, *sk_z = nullptr;
+  double* sk_res[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  double* sk_io[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  unsigned long long *sk_k0 = nullptr, *sk_k1 = nullptr;
+  uint32_t *sk_v0 = nullptr, *sk_v1 = nullptr, *sk_gfirst = nullptr, *sk_gcount = nullptr;
+  uint32_t *sk_p2p = nullptr, *sk_m2p = nullptr, *sk_hist = nullptr, *sk_bsum = nullptr;
 };
 
 int fail(Ctx* c, int code, const std::string& msg) {
@@ -153,7 +162,11 @@ void free_all(Ctx* c) {
                   c->nadj, c->parent, c->leafof, c->aqueue, c->lcount, c->loff, c->lent,
                   c->ltotal, c->nparent, c->echunk, c->esyschunk, c->epartial, c->prec4,
                   c->vbox, c->fact, c->ml, c->mstar, c->lstar, c->mu, c->lpart, c->ldpart,
-                  c->fict, c->linv, c->ldip, c->lom, c->lcoef};
+                  c->fict, c->linv, c->ldip, c->lom, c->lcoef, c->sk_x, c->sk_y, c->sk_z,
+                  c->sk_res[0], c->sk_res[1], c->sk_res[2], c->sk_res[3], c->sk_res[4],
+                  c->sk_io[0], c->sk_io[1], c->sk_io[2], c->sk_io[3], c->sk_io[4], c->sk_io[5],
+                  c->sk_io[6], c->sk_io[7], c->sk_k0, c->sk_k1, c->sk_v0, c->sk_v1, c->sk_gfirst,
+                  c->sk_gcount, c->sk_p2p, c->sk_m2p, c->sk_hist, c->sk_bsum};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (double* p : c->st)
@@ -404,6 +417,89 @@ int sync_impl(Ctx* c, pepc_gpu_stats* st) {
   return check_flags(c, h.flags);
 }
 
+int ensure_sinks(Ctx* c, int64_t m) {
+  if (m <= c->scap) return 0;
+  const int64_t mc = m + m / 8 + 64;
+  ALLOC(c, c->sk_x, mc); ALLOC(c, c->sk_y, mc); ALLOC(c, c->sk_z, mc);
+  for (int k = 0; k < 5; ++k) ALLOC(c, c->sk_res[k], mc);
+  for (int k = 0; k < 8; ++k) ALLOC(c, c->sk_io[k], mc);
+  ALLOC(c, c->sk_k0, mc); ALLOC(c, c->sk_k1, mc);
+  ALLOC(c, c->sk_v0, mc); ALLOC(c, c->sk_v1, mc);
+  ALLOC(c, c->sk_gfirst, mc); ALLOC(c, c->sk_gcount, mc);
+  ALLOC(c, c->sk_p2p, mc); ALLOC(c, c->sk_m2p, mc);
+  const int64_t hc = 256 * sort_blocks(mc);
+  ALLOC(c, c->sk_hist, hc);
+  int64_t bc = scan_blocks(hc) + 64;
+  if (bc < 512) bc = 512;
+  ALLOC(c, c->sk_bsum, bc);
+  c->scap = mc;
+  return 0;
+}
+
+// Traversal of m sinks (device arrays) through the grown tree; outputs in
+// the sinks' order.  mode 0: one sink per walk group, in the given order;
+// mode 1: sinks sorted along the Hilbert curve of the source box, runs of
+// group_size per group, results scattered back.
+int traverse_sinks_impl(Ctx* c, int64_t m, const double* x, const double* y, const double* z,
+                        int mode, double* ex, double* ey, double* ez, double* pot, double* work) {
+  if (!c->grown) return fail(c, PEPC_GPU_EINVAL, "traverse_sinks: no tree (call grow first)");
+  if (c->nsys > 1) return fail(c, PEPC_GPU_EINVAL, "traverse_sinks: needs a single grown system");
+  if (m < 0 || m > (int64_t)0x7fffffff) return fail(c, PEPC_GPU_EINVAL, "traverse_sinks: bad sink count");
+  if (mode != PEPC_GPU_SINKS_PER_PARTICLE && mode != PEPC_GPU_SINKS_GROUPED)
+    return fail(c, PEPC_GPU_EINVAL, "traverse_sinks: unknown mode");
+  if (m == 0) return 0;
+  if (!x || !y || !z || !ex || !ey || !ez || !pot || !work)
+    return fail(c, PEPC_GPU_EINVAL, "traverse_sinks: null argument");
+  int r = ensure_sinks(c, m);
+  if (r) return r;
+  hipStream_t s = c->stream;
+  WalkArgs a;
+  a.n = c->n;
+  a.x = c->xs; a.y = c->ys; a.z = c->zs; a.q = c->qs;
+  a.geom = c->geom; a.link = c->link; a.wm32 = c->wm32; a.wm64 = c->wm64;
+  a.sc = c->sc;
+  a.theta2 = c->cfg.theta2; a.eps2 = c->cfg.eps2; a.sign = c->cfg.sign;
+  a.np2p = c->sk_p2p; a.nm2p = c->sk_m2p; a.gstat = nullptr; a.queue = c->queue;
+  a.sys = nullptr; a.sysroot = c->sysroot;
+  a.nbox = c->periodic ? c->nbox : 1;
+  a.vbox = c->periodic ? c->vbox : nullptr;
+  const double *tx = x, *ty = y, *tz = z;
+  double* out[5] = {ex, ey, ez, pot, work};
+  if (mode == PEPC_GPU_SINKS_GROUPED) {
+    launch_sink_keys(s, m, x, y, z, c->box, c->sk_k0, c->sk_v0);
+    launch_sort_keys(s, c->sk_k0, c->sk_v0, c->sk_k1, c->sk_v1, m, c->sk_hist, c->sk_bsum, nullptr, 1);
+    // the fourth gathered array is a scratch copy of x (gather4 moves four)
+    launch_gather4(s, m, c->sk_v0, x, y, z, x, c->sk_x, c->sk_y, c->sk_z, c->sk_res[4]);
+    const int G = c->cfg.group_size;
+    launch_sink_groups(s, m, G, c->sk_gfirst, c->sk_gcount);
+    a.gfirst = c->sk_gfirst; a.gcount = c->sk_gcount;
+    a.nsink_groups = (uint32_t)((m + G - 1) / G);
+    tx = c->sk_x; ty = c->sk_y; tz = c->sk_z;
+    for (int k = 0; k < 5; ++k) out[k] = c->sk_res[k];
+  } else {
+    a.gfirst = nullptr; a.gcount = nullptr;
+    a.nsink_groups = (uint32_t)m;
+  }
+  a.tx = tx; a.ty = ty; a.tz = tz;
+  a.ex = out[0]; a.ey = out[1]; a.ez = out[2]; a.pot = out[3]; a.work = out[4];
+  launch_queue_reset(s, c->queue);
+  launch_walk(s, a, c->cfg.precision, c->walk_blocks, c->cfg.list_tile);
+  if (c->periodic && c->far_field) {
+    launch_lattice_sources(s, c->n, c->xs, c->ys, c->zs, c->qs, c->lgeom, c->ml, c->fact, c->lpart,
+                           c->ldpart, c->lom, c->ldip, c->mu, c->fict, c->lcoef);
+    launch_lattice_field(s, m, tx, ty, tz, c->lgeom, c->lcoef, c->linv, c->fict, c->vbox, c->nbox,
+                         c->cfg.sign, out[0], out[1], out[2], out[3]);
+  }
+  if (mode == PEPC_GPU_SINKS_GROUPED) {
+    double* dst[5] = {ex, ey, ez, pot, work};
+    for (int k = 0; k < 5; ++k) launch_scatter_back(s, m, c->sk_v0, c->sk_res[k], dst[k]);
+  }
+  HIPCHECK(c, hipGetLastError());
+  c->sk_m = m;
+  c->sk_mode = mode;
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -560,6 +656,63 @@ int pepc_gpu_grow(void* p, int64_t n, const double* x, const double* y, const do
   const double* dsrc[3] = {c->xs, c->ys, c->zs};
   for (int k = 0; k < 3; ++k)
     if (dst[k]) HIPCHECK(c, hipMemcpy(dst[k], dsrc[k], sizeof(double) * n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pepc_gpu_traverse_sinks_device(void* p, int64_t m, const double* x, const double* y,
+                                   const double* z, int32_t mode, double* ex, double* ey, double* ez,
+                                   double* pot, double* work) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PEPC_GPU_EINVAL;
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  return traverse_sinks_impl(c, m, x, y, z, mode, ex, ey, ez, pot, work);
+}
+
+int pepc_gpu_traverse_sinks(void* p, int64_t m, const double* x, const double* y, const double* z,
+                            int32_t mode, double* ex, double* ey, double* ez, double* pot,
+                            double* work) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PEPC_GPU_EINVAL;
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  if (m < 0) return fail(c, PEPC_GPU_EINVAL, "traverse_sinks: bad sink count");
+  if (m == 0) return 0;
+  if (!x || !y || !z) return fail(c, PEPC_GPU_EINVAL, "traverse_sinks: null argument");
+  int r = ensure_sinks(c, m);
+  if (r) return r;
+  const double* src[3] = {x, y, z};
+  for (int k = 0; k < 3; ++k)
+    HIPCHECK(c, hipMemcpyAsync(c->sk_io[k], src[k], sizeof(double) * m, hipMemcpyHostToDevice, c->stream));
+  r = traverse_sinks_impl(c, m, c->sk_io[0], c->sk_io[1], c->sk_io[2], mode, c->sk_io[3], c->sk_io[4],
+                          c->sk_io[5], c->sk_io[6], c->sk_io[7]);
+  if (r) return r;
+  r = sync_impl(c, nullptr);
+  if (r) return r;
+  double* dst[5] = {ex, ey, ez, pot, work};
+  for (int k = 0; k < 5; ++k)
+    if (dst[k]) HIPCHECK(c, hipMemcpy(dst[k], c->sk_io[3 + k], sizeof(double) * m, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pepc_gpu_read_sink_counts(void* p, int64_t cap, uint32_t* p2p, uint32_t* m2p) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PEPC_GPU_EINVAL;
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  const int64_t m = c->sk_m;
+  if (cap < m) return fail(c, PEPC_GPU_EINVAL, "read_sink_counts: capacity below the sink count");
+  if (m == 0) return 0;
+  std::vector<uint32_t> a(m), b(m), perm;
+  HIPCHECK(c, hipMemcpy(a.data(), c->sk_p2p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(b.data(), c->sk_m2p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+  if (c->sk_mode == PEPC_GPU_SINKS_GROUPED) {
+    perm.resize(m);
+    HIPCHECK(c, hipMemcpy(perm.data(), c->sk_v0, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+  }
+  for (int64_t i = 0; i < m; ++i) {
+    const int64_t d = perm.empty() ? i : (int64_t)perm[i];
+    if (p2p) p2p[d] = a[i];
+    if (m2p) m2p[d] = b[i];
+  }
   return 0;
 }
 

@@ -33,6 +33,14 @@
 // of each box; FP32 puts the shift into the node frame instead (node entries
 // relative to first particle - vbox), so its lists carry over.
 //
+// Sinks (SINK): targets other than the grown particles (the reference's
+// traversal of another particle array, module_pepc.f90:394-433).  Targets
+// come from their own arrays, one per group (the reference's rule for
+// particles without keys, module_tree_walk_simple.f90:155-159) or in given
+// runs; self is the reference's test, a zero separation from the leaf's
+// particle (:331-340), checked exactly in FP64 against the source
+// coordinates; lists are flushed at the end of each box.
+//
 // The M2P evaluation regroups the reference's expression
 // (module_coulomb_kernels.f90:60-145) around the quadratic form
 // S = d.A.d of the quadrupole tensor; it is the same polynomial in d and the
@@ -145,7 +153,7 @@ __device__ __forceinline__ bool mac_fails(double xt, double yt, double zt, doubl
 
 // LM: capacity of the M2P list staged in LDS (the list tile); the P2P list
 // holds kListP entries.
-template <typename R, int LM, bool PER>
+template <typename R, int LM, bool PER, bool SINK>
 __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
   using P = Prec<R>;
   using V4 = typename P::V4;
@@ -159,7 +167,10 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
   const unsigned long long lt = lanemask_lt();
   const R eps2 = (R)a.eps2;
   const double theta2 = a.theta2;
-  const uint32_t ngroups = a.sc->ngroups;
+  const uint32_t ngroups = SINK ? a.nsink_groups : a.sc->ngroups;
+  const double* __restrict__ TX = SINK ? a.tx : a.x;
+  const double* __restrict__ TY = SINK ? a.ty : a.y;
+  const double* __restrict__ TZ = SINK ? a.tz : a.z;
 
   // Work queue: the first group of each wave is its block index, later ones
   // come from a counter on a cache line of its own.
@@ -175,16 +186,17 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
     // Lanes s*cnt .. s*cnt+cnt-1 form slice s (s < S = 64/cnt) and hold the
     // group's targets in order; the 64 - S*cnt remaining lanes hold copies
     // of targets for the box and take no part in the sums.
-    const uint32_t a0 = a.gfirst[g];
-    const uint32_t cnt = a.gcount[g];
+    uint32_t a0, cnt;
+    if (!SINK || a.gfirst) { a0 = a.gfirst[g]; cnt = a.gcount[g]; }
+    else { a0 = g; cnt = 1u; }
     const uint32_t T = cnt;
     const uint32_t S = kWave / T;
     const uint32_t s = (uint32_t)lane / T;
     const uint32_t t = (uint32_t)lane - s * T;
     const bool valid = s < S;
     const uint32_t tid = a0 + t;
-    const double xt = a.x[tid], yt = a.y[tid], zt = a.z[tid];
-    const double ox = a.x[a0], oy = a.y[a0], oz = a.z[a0];
+    const double xt = TX[tid], yt = TY[tid], zt = TZ[tid];
+    const double ox = TX[a0], oy = TY[a0], oz = TZ[a0];
     const double lox = wave_min(xt), loy = wave_min(yt), loz = wave_min(zt);
     const double hix = wave_max(xt), hiy = wave_max(yt), hiz = wave_max(zt);
     // target position in the kernel's frame (FP64: set per box below)
@@ -200,7 +212,8 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
     uint32_t sp = 0, nm = 0, np = 0, cnt_p2p = 0, cnt_m2p = 0, namb = 0, nsteps = 0;
     unsigned long long nmac = 0;
     bool overflow = false;
-    const uint32_t root = a.sys ? a.sysroot[a.sys[a0]] : 0u;   // the group's own root
+    const uint32_t root = (!SINK && a.sys) ? a.sysroot[a.sys[a0]] : 0u;   // the group's own root
+    double sx = xt, sy = yt, sz = zt;    // targets of the current box: x - vbox
 
     // Batches: entry k goes to slice k mod S; the loop count is uniform.
     // FP32: two entries per pass (their LDS reads issued together), then the
@@ -233,7 +246,13 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
         const uint32_t k = kk + s;
         if (valid && k < np) {
           const V4 e = lp[k];
-          const bool self = lpi[k] == tid;   // mirror-box entries hold ~0u
+          bool self;
+          if constexpr (SINK) {
+            const uint32_t j = lpi[k];
+            self = sx == a.x[j] && sy == a.y[j] && sz == a.z[j];
+          } else {
+            self = lpi[k] == tid;   // mirror-box entries hold ~0u
+          }
           p2p<R>(tx - (R)e.x, ty - (R)e.y, tz - (R)e.z, (R)e.w, eps2, self, acc.p);
         }
       }
@@ -247,7 +266,7 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
     const bool central = !PER || (vx == 0.0 && vy == 0.0 && vz == 0.0);
     // targets of this box: x - vbox; group box likewise (rounding is monotone,
     // so the shifted extremes are the extremes of the shifted targets)
-    const double sx = xt - vx, sy = yt - vy, sz = zt - vz;
+    sx = xt - vx; sy = yt - vy; sz = zt - vz;
     const double blox = lox - vx, bloy = loy - vy, bloz = loz - vz;
     const double bhix = hix - vx, bhiy = hiy - vy, bhiz = hiz - vz;
     const double fox = ox - vx, foy = oy - vy, foz = oz - vz;   // FP32 node frame
@@ -362,7 +381,7 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
           e = make_double4(gm.cx, gm.cy, gm.cz, gm.b2);
         }
         lp[pos] = e;
-        lpi[pos] = central ? lk.first : ~0u;
+        lpi[pos] = (SINK || central) ? lk.first : ~0u;
       }
       const uint32_t nma = (uint32_t)popc64(am), npa = (uint32_t)popc64(pm);
       nm += nma; np += npa; cnt_m2p += nma; cnt_p2p += npa;
@@ -375,7 +394,7 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
       sp += tot;
       __syncthreads();
     }
-    if constexpr (PER && sizeof(R) == 8) {   // next box shifts the targets
+    if constexpr (PER && (sizeof(R) == 8 || SINK)) {   // next box shifts the targets
       __syncthreads();
       if (nm) flush_m2p();
       if (np) flush_p2p();
@@ -403,11 +422,11 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
       a.ey[tid] = out[1];
       a.ez[tid] = out[2];
       a.pot[tid] = out[3];
-      a.work[tid] = 1.0 + (double)(cnt_p2p + cnt_m2p);
+      a.work[tid] = (SINK ? 0.0 : 1.0) + (double)(cnt_p2p + cnt_m2p);
       a.np2p[tid] = cnt_p2p;
       a.nm2p[tid] = cnt_m2p;
     }
-    if (lane == 0) {   // per-group record, summed by walk_stats_kernel
+    if (!SINK && lane == 0) {   // per-group record, summed by walk_stats_kernel
       uint4* gs = reinterpret_cast<uint4*>(a.gstat + 8 * (size_t)g);
       gs[0] = make_uint4((uint32_t)nmac, cnt_p2p, cnt_m2p, namb);
       gs[1] = make_uint4(nsteps, cnt, 0u, 0u);
@@ -463,6 +482,10 @@ __global__ void walk_reset_kernel(DevScalars* sc, uint32_t* queue) {
     sc->amb_nodes = 0;
     sc->walk_steps = 0;
   }
+}
+
+__global__ void queue_reset_kernel(uint32_t* queue) {
+  if (threadIdx.x == 0) *queue = 0;
 }
 
 // FP64 direct sum (src/utils/module_directsum.f90:34-172): all sources
@@ -527,12 +550,12 @@ int walk_default_blocks(int precision, int list_tile) {
   hipError_t e;
   if (precision == kWalkFP64)
     e = list_tile == 32
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<double, 32, false>, kWave, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<double, 64, false>, kWave, 0);
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<double, 32, false, false>, kWave, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<double, 64, false, false>, kWave, 0);
   else
     e = list_tile == 64
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<float, 64, false>, kWave, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<float, 128, false>, kWave, 0);
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<float, 64, false, false>, kWave, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<float, 128, false, false>, kWave, 0);
   if (e != hipSuccess || per < 1) per = 8;
   return cus * per;
 }
@@ -541,33 +564,38 @@ void launch_walk_reset(hipStream_t s, const WalkArgs& a) {
   walk_reset_kernel<<<1, 64, 0, s>>>(a.sc, a.queue);
 }
 
+void launch_queue_reset(hipStream_t s, uint32_t* queue) { queue_reset_kernel<<<1, 64, 0, s>>>(queue); }
+
 bool walk_list_tile_ok(int precision, int list_tile) {
   if (list_tile == 0) return true;
   return precision == kWalkFP64 ? (list_tile == 32 || list_tile == 64)
                                 : (list_tile == 64 || list_tile == 128);
 }
 
-template <bool PER>
+template <bool PER, bool SINK>
 static void launch_walk_t(hipStream_t s, const WalkArgs& a, int precision, int blocks,
                           int list_tile) {
   if (precision == kWalkFP64) {
     if (list_tile == 32)
-      walk_kernel<double, 32, PER><<<blocks, kWave, 0, s>>>(a);
+      walk_kernel<double, 32, PER, SINK><<<blocks, kWave, 0, s>>>(a);
     else
-      walk_kernel<double, 64, PER><<<blocks, kWave, 0, s>>>(a);
+      walk_kernel<double, 64, PER, SINK><<<blocks, kWave, 0, s>>>(a);
   } else {
     if (list_tile == 64)
-      walk_kernel<float, 64, PER><<<blocks, kWave, 0, s>>>(a);
+      walk_kernel<float, 64, PER, SINK><<<blocks, kWave, 0, s>>>(a);
     else
-      walk_kernel<float, 128, PER><<<blocks, kWave, 0, s>>>(a);
+      walk_kernel<float, 128, PER, SINK><<<blocks, kWave, 0, s>>>(a);
   }
 }
 
 void launch_walk(hipStream_t s, const WalkArgs& a, int precision, int blocks, int list_tile) {
+  const bool sink = a.tx != nullptr;
   if (a.nbox > 1)
-    launch_walk_t<true>(s, a, precision, blocks, list_tile);
+    sink ? launch_walk_t<true, true>(s, a, precision, blocks, list_tile)
+         : launch_walk_t<true, false>(s, a, precision, blocks, list_tile);
   else
-    launch_walk_t<false>(s, a, precision, blocks, list_tile);
+    sink ? launch_walk_t<false, true>(s, a, precision, blocks, list_tile)
+         : launch_walk_t<false, false>(s, a, precision, blocks, list_tile);
 }
 
 void launch_walk_stats(hipStream_t s, const WalkArgs& a) {

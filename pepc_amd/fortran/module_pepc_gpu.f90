@@ -148,6 +148,16 @@ module pepc_gpu_c
          type(c_ptr), value :: ctx
          type(pepc_gpu_periodic), intent(in) :: p
       end function
+
+      integer(c_int) function pepc_gpu_traverse_sinks(ctx, m, x, y, z, mode, ex, ey, ez, pot, work) &
+         bind(C, name="pepc_gpu_traverse_sinks")
+         import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: m
+         real(c_double), intent(in) :: x(*), y(*), z(*)
+         integer(c_int32_t), value :: mode
+         real(c_double), intent(out) :: ex(*), ey(*), ez(*), pot(*), work(*)
+      end function
    end interface
 end module pepc_gpu_c
 
@@ -165,6 +175,7 @@ module module_pepc
    public :: pepc_read_parameters_from_first_argument
    public :: pepc_particleresults_clear, pepc_grow_tree, pepc_traverse_tree
    public :: pepc_timber_tree, pepc_restore_particles, pepc_grow_and_traverse
+   public :: pepc_grow_and_traverse_for_others
    public :: pepc_statistics, pepc_last_stats, pepc_last_timings
 
    ! &calc_force_coulomb (coulomb/module_interaction_specific.f90:35-43)
@@ -197,6 +208,7 @@ module module_pepc
    namelist /walk_para_gpu/ group_size, group_pack, list_tile, precision, interaction_sign
 
    type(c_ptr) :: ctx = c_null_ptr
+   type(c_ptr) :: grown_addr = c_null_ptr      ! first element of the grown array
    integer(c_int64_t), allocatable :: last_perm(:)
    integer :: my_rank_ = 0
 
@@ -326,7 +338,7 @@ contains
 
    !> module_pepc.f90:435-446: builds the tree and reorders particles by key
    subroutine pepc_grow_tree(particles)
-      type(t_particle), allocatable, intent(inout) :: particles(:)
+      type(t_particle), allocatable, target, intent(inout) :: particles(:)
       real(c_double), allocatable :: x(:), y(:), z(:), q(:), xs(:), ys(:), zs(:)
       integer(c_int64_t), allocatable :: keys(:)
       type(t_particle), allocatable :: tmp(:)
@@ -352,29 +364,49 @@ contains
          particles(i)%work = 1.0_8                ! module_domains.f90:318
          particles(i)%node_leaf = 0
       end do
+      grown_addr = c_null_ptr
+      if (n > 0) grown_addr = c_loc(particles(1))
       deallocate (tmp, x, y, z, q, xs, ys, zs, keys)
    end subroutine pepc_grow_tree
 
-   !> module_pepc.f90:459-468 for the grown particles (in their grown order)
+   !> module_pepc.f90:459-468.  The grown particles (in their grown order) take
+   !> the grown groups; any other array is walked particle by particle, as the
+   !> reference walks particles without keys (module_tree_walk_simple.f90:155-159),
+   !> and its interaction counts are added to %work.
    subroutine pepc_traverse_tree(particles)
       type(t_particle), target, intent(inout) :: particles(:)
-      real(c_double), allocatable :: ex(:), ey(:), ez(:), pot(:), work(:)
+      real(c_double), allocatable :: x(:), y(:), z(:), ex(:), ey(:), ez(:), pot(:), work(:)
       integer(c_int64_t) :: n, i
+      logical :: grown
       n = size(particles, kind=c_int64_t)
       if (.not. allocated(last_perm)) then
          write (*, *) 'PEPC-GPU: pepc_traverse_tree before pepc_grow_tree'
          error stop 1
       end if
-      if (n /= size(last_perm, kind=c_int64_t)) then
-         write (*, *) 'PEPC-GPU: traverse supports the grown particle set only'
-         error stop 1
-      end if
+      if (n == 0) return
+      grown = n == size(last_perm, kind=c_int64_t) .and. c_associated(grown_addr, c_loc(particles(1)))
       allocate (ex(n), ey(n), ez(n), pot(n), work(n))
-      call check(pepc_gpu_traverse(ctx, ex, ey, ez, pot, work), 'pepc_traverse_tree')
+      if (grown) then
+         call check(pepc_gpu_traverse(ctx, ex, ey, ez, pot, work), 'pepc_traverse_tree')
+      else
+         allocate (x(n), y(n), z(n))
+         do i = 1, n
+            x(i) = particles(i)%x(1)
+            y(i) = particles(i)%x(2)
+            z(i) = particles(i)%x(3)
+         end do
+         call check(pepc_gpu_traverse_sinks(ctx, n, x, y, z, 0_c_int32_t, ex, ey, ez, pot, work), &
+                    'pepc_traverse_tree (other particles)')
+         deallocate (x, y, z)
+      end if
       do i = 1, n
          particles(i)%results%e = particles(i)%results%e + [ex(i), ey(i), ez(i)]
          particles(i)%results%pot = particles(i)%results%pot + pot(i)
-         particles(i)%work = work(i)
+         if (grown) then
+            particles(i)%work = work(i)
+         else
+            particles(i)%work = particles(i)%work + work(i)
+         end if
       end do
       deallocate (ex, ey, ez, pot, work)
    end subroutine pepc_traverse_tree
@@ -414,6 +446,29 @@ contains
       if (restore) call pepc_restore_particles(particles)
       if (dealloc) call pepc_timber_tree()
    end subroutine pepc_grow_and_traverse
+
+   !> module_pepc.f90:394-433: forces of particles_source on particles_sink
+   subroutine pepc_grow_and_traverse_for_others(particles_source, particles_sink, itime, no_dealloc, &
+                                                no_restore)
+      type(t_particle), allocatable, target, intent(inout) :: particles_source(:)
+      type(t_particle), allocatable, optional, target, intent(inout) :: particles_sink(:)
+      integer, intent(in) :: itime
+      logical, optional, intent(in) :: no_dealloc, no_restore
+      logical :: restore, dealloc
+      restore = .true.
+      dealloc = .true.
+      if (present(no_dealloc)) dealloc = .not. no_dealloc
+      if (present(no_restore)) restore = .not. no_restore
+      call pepc_grow_tree(particles_source)
+      if (present(particles_sink)) then
+         call pepc_traverse_tree(particles_sink)
+      else
+         call pepc_traverse_tree(particles_source)
+      end if
+      if (print_stats) call pepc_statistics(itime)
+      if (restore) call pepc_restore_particles(particles_source)
+      if (dealloc) call pepc_timber_tree()
+   end subroutine pepc_grow_and_traverse_for_others
 
    function pepc_last_stats() result(st)
       type(pepc_gpu_stats) :: st

@@ -9,21 +9,27 @@
 !   pepc_mini_gpu params                      ! timestep run, &pepcmini in params
 !   pepc_mini_gpu params in.bin out.bin       ! one grow+traverse of the particles
 !                                             ! in in.bin, results to out.bin
+!   pepc_mini_gpu params in.bin out.bin sinks.bin sinks_out.bin
+!                                             ! then also traverse the particles of
+!                                             ! sinks.bin (another array) through
+!                                             ! the same tree
 ! in.bin : int64 n, then float64 x(n), y(n), z(n), q(n)
 ! out.bin: int64 n, int64 label(n) (0-based input index), float64 x,y,z(n),
 !          ex,ey,ez,pot,work(n), all in the grown (key) order.
+! sinks.bin: int64 m, float64 x(m), y(m), z(m); sinks_out.bin: int64 m, float64
+!          ex,ey,ez,pot,work(m) in the sinks' order (work starts at 1).
 program pepc_mini_gpu
    use module_pepc_kinds
    use module_pepc_types
    use module_pepc
    implicit none
 
-   type(t_particle), allocatable :: particles(:)
+   type(t_particle), allocatable :: particles(:), sinks(:)
    integer :: my_rank, n_ranks, step, nt, ip
    integer(kind_particle) :: tnp, i, n
    real(kind_physics) :: dt, epot, ekin
    real(kind_physics), allocatable :: v(:, :), a(:)
-   character(len=255) :: fin, fout
+   character(len=255) :: fin, fout, fsin, fsout
    integer(8) :: t0, t1, rate
    real(8) :: tg, tw
    namelist /pepcmini/ tnp, nt, dt
@@ -62,6 +68,30 @@ program pepc_mini_gpu
       write (12) particles(:)%results%pot
       write (12) particles(:)%work
       close (12)
+      if (command_argument_count() >= 5) then
+         call get_command_argument(4, fsin)
+         call get_command_argument(5, fsout)
+         open (11, file=trim(fsin), access='stream', form='unformatted', status='old')
+         read (11) n
+         deallocate (a)
+         allocate (sinks(n), a(n))
+         read (11) a; sinks(:)%x(1) = a
+         read (11) a; sinks(:)%x(2) = a
+         read (11) a; sinks(:)%x(3) = a
+         close (11)
+         sinks(:)%data%q = 0.0_8
+         sinks(:)%work = 1.0_8
+         call pepc_particleresults_clear(sinks)
+         call pepc_traverse_tree(sinks)          ! another array: walked sink by sink
+         open (12, file=trim(fsout), access='stream', form='unformatted', status='replace')
+         write (12) n
+         write (12) sinks(:)%results%e(1)
+         write (12) sinks(:)%results%e(2)
+         write (12) sinks(:)%results%e(3)
+         write (12) sinks(:)%results%pot
+         write (12) sinks(:)%work
+         close (12)
+      end if
       call pepc_statistics(0)
       call pepc_timber_tree()
       call pepc_finalize()

@@ -127,6 +127,34 @@ class Solver:
             self.sync()
         return out
 
+    def traverse_sinks(self, x, y, z, grouped=False, sync=True):
+        """Fields, potentials and interaction counts (work) at other points
+        (any order; results in the same order) from the grown tree, as the
+        reference's traversal of another particle array."""
+        t = self.torch
+        xs = [self._dev(v) for v in (x, y, z)]
+        self._sinks = xs
+        m = xs[0].numel()
+        out = TraverseResult(t.empty((3, m), dtype=t.float64, device=self.device),
+                             t.empty(m, dtype=t.float64, device=self.device),
+                             t.empty(m, dtype=t.float64, device=self.device))
+        self._bind_stream()
+        mode = _lib.SINKS_GROUPED if grouped else _lib.SINKS_PER_PARTICLE
+        self._check(self.lib.pepc_gpu_traverse_sinks_device(
+            self.h, m, *[_ptr(v) for v in xs], mode, _ptr(out.e[0]), _ptr(out.e[1]), _ptr(out.e[2]),
+            _ptr(out.pot), _ptr(out.work)))
+        if sync:
+            self.sync()
+        return out
+
+    def sink_counts(self, m):
+        """(p2p, m2p) per sink of the last traverse_sinks (sinks' order)."""
+        p2p = np.empty(m, np.uint32)
+        m2p = np.empty(m, np.uint32)
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)
+        self._check(self.lib.pepc_gpu_read_sink_counts(self.h, m, vp(p2p), vp(m2p)))
+        return p2p, m2p
+
     def sync(self) -> dict:
         st = _lib.Stats()
         self._check(self.lib.pepc_gpu_sync(self.h, C.byref(st)))

@@ -50,6 +50,45 @@ def test_fortran_frontend_matches_oracle(orc, tmp_path):
     assert np.array_equal(f[7], w.work)
 
 
+def test_fortran_traverse_other_particles(tmp_path):
+    """pepc_traverse_tree on an array other than the grown one walks it sink
+    by sink (the reference's rule for particles without keys): fields and
+    work equal the oracle's per-sink walk."""
+    from oracle import oracle as orc
+    from oracle import periodic as P
+    if not os.path.exists(EXE):
+        pytest.fail("pepc_mini_gpu not built (python -m pepc_amd.build)")
+    s = workloads.mini(n=8192, seed=42)
+    r0 = np.random.default_rng(43)
+    sk = r0.uniform(-0.1, 1.1, (2000, 3))
+    sk[:50] = np.stack([s.x[:50], s.y[:50], s.z[:50]], 1)       # on sources: self skipped
+    fin, fout, par = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "params"
+    sin, sout = tmp_path / "sinks.bin", tmp_path / "sinks_out.bin"
+    with open(fin, "wb") as f:
+        np.array([s.n], np.int64).tofile(f)
+        for a in (s.x, s.y, s.z, s.q):
+            a.astype(np.float64).tofile(f)
+    with open(sin, "wb") as f:
+        np.array([len(sk)], np.int64).tofile(f)
+        for d in range(3):
+            sk[:, d].astype(np.float64).tofile(f)
+    _params(par, theta2=s.theta2, eps2=s.eps2, G=16, prec="fp64")
+    r = subprocess.run([EXE, str(par), str(fin), str(fout), str(sin), str(sout)],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    raw = open(sout, "rb").read()
+    m = int(np.frombuffer(raw[:8], np.int64)[0])
+    f = np.frombuffer(raw[8:], np.float64).reshape(5, m)
+    d = orc.domain_decompose(s.x, s.y, s.z, s.q)
+    t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
+    w = P.walk_sinks(sk[:, 0], sk[:, 1], sk[:, 2], t, s.n, s.theta2, s.eps2)
+    e = f[0:3].T
+    assert np.max(np.abs(e - w.e)) <= 1e-12 * np.abs(w.e).max()
+    assert np.max(np.abs(f[3] - w.pot)) <= 1e-12 * np.abs(w.pot).max()
+    assert np.array_equal(f[4], w.work)
+
+
 def test_fortran_timestep_loop_runs(tmp_path):
     par = tmp_path / "params"
     _params(par, theta2=0.36, eps2=1e-4, G=64, prec="fp32", tnp=20000, nt=3)
