@@ -142,6 +142,18 @@ def force_error(sol, out, sys_, nsamp):
     return float(np.median(rel)), float(np.max(rel)), float(np.median(prel))
 
 
+def host_cpu():
+    """Model name of this host's CPU (from /proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(sys_, group, stride, nthreads, pack=0):
     from oracle import oracle as orc
     t0 = time.perf_counter()
@@ -161,7 +173,8 @@ def cpu_baseline(sys_, group, stride, nthreads, pack=0):
                        f"(box, keys, reference MSD presort, tree, upward pass) timed whole on 1 "
                        f"thread ({t_grow:.2f} s); walk over every {stride}-th of {len(tiles[0])} "
                        f"G={group} tiles on {nthreads} threads ({t3 - t2:.2f} s), scaled by {stride}"),
-            "t_grow_s": t_grow, "t_walk_s": t_walk}
+            "t_grow_s": t_grow, "t_walk_s": t_walk, "host_cpu": host_cpu(),
+            "host_threads_visible": os.cpu_count()}
 
 
 def main():
