@@ -147,6 +147,13 @@ int pepc_gpu_traverse_sinks(void *ctx, int64_t m, const double *x, const double 
 /* Per-sink P2P and M2P counts of the last sink traversal (sinks' order). */
 int pepc_gpu_read_sink_counts(void *ctx, int64_t cap, uint32_t *p2p, uint32_t *m2p);
 
+/* Replaces tree_check (module_tree.f90:438-521) behind pepc_check_sanity
+ * (module_pepc.f90:504-520): checks the grown tree on the device (child
+ * ranges, levels, parent links, particle counts, one leaf per particle, one
+ * root per system) and sets *nbad to the number of violations (0 = sound).
+ * Synchronous; allocates a small result buffer. */
+int pepc_gpu_check_tree(void *ctx, int64_t *nbad);
+
 /* Waits for the context's work, reads its counters into *st (may be NULL)
  * and reports errors raised by the kernels. */
 int pepc_gpu_sync(void *ctx, pepc_gpu_stats *st);

@@ -30,6 +30,7 @@ EXPORTS = [
     "pepc_gpu_replay_device", "pepc_gpu_restore_device", "pepc_gpu_periodic_default",
     "pepc_gpu_set_periodic", "pepc_gpu_lattice_get", "pepc_gpu_constrain_periodic_device",
     "pepc_gpu_traverse_sinks_device", "pepc_gpu_traverse_sinks", "pepc_gpu_read_sink_counts",
+    "pepc_gpu_check_tree",
 ]
 
 SINKS_PER_PARTICLE = 0
@@ -135,6 +136,7 @@ def load(path: str = LIB_PATH):
     L.pepc_gpu_traverse_sinks_device.argtypes = [_vp, _i64, _vp, _vp, _vp, C.c_int32] + [_vp] * 5
     L.pepc_gpu_traverse_sinks.argtypes = [_vp, _i64, _vp, _vp, _vp, C.c_int32] + [_vp] * 5
     L.pepc_gpu_read_sink_counts.argtypes = [_vp, _i64, _vp, _vp]
+    L.pepc_gpu_check_tree.argtypes = [_vp, C.POINTER(_i64)]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is not None and name not in ("pepc_gpu_config_default",

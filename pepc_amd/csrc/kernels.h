@@ -101,6 +101,9 @@ void launch_dupfix(hipStream_t s, int64_t n, unsigned long long* keys, double* x
                    uint32_t* bsum, DevScalars* sc);
 void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size, int group_pack);
 void launch_leaf_moments(hipStream_t s, const TreeBuffers& t);
+// Tree consistency counters res[7] (zeroed by the caller), see tree.hip.
+void launch_tree_check(hipStream_t s, const NodeLink* link, const uint32_t* parent, int64_t n,
+                       const DevScalars* sc, unsigned long long* res);
 // Upward pass alone (all levels) over an existing topology.
 void launch_upward(hipStream_t s, const TreeBuffers& t);
 

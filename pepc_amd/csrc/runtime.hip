@@ -716,6 +716,40 @@ int pepc_gpu_read_sink_counts(void* p, int64_t cap, uint32_t* p2p, uint32_t* m2p
   return 0;
 }
 
+int pepc_gpu_check_tree(void* p, int64_t* nbad) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PEPC_GPU_EINVAL;
+  if (!c->grown) return fail(c, PEPC_GPU_EINVAL, "check_tree: no tree (call grow first)");
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  int r = sync_impl(c, nullptr);
+  if (r) return r;
+  unsigned long long* res = nullptr;
+  HIPCHECK(c, hipMalloc((void**)&res, 8 * sizeof(unsigned long long)));
+  unsigned long long h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  hipError_t e = hipMemcpyAsync(res, h, sizeof(h), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    launch_tree_check(c->stream, c->link, c->nparent, c->n, c->sc, res);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(res);
+  if (e != hipSuccess) return hip_fail(c, e, "check_tree");
+  const unsigned long long n = (unsigned long long)c->n;
+  const DevScalars& st = *c->sc_host;
+  int64_t bad = (int64_t)h[3];
+  if (h[0] != n) ++bad;                                  // leaves = particles
+  if (h[1] != st.ntwig) ++bad;                           // twigs = recorded twigs
+  if (h[2] != (unsigned long long)c->nsys && !(c->nsys > 1 && h[2] < (unsigned long long)c->nsys))
+    ++bad;                                               // one root per non-empty system
+  if (h[4] != n) ++bad;                                  // roots cover every particle
+  if (h[5] != n * (n - 1) / 2) ++bad;                    // leaves hold 0..n-1 once
+  const unsigned __int128 n128 = n;
+  if (h[6] != (unsigned long long)((n128 - 1) * n128 * (2 * n128 - 1) / 6)) ++bad;   // mod 2^64
+  if (nbad) *nbad = bad;
+  return 0;
+}
+
 int pepc_gpu_traverse(void* p, double* ex, double* ey, double* ez, double* pot, double* work) {
   Ctx* c = as_ctx(p);
   if (!c) return PEPC_GPU_EINVAL;

@@ -477,6 +477,61 @@ __global__ __launch_bounds__(kT) void upward_level_kernel(int level, UpArgs u) {
   }
 }
 
+// Consistency of the grown tree (tree_check, module_tree.f90:438-521, counts
+// the leaves and twigs reachable from the root against the recorded totals).
+// Every node is checked locally: a leaf holds one particle; a twig's child
+// range lies inside the node array and holds 1..8 nodes one level deeper
+// whose parent is the twig and whose particle counts add up to the twig's;
+// a node with a parent lies in that parent's child range.  res[0..6] =
+// {leaves, twigs, roots, violations, particles below roots, sum and sum of
+// squares of the leaves' particle indices}; the host compares them with n,
+// the twig count, the system count and the sums of 0..n-1.
+__global__ __launch_bounds__(kT) void tree_check_kernel(const NodeLink* __restrict__ link,
+                                                       const uint32_t* __restrict__ parent,
+                                                       int64_t n, const DevScalars* __restrict__ sc,
+                                                       unsigned long long* __restrict__ res) {
+  const int64_t m = sc->nnodes;
+  unsigned long long v[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const NodeLink lk = link[i];
+    const uint32_t p = parent[i];
+    if (p == 0xFFFFFFFFu) {
+      v[2] += 1;
+      v[4] += lk.count;
+    } else if ((int64_t)p >= m) {
+      v[3] += 1;
+    } else {
+      const NodeLink pl = link[p];
+      if (!(pl.nchild > 0 && (uint32_t)i >= pl.first && (uint32_t)i - pl.first < pl.nchild)) v[3] += 1;
+    }
+    if (lk.nchild == 0) {
+      v[0] += 1;
+      if ((int64_t)lk.first >= n || lk.count != 1u) v[3] += 1;
+      v[5] += lk.first;
+      v[6] += (unsigned long long)lk.first * lk.first;
+    } else {
+      v[1] += 1;
+      if (lk.nchild > 8u || (int64_t)lk.first + lk.nchild > m) {
+        v[3] += 1;
+      } else {
+        uint32_t sum = 0;
+        for (uint32_t j = 0; j < lk.nchild; ++j) {
+          const NodeLink c = link[lk.first + j];
+          sum += c.count;
+          if (c.level != lk.level + 1 || parent[lk.first + j] != (uint32_t)i) v[3] += 1;
+        }
+        if (sum != lk.count) v[3] += 1;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    const unsigned long long t = wave_sum(v[k]);
+    if (lane_id() == 0 && t) atomicAdd(&res[k], t);
+  }
+}
+
 // Leaf records in the reference layout, for read-back only (not part of a
 // solve): coc = x, charge = q, abs_charge = |q|, the rest 0.
 __global__ void leaf_mom_kernel(const NodeLink* __restrict__ link, const double* __restrict__ x,
@@ -633,6 +688,11 @@ void launch_upward(hipStream_t s, const TreeBuffers& t) {
   u.sc = t.sc;
   for (int l = kMaxLevel; l >= 0; --l)
     upward_level_kernel<<<grid_for(t.cap / 4), kT, 0, s>>>(l, u);
+}
+
+void launch_tree_check(hipStream_t s, const NodeLink* link, const uint32_t* parent, int64_t n,
+                       const DevScalars* sc, unsigned long long* res) {
+  tree_check_kernel<<<256, kT, 0, s>>>(link, parent, n, sc, res);
 }
 
 void launch_leaf_moments(hipStream_t s, const TreeBuffers& t) {

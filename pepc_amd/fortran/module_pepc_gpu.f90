@@ -149,6 +149,12 @@ module pepc_gpu_c
          type(pepc_gpu_periodic), intent(in) :: p
       end function
 
+      integer(c_int) function pepc_gpu_check_tree(ctx, nbad) bind(C, name="pepc_gpu_check_tree")
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), intent(out) :: nbad
+      end function
+
       integer(c_int) function pepc_gpu_traverse_sinks(ctx, m, x, y, z, mode, ex, ey, ez, pot, work) &
          bind(C, name="pepc_gpu_traverse_sinks")
          import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double
@@ -175,7 +181,7 @@ module module_pepc
    public :: pepc_read_parameters_from_first_argument
    public :: pepc_particleresults_clear, pepc_grow_tree, pepc_traverse_tree
    public :: pepc_timber_tree, pepc_restore_particles, pepc_grow_and_traverse
-   public :: pepc_grow_and_traverse_for_others
+   public :: pepc_grow_and_traverse_for_others, pepc_check_sanity
    public :: pepc_statistics, pepc_last_stats, pepc_last_timings
 
    ! &calc_force_coulomb (coulomb/module_interaction_specific.f90:35-43)
@@ -469,6 +475,30 @@ contains
       if (restore) call pepc_restore_particles(particles_source)
       if (dealloc) call pepc_timber_tree()
    end subroutine pepc_grow_and_traverse_for_others
+
+   !> module_pepc.f90:504-520: aborts if the grown tree is inconsistent;
+   !> dump prints the tree's counters (the reference dumps its hash table)
+   subroutine pepc_check_sanity(caller, dump, particles)
+      character(*), intent(in) :: caller
+      logical, optional, intent(in) :: dump
+      type(t_particle), optional, intent(in) :: particles(:)
+      integer(c_int64_t) :: nbad
+      type(pepc_gpu_stats) :: st
+      logical :: do_dump
+      do_dump = .false.
+      if (present(dump)) do_dump = dump
+      call check(pepc_gpu_check_tree(ctx, nbad), 'pepc_check_sanity')
+      if (nbad /= 0 .or. do_dump) then
+         st = pepc_last_stats()
+         write (*, '(a,a,a,i0,a,3i14)') ' tree check (', trim(caller), '): ', nbad, &
+            ' violations; nodes/leaves/twigs ', st%nnodes, st%nleaf, st%ntwig
+         if (present(particles)) write (*, '(a,i0)') ' particles: ', size(particles)
+      end if
+      if (nbad /= 0) then
+         write (*, *) 'PEPC-GPU: Sanity check failed, aborting!'
+         error stop 1
+      end if
+   end subroutine pepc_check_sanity
 
    function pepc_last_stats() result(st)
       type(pepc_gpu_stats) :: st

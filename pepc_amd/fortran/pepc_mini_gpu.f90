@@ -55,6 +55,7 @@ program pepc_mini_gpu
       end do
       call pepc_particleresults_clear(particles)
       call pepc_grow_tree(particles)
+      call pepc_check_sanity('pepc_mini_gpu', .false.)
       call pepc_traverse_tree(particles)
       open (12, file=trim(fout), access='stream', form='unformatted', status='replace')
       write (12) n

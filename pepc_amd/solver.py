@@ -155,6 +155,12 @@ class Solver:
         self._check(self.lib.pepc_gpu_read_sink_counts(self.h, m, vp(p2p), vp(m2p)))
         return p2p, m2p
 
+    def check_tree(self) -> int:
+        """Number of consistency violations of the grown tree (0 = sound)."""
+        nb = C.c_int64()
+        self._check(self.lib.pepc_gpu_check_tree(self.h, C.byref(nb)))
+        return nb.value
+
     def sync(self) -> dict:
         st = _lib.Stats()
         self._check(self.lib.pepc_gpu_sync(self.h, C.byref(st)))

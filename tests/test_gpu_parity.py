@@ -171,3 +171,18 @@ def test_error_paths(gpu):
         sol.grow(s.x, s.y, s.z, s.q)
     assert ei.value.code == _lib.EOVERFLOW
     sol.close()
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_tree_check_sound(gpu, name):
+    """pepc_gpu_check_tree (tree_check behind pepc_check_sanity) finds the
+    grown trees sound, also for a batch with an empty system."""
+    s = SYSTEMS[name]()
+    sol = gpu(theta2=s.theta2, eps2=s.eps2, precision="fp32", sign=s.sign)
+    sol.grow(s.x, s.y, s.z, s.q)
+    assert sol.check_tree() == 0
+    k = s.n // 3
+    off = np.array([0, k, k, s.n], np.int64)
+    sol.grow_batch(s.x, s.y, s.z, s.q, off)
+    assert sol.check_tree() == 0
+    sol.close()
