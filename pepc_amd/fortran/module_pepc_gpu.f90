@@ -177,7 +177,7 @@ module module_pepc
    private
 
    public :: pepc_initialize, pepc_finalize, pepc_prepare
-   public :: pepc_read_parameters, pepc_read_parameters_from_file_name
+   public :: pepc_read_parameters, pepc_read_parameters_from_file_name, pepc_write_parameters
    public :: pepc_read_parameters_from_first_argument
    public :: pepc_particleresults_clear, pepc_grow_tree, pepc_traverse_tree
    public :: pepc_timber_tree, pepc_restore_particles, pepc_grow_and_traverse
@@ -261,6 +261,14 @@ contains
       rewind (filehandle)
       read (filehandle, NML=walk_para_gpu, iostat=ios)
    end subroutine pepc_read_parameters
+
+   !> module_pepc.f90:250-262: the namelists this library reads
+   subroutine pepc_write_parameters(filehandle)
+      integer, intent(in) :: filehandle
+      write (filehandle, NML=calc_force_coulomb)
+      write (filehandle, NML=walk_para_gpu)
+      write (filehandle, NML=libpepc)
+   end subroutine pepc_write_parameters
 
    subroutine pepc_read_parameters_from_file_name(filename)
       character(*), intent(in) :: filename
@@ -524,3 +532,183 @@ contains
    end subroutine pepc_statistics
 
 end module module_pepc
+
+!> Particle checkpoints in the reference's file formats
+!> (src/utils/module_checkpoint.f90), without MPI: one process writes and
+!> reads the whole particle array.  Every t_particle component of the Coulomb
+!> XYZQ type is 8 bytes (module_pepc_kinds.f90:63-70), so a record is the 96
+!> bytes {x(3), work, key, node_leaf, label, q, e(3), pot} that
+!> MPI_TYPE_particle_vec describes (module_pepc_types.f90:107-147).
+!>   MPI-IO file  <dir>/mpi/particle_<itime:i12.12>.mpi: a 128-byte header
+!>                {int64 n_total, int32 itime, zeros}, then n_total records
+!>   binary file  <dir>/binary/particle_<itime:i12.12>_<rank:i6.6>.bin: records
+!>   ASCII file   <dir>/ascii/particle_<itime:i12.12>_<rank:i6.6>.dat: one
+!>                list-directed line per particle
+!> each with a parameter file <name>.h (pepc_write_parameters).
+module module_checkpoint
+   use iso_c_binding
+   use module_pepc_kinds
+   use module_pepc_types
+   implicit none
+   private
+
+   integer, parameter :: filehandle = 91
+   character(12), parameter :: directory = './particles'
+   integer(c_int64_t), parameter :: max_header_size = 128
+   integer, parameter :: kind_pe = kind_default
+
+   public write_particles_ascii, write_particles_binary, write_particles_mpiio
+   public read_particles_mpiio, read_particles_mpiio_from_filename
+
+   interface
+      integer(c_int) function c_mkdir(path, mode) bind(C, name="mkdir")
+         import :: c_int, c_char
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int), value :: mode
+      end function
+   end interface
+
+contains
+
+   subroutine create_directory(dir)      ! module_utils.f90 create_directory
+      character(*), intent(in) :: dir
+      integer(c_int) :: rc
+      rc = c_mkdir(trim(dir)//c_null_char, int(o'755', c_int))   ! an existing directory is fine
+   end subroutine create_directory
+
+   subroutine write_params(filename)
+      use module_pepc, only: pepc_write_parameters
+      character(*), intent(in) :: filename
+      open (filehandle, file=trim(filename)//'.h', status='replace', action='write')
+      call pepc_write_parameters(filehandle)
+      close (filehandle)
+   end subroutine write_params
+
+   !> module_checkpoint.f90:43-82
+   subroutine write_particles_ascii(my_rank, itime, dp, filename_out)
+      integer(kind_pe), intent(in) :: my_rank
+      integer(kind_default), intent(in) :: itime
+      type(t_particle), intent(in), dimension(:) :: dp
+      character(*), optional, intent(out) :: filename_out
+      character(100) :: filename
+      integer(kind_particle) :: i
+      call create_directory(trim(directory))
+      call create_directory(trim(directory)//'/ascii')
+      write (filename, '(a,"/ascii/particle_",i12.12,"_",i6.6,".dat")') trim(directory), itime, my_rank
+      open (filehandle, file=trim(filename), status='replace', action='write')
+      do i = 1, size(dp, kind=kind(i))
+         write (filehandle, *) dp(i)
+      end do
+      close (filehandle)
+      if (my_rank == 0) call write_params(filename)
+      if (present(filename_out)) filename_out = trim(filename)//'.h'
+   end subroutine write_particles_ascii
+
+   !> module_checkpoint.f90:84-121
+   subroutine write_particles_binary(my_rank, itime, dp, filename)
+      integer(kind_pe), intent(in) :: my_rank
+      integer(kind_default), intent(in) :: itime
+      type(t_particle), intent(in), dimension(:) :: dp
+      character(*), intent(out) :: filename
+      call create_directory(trim(directory))
+      call create_directory(trim(directory)//'/binary')
+      write (filename, '(a,"/binary/particle_",i12.12,"_",i6.6,".bin")') trim(directory), itime, my_rank
+      open (filehandle, file=trim(filename), status='replace', access='stream', form='unformatted')
+      write (filehandle) dp(1:size(dp))
+      close (filehandle)
+      if (my_rank == 0) call write_params(filename)
+      filename = trim(filename)//'.h'
+   end subroutine write_particles_binary
+
+   !> module_checkpoint.f90:122-193 on one process
+   subroutine write_particles_mpiio(comm, itime, n_total, dp, filename)
+      integer, intent(in) :: comm
+      integer(kind_default), intent(in) :: itime
+      integer*8, intent(in) :: n_total
+      type(t_particle), intent(in) :: dp(:)
+      character(*), intent(out) :: filename
+      integer(c_int8_t) :: pad(max_header_size - 12)
+      if (size(dp, kind=8) /= n_total) then
+         write (*, '(a,i0,a,i0)') 'PEPC-GPU: Invalid total particle number: size(dp) = ', size(dp, kind=8), &
+            ' but n_total = ', n_total
+         error stop 1
+      end if
+      call create_directory(trim(directory))
+      call create_directory(trim(directory)//'/mpi')
+      write (filename, '(a,"/mpi/particle_",i12.12,".mpi")') trim(directory), itime
+      pad = 0_c_int8_t
+      open (filehandle, file=trim(filename), status='replace', access='stream', form='unformatted')
+      write (filehandle) n_total, int(itime, c_int32_t), pad
+      write (filehandle, pos=max_header_size + 1) dp(1:size(dp))
+      close (filehandle)
+      call write_params(filename)
+      filename = trim(filename)//'.h'
+      if (comm == comm) continue
+   end subroutine write_particles_mpiio
+
+   !> module_checkpoint.f90:195-214
+   subroutine read_particles_mpiio(itime_in, comm, itime, n_total, dp, filename, nparticles_local, &
+                                   file_exists, noparams)
+      integer(kind_default) :: itime_in
+      integer, intent(in) :: comm
+      integer(kind_default), intent(out) :: itime
+      integer(kind_particle), intent(out) :: n_total
+      type(t_particle), allocatable :: dp(:)
+      character(*), intent(out) :: filename
+      integer(kind_default), optional, intent(in) :: nparticles_local
+      logical, optional, intent(out) :: file_exists
+      logical, optional, intent(in) :: noparams
+      write (filename, '(a,"/mpi/particle_",i12.12,".mpi")') trim(directory), itime_in
+      call read_particles_mpiio_from_filename(comm, itime, n_total, dp, filename, nparticles_local, &
+                                              file_exists, noparams)
+      filename = trim(filename)//'.h'
+   end subroutine read_particles_mpiio
+
+   !> module_checkpoint.f90:216-300 on one process (all records to this one)
+   subroutine read_particles_mpiio_from_filename(comm, itime, n_total, dp, filename, nparticles_local, &
+                                                 file_exists, noparams)
+      use module_pepc, only: pepc_read_parameters
+      integer, intent(in) :: comm
+      integer(kind_default), intent(out) :: itime
+      integer(kind_particle), intent(out) :: n_total
+      type(t_particle), allocatable :: dp(:)
+      character(*), intent(in) :: filename
+      integer(kind_default), optional, intent(in) :: nparticles_local
+      logical, optional, intent(out) :: file_exists
+      logical, optional, intent(in) :: noparams
+      logical :: there, noparams_
+      integer(c_int32_t) :: it32
+      integer(kind_particle) :: np_local
+      noparams_ = .false.
+      if (present(noparams)) noparams_ = noparams
+      inquire (file=trim(filename), exist=there)
+      if (present(file_exists)) file_exists = there
+      if (.not. there) then
+         if (present(file_exists)) return
+         write (*, '(3a)') 'PEPC-GPU: read_particles_mpiio_from_filename(filename=', trim(filename), &
+            ') - file does not exist'
+         error stop 1
+      end if
+      open (filehandle, file=trim(filename), status='old', access='stream', form='unformatted', action='read')
+      read (filehandle) n_total, it32
+      itime = it32
+      np_local = n_total
+      if (present(nparticles_local)) then
+         if (int(nparticles_local, kind_particle) /= n_total) then
+            write (*, '(a,i0,a,i0)') 'PEPC-GPU: Invalid total particle number: nparticles_local = ', &
+               nparticles_local, ' but the data file says n_total = ', n_total
+            error stop 1
+         end if
+      end if
+      if (allocated(dp)) deallocate (dp)
+      allocate (dp(1:np_local))
+      if (np_local > 0) read (filehandle, pos=max_header_size + 1) dp
+      close (filehandle)
+      if (.not. noparams_) then
+         open (filehandle, file=trim(filename)//'.h', action='read', status='old')
+         call pepc_read_parameters(filehandle)
+         close (filehandle)
+      end if
+      if (comm == comm) continue
+   end subroutine read_particles_mpiio_from_filename
+end module module_checkpoint

@@ -13,6 +13,10 @@
 !                                             ! then also traverse the particles of
 !                                             ! sinks.bin (another array) through
 !                                             ! the same tree
+!   pepc_mini_gpu --checkpoint in.bin out.bin ! write the particles of in.bin as
+!                                             ! MPI-IO and binary checkpoints of
+!                                             ! step 7 under ./particles, read the
+!                                             ! MPI-IO one back into out.bin (no GPU)
 ! in.bin : int64 n, then float64 x(n), y(n), z(n), q(n)
 ! out.bin: int64 n, int64 label(n) (0-based input index), float64 x,y,z(n),
 !          ex,ey,ez,pot,work(n), all in the grown (key) order.
@@ -22,6 +26,7 @@ program pepc_mini_gpu
    use module_pepc_kinds
    use module_pepc_types
    use module_pepc
+   use module_checkpoint
    implicit none
 
    type(t_particle), allocatable :: particles(:), sinks(:)
@@ -33,6 +38,14 @@ program pepc_mini_gpu
    integer(8) :: t0, t1, rate
    real(8) :: tg, tw
    namelist /pepcmini/ tnp, nt, dt
+
+   if (command_argument_count() >= 3) then
+      call get_command_argument(1, fin)
+      if (trim(fin) == '--checkpoint') then
+         call checkpoint_roundtrip()
+         stop
+      end if
+   end if
 
    call pepc_initialize("pepc-mini-gpu", my_rank, n_ranks, .true.)
    call pepc_read_parameters_from_first_argument()
@@ -132,6 +145,38 @@ program pepc_mini_gpu
    call pepc_finalize()
 
 contains
+
+   subroutine checkpoint_roundtrip()
+      type(t_particle), allocatable :: back(:)
+      character(len=255) :: f1, f2, f3
+      integer(kind_default) :: itime
+      integer(kind_particle) :: ntot
+      call get_command_argument(2, fin)
+      call get_command_argument(3, fout)
+      open (11, file=trim(fin), access='stream', form='unformatted', status='old')
+      read (11) n
+      allocate (particles(n), a(n))
+      read (11) a; particles(:)%x(1) = a
+      read (11) a; particles(:)%x(2) = a
+      read (11) a; particles(:)%x(3) = a
+      read (11) a; particles(:)%data%q = a
+      close (11)
+      do i = 1, n
+         particles(i)%work = 0.5_8*real(i, 8)
+         particles(i)%key = 1000_8 + i
+         particles(i)%node_leaf = i
+         particles(i)%label = i - 1
+         particles(i)%results%e = [real(i, 8), 2.0_8*real(i, 8), 3.0_8*real(i, 8)]
+         particles(i)%results%pot = -real(i, 8)
+      end do
+      call write_particles_mpiio(0, 7, int(n, 8), particles, f1)
+      call write_particles_binary(0, 7, particles, f2)
+      call read_particles_mpiio(7, 0, itime, ntot, back, f3, noparams=.true.)
+      open (12, file=trim(fout), access='stream', form='unformatted', status='replace')
+      write (12) ntot, int(itime, 8)
+      write (12) back
+      close (12)
+   end subroutine checkpoint_roundtrip
 
    subroutine init_particles(p)
       type(t_particle), intent(inout) :: p(:)
