@@ -229,6 +229,7 @@ __device__ __forceinline__ void moments_row(int64_t n, const double* __restrict_
                                             const double* __restrict__ q, const LatticeGeom& g,
                                             double2* __restrict__ part, double* __restrict__ dpart,
                                             double (*red)[kMomRed]) {
+#pragma clang fp contract(fast)   // tolerance-checked sums: fused multiply-adds allowed
   constexpr int K = kLM + 1 - M;
   double ar[K], ai[K], dp[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -464,6 +465,7 @@ __global__ __launch_bounds__(kFieldT) void lattice_field_kernel(int64_t n, const
                                                                double* __restrict__ ey,
                                                                double* __restrict__ ez,
                                                                double* __restrict__ pot) {
+#pragma clang fp contract(fast)   // tolerance-checked sums: fused multiply-adds allowed
   const int64_t i = blockIdx.x * (int64_t)kFieldT + threadIdx.x;
   if (i >= n) return;
   const double px = x[i], py = y[i], pz = z[i];
@@ -478,6 +480,7 @@ __global__ __launch_bounds__(kFieldT) void lattice_field_kernel(int64_t n, const
       wr = tr; wi = ti;
     }
     double cr = wr, ci = wi, p1r = 0.0, p1i = 0.0;
+#pragma unroll 4
     for (int l = m; l <= kLM; ++l) {
       if (l > m) {
         const double a = (double)(2 * l - 1) * rz, sc = inv[tix(l, m)];
@@ -488,10 +491,10 @@ __global__ __launch_bounds__(kFieldT) void lattice_field_kernel(int64_t n, const
         p1r = wr; p1i = wi; cr = 0.0; ci = 0.0;
       }
       const double* k = coef + 8 * tix(l, m);
-      phi += p1r * k[0] + p1i * k[1];
-      l10 += p1r * k[2] + p1i * k[3];
-      l11r += p1r * k[4] + p1i * k[5];
-      l11i += p1r * k[6] + p1i * k[7];
+      phi = fma(p1r, k[0], fma(p1i, k[1], phi));
+      l10 = fma(p1r, k[2], fma(p1i, k[3], l10));
+      l11r = fma(p1r, k[4], fma(p1i, k[5], l11r));
+      l11i = fma(p1r, k[6], fma(p1i, k[7], l11i));
     }
   }
   double e0 = -l11r, e1 = -l11i, e2 = -l10;

@@ -236,3 +236,48 @@ def test_periodic_timestep_energy_parity(gpu, P, table):
         if step < 5:
             sol.run(1, s.dt)
     sol.close()
+
+
+def test_periodic_graph_replay_equals_eager(gpu):
+    """The periodic step (wrap, mirror walks, lattice kernels) captured in a
+    HIP graph and replayed gives the eager steps' energies bit for bit."""
+    s = workloads.periodic(n=3000, seed=36)
+    runs = []
+    for graph in (False, True):
+        sol = gpu(theta2=s.theta2, eps2=s.eps2, precision="fp32", group_size=32)
+        sol.set_periodic()
+        sol.state_set(s.x, s.y, s.z, s.v, s.q, s.m)
+        for _ in range(3):
+            sol.run(1, s.dt, graph=graph)
+        runs.append((sol.energies(), sol.state_get()))
+        sol.close()
+    (e0, st0), (e1, st1) = runs
+    assert np.array_equal(e0, e1)
+    assert np.array_equal(st0["x"], st1["x"]) and np.array_equal(st0["v"], st1["v"])
+
+
+def test_periodic_errors(gpu):
+    from pepc_amd._lib import PepcGpuError
+    sol = gpu(precision="fp64")
+    with pytest.raises(PepcGpuError):
+        sol.set_periodic(periodicity=(1, 1, 0))                  # far field needs 3-D
+    with pytest.raises(PepcGpuError):
+        sol.set_periodic(layers=2)                               # far field: one layer
+    with pytest.raises(PepcGpuError):
+        sol.set_periodic(layers=4, far_field=False)
+    with pytest.raises(PepcGpuError):
+        sol.set_periodic(lattice=np.zeros((3, 3)))               # degenerate lattice
+    s = workloads.periodic(n=2000, seed=37)
+    sol.set_periodic()
+    sol.grow(s.x, s.y, s.z, s.q)
+    sol.traverse()
+    with pytest.raises(PepcGpuError):
+        sol.freeze()                                             # no frozen lists with periodic
+    import torch
+    g = torch.zeros((3, s.n), dtype=torch.float64, device="cuda")
+    with pytest.raises(PepcGpuError):
+        sol.adjoint(g, g[0])
+    sol.set_periodic(periodicity=None)                           # cleared: plain solve again
+    sol.grow(s.x, s.y, s.z, s.q)
+    sol.traverse()
+    sol.close()
