@@ -155,26 +155,36 @@ def host_cpu():
 
 
 def cpu_baseline(sys_, group, stride, nthreads, pack=0):
+    """The oracle (CPU restatement of the reference path, FP64) on this host:
+    grow timed whole on one thread (one reference rank), then the group walk
+    on every host thread at the reference's default tile rule (module_tree_walk_simple
+    TILE_SIZE = 16, no packing; the value) and at the GPU run's rule."""
     from oracle import oracle as orc
     t0 = time.perf_counter()
     d = orc.domain_decompose(sys_.x, sys_.y, sys_.z, sys_.q)
     t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
-    t1 = time.perf_counter()
-    tiles = orc.walk_tiles(d.keys, group, pack)
-    t2 = time.perf_counter()
-    w = orc.walk_group(d.x, d.y, d.z, t, sys_.theta2, sys_.eps2, tile_size=group, sign=sys_.sign,
-                       tile_stride=stride, nthreads=nthreads, tiles=tiles)
-    t3 = time.perf_counter()
-    t_grow = t1 - t0
-    t_walk = (t3 - t2) * stride
-    return {"value": sys_.n / (t_grow + t_walk), "unit": "particle updates/s", "cores": nthreads,
+    t_grow = time.perf_counter() - t0
+
+    def walk(G, P):
+        tiles = orc.walk_tiles(d.keys, G, P)
+        t2 = time.perf_counter()
+        w = orc.walk_group(d.x, d.y, d.z, t, sys_.theta2, sys_.eps2, tile_size=G, sign=sys_.sign,
+                           tile_stride=stride, nthreads=nthreads, tiles=tiles)
+        return (time.perf_counter() - t2) * stride, len(tiles[0]), w.num_interactions
+
+    t_ref, nt_ref, ni_ref = walk(16, 0)
+    t_same, nt_same, ni_same = walk(group, pack)
+    return {"value": sys_.n / (t_grow + t_ref), "unit": "particle updates/s", "cores": nthreads,
             "kind": "port",
             "sample": (f"oracle (CPU restatement, FP64) on the same {sys_.n}-particle system: grow "
                        f"(box, keys, reference MSD presort, tree, upward pass) timed whole on 1 "
-                       f"thread ({t_grow:.2f} s); walk over every {stride}-th of {len(tiles[0])} "
-                       f"G={group} tiles on {nthreads} threads ({t3 - t2:.2f} s), scaled by {stride}"),
-            "t_grow_s": t_grow, "t_walk_s": t_walk, "host_cpu": host_cpu(),
-            "host_threads_visible": os.cpu_count()}
+                       f"thread ({t_grow:.2f} s); walk at the reference's default tile rule "
+                       f"(G=16, {nt_ref} tiles, every {stride}-th, scaled) on {nthreads} threads "
+                       f"({t_ref:.2f} s)"),
+            "t_grow_s": t_grow, "t_walk_s": t_ref, "interactions": ni_ref * stride,
+            "same_rule_as_gpu": {"group_size": group, "group_pack": pack, "t_walk_s": t_same,
+                                 "value": sys_.n / (t_grow + t_same), "tiles": nt_same},
+            "host_cpu": host_cpu(), "host_threads_visible": os.cpu_count()}
 
 
 def main():
