@@ -300,3 +300,34 @@ def test_c5_adjoint_full(gpu):
     assert max(rels) < 0.1
     ds.close()
     sol.close()
+
+
+def test_periodic_plasma_full(gpu):
+    """Periodic plasma, N = 4,194,304 in the unit cube, 27 mirror boxes +
+    lattice far field: the tree and group invariants, the device tree check,
+    a neutral set of fictitious charges, and FP32 + Kahan within rounding of
+    the FP64 periodic tree."""
+    s = workloads.periodic(n=4194304, seed=71)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        sol = gpu(theta2=s.theta2, eps2=s.eps2, precision=prec, group_size=64, group_pack=64)
+        sol.set_periodic()
+        t0 = time.perf_counter()
+        st = sol.grow(s.x, s.y, s.z, s.q)
+        res = sol.traverse()
+        dt = time.perf_counter() - t0
+        st = sol.sync()
+        _invariants(sol, st, res, s.x, s.y, s.z)
+        assert sol.check_tree() == 0
+        out[prec] = res.e.cpu().numpy().T
+        print(f"[periodic 4M {prec}] {dt:.2f} s, {st['interactions'] / s.n:.0f} interactions/target",
+              flush=True)
+        if prec == "fp64":
+            ml, mu, fict = sol.lattice()
+            assert np.isfinite(mu).all() and len(fict) == 4
+            assert abs(fict[:, 3].sum()) <= 1e-12 * max(1.0, np.abs(fict[:, 3]).max())
+        sol.close()
+    rel = np.linalg.norm(out["fp32"] - out["fp64"], axis=1) / np.linalg.norm(out["fp64"], axis=1)
+    print(f"[periodic 4M] fp32 vs fp64 tree: median {np.median(rel):.2e} max {rel.max():.2e}",
+          flush=True)
+    assert np.median(rel) < 2e-6 and np.quantile(rel, 0.999) < 1e-4
