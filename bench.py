@@ -194,8 +194,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=4194304)
     ap.add_argument("--theta", type=float, default=0.6)
-    ap.add_argument("--group", type=int, default=64)
-    ap.add_argument("--pack", type=int, default=64,
+    ap.add_argument("--group", type=int, default=16)
+    ap.add_argument("--pack", type=int, default=32,
                     help="merge runs of sibling walk tiles up to this many targets (0 = off)")
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--err-samples", type=int, default=65536)

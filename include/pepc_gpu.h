@@ -65,7 +65,7 @@ typedef struct {
                                up to this many particles per group (no reference
                                counterpart; fuller wavefronts) */
     int32_t list_tile;      /* M2P interaction list entries staged in LDS per flush:
-                               0 = default (128 FP32, 64 FP64); FP32 64|128, FP64 32|64 */
+                               0 = default (64 FP32, 32 FP64); FP32 64|128, FP64 32|64 */
 } pepc_gpu_config;
 
 /* Counters of the last grow/traverse (tree_stats, module_tree.f90:622-699;
@@ -273,7 +273,7 @@ int pepc_gpu_replay_device(void *ctx, const double *x, const double *y, const do
 /* ---------------------------------------------------------------------------
  * Autotuner (no reference counterpart).  Times grow + traverse of the given
  * device arrays (one system, or nsys batched systems with host offsets) for
- * every candidate of group size {16, 32, 64} x group pack {0, 64} x list tile
+ * every candidate of group size {16, 32, 64} x group pack {0, 32, 64} x list tile
  * (FP32 {64, 128}, FP64 {32, 64}) x persistent grid {full, half}: one
  * warm-up and reps timed runs each, device time (HIP events), best of reps.
  * The fastest becomes the context's configuration (read it back with

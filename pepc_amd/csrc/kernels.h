@@ -140,7 +140,7 @@ struct WalkArgs {
 };
 void launch_walk_reset(hipStream_t s, const WalkArgs& a);
 void launch_queue_reset(hipStream_t s, uint32_t* queue);   // work-queue head only
-// list_tile: M2P list capacity in LDS (0 = default: 128 FP32, 64 FP64;
+// list_tile: M2P list capacity in LDS (0 = default: 64 FP32, 32 FP64;
 // FP32 takes 64 or 128, FP64 32 or 64).
 bool walk_list_tile_ok(int precision, int list_tile);
 void launch_walk(hipStream_t s, const WalkArgs& a, int precision, int blocks, int list_tile);

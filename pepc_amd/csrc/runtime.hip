@@ -1245,7 +1245,7 @@ int pepc_gpu_autotune(void* p, int64_t n, int64_t nsys, const int64_t* offsets, 
   const int prec = c->cfg.precision == PEPC_GPU_FP64 ? kWalkFP64 : kWalkFP32Kahan;
   const int tiles[2] = {prec == kWalkFP64 ? 32 : 64, prec == kWalkFP64 ? 64 : 128};
   const int sizes[3] = {16, 32, 64};
-  const int packs[2] = {0, 64};
+  const int packs[3] = {0, 32, 64};
   struct Cand { int g, pk, lt, wb; double tg, tw; };
   std::vector<Cand> cands;
   for (int g : sizes)

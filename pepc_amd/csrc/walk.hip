@@ -549,13 +549,13 @@ int walk_default_blocks(int precision, int list_tile) {
   int per = 0;
   hipError_t e;
   if (precision == kWalkFP64)
-    e = list_tile == 32
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<double, 32, false, false>, kWave, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<double, 64, false, false>, kWave, 0);
-  else
     e = list_tile == 64
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<float, 64, false, false>, kWave, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<float, 128, false, false>, kWave, 0);
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<double, 64, false, false>, kWave, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<double, 32, false, false>, kWave, 0);
+  else
+    e = list_tile == 128
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<float, 128, false, false>, kWave, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, walk_kernel<float, 64, false, false>, kWave, 0);
   if (e != hipSuccess || per < 1) per = 8;
   return cus * per;
 }
@@ -575,16 +575,18 @@ bool walk_list_tile_ok(int precision, int list_tile) {
 template <bool PER, bool SINK>
 static void launch_walk_t(hipStream_t s, const WalkArgs& a, int precision, int blocks,
                           int list_tile) {
+  // default (0): the smaller tile of each precision, whose lower LDS use
+  // lets one more wavefront per SIMD resident (measured faster on MI355X)
   if (precision == kWalkFP64) {
-    if (list_tile == 32)
-      walk_kernel<double, 32, PER, SINK><<<blocks, kWave, 0, s>>>(a);
-    else
-      walk_kernel<double, 64, PER, SINK><<<blocks, kWave, 0, s>>>(a);
-  } else {
     if (list_tile == 64)
-      walk_kernel<float, 64, PER, SINK><<<blocks, kWave, 0, s>>>(a);
+      walk_kernel<double, 64, PER, SINK><<<blocks, kWave, 0, s>>>(a);
     else
+      walk_kernel<double, 32, PER, SINK><<<blocks, kWave, 0, s>>>(a);
+  } else {
+    if (list_tile == 128)
       walk_kernel<float, 128, PER, SINK><<<blocks, kWave, 0, s>>>(a);
+    else
+      walk_kernel<float, 64, PER, SINK><<<blocks, kWave, 0, s>>>(a);
   }
 }
 

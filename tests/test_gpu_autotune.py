@@ -66,7 +66,7 @@ def test_autotune_installs_fastest(gpu):
     sol = gpu(theta2=s.theta2, eps2=s.eps2, precision="fp32")
     best, cands = sol.autotune(s.x, s.y, s.z, s.q, reps=1)
     print(f"best {best}", flush=True)
-    assert len(cands) == 24
+    assert len(cands) == 36
     tot = [c["t_grow_ms"] + c["t_walk_ms"] for c in cands]
     assert all(t > 0 for t in tot)
     assert best["t_grow_ms"] + best["t_walk_ms"] == pytest.approx(min(tot))
