@@ -115,6 +115,26 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
   return v;
 }
 
+// Number of set bits of m in the lanes below this one (v_mbcnt_lo/hi).
+__device__ __forceinline__ uint32_t mask_rank(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Exclusive prefix sum across the wave of values below 16, built from four
+// ballots (bit planes) instead of lane shuffles; *total gets the wave's sum.
+__device__ __forceinline__ uint32_t wave_exclusive_scan16(uint32_t v, uint32_t* total) {
+  uint32_t ex = 0, tot = 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const unsigned long long m = __ballot((v >> b) & 1u);
+    ex += mask_rank(m) << b;
+    tot += (uint32_t)__popcll(m) << b;
+  }
+  *total = tot;
+  return ex;
+}
+
 // Level of a key: position of the placeholder bit, module_spacefilling.f90:47-57.
 __device__ __forceinline__ int level_of_key(unsigned long long k) {
   return (63 - __clzll((long long)k)) / 3;

@@ -164,7 +164,6 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
   __shared__ uint32_t gsh;
 
   const int lane = threadIdx.x;
-  const unsigned long long lt = lanemask_lt();
   const R eps2 = (R)a.eps2;
   const double theta2 = a.theta2;
   const uint32_t ngroups = SINK ? a.nsink_groups : a.sc->ngroups;
@@ -210,7 +209,7 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
     for (int k = 0; k < 4; ++k) { acc.s[k] = R(0); acc.c[k] = R(0); acc.p[k] = R(0); }
 
     uint32_t sp = 0, nm = 0, np = 0, cnt_p2p = 0, cnt_m2p = 0, namb = 0, nsteps = 0;
-    unsigned long long nmac = 0;
+    uint32_t mac_lane = 0;   // this lane's MAC evaluations, summed over the wave at the end
     bool overflow = false;
     const uint32_t root = (!SINK && a.sys) ? a.sysroot[a.sys[a0]] : 0u;   // the group's own root
     double sx = xt, sy = yt, sz = zt;    // targets of the current box: x - vbox
@@ -331,7 +330,7 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
       const bool amb_me = twig && !acc_ok && !rej;
       unsigned long long amb = __ballot(amb_me);
       namb += (uint32_t)popc64(amb);
-      const uint32_t my_rank = (uint32_t)popc64(amb & lt);
+      const uint32_t my_rank = mask_rank(amb);
       uint32_t base = 0;
       while (amb) {
         unsigned long long mk = amb;
@@ -351,13 +350,13 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
         for (uint32_t k = 0; k < S && amb; ++k) amb &= amb - 1ull;
         base += S;
       }
-      nmac += (unsigned long long)wave_sum(twig ? macc : 0u);
+      mac_lane += twig ? macc : 0u;
 
       const bool take_m = twig && acc_ok;
       const unsigned long long am = __ballot(take_m);
       const unsigned long long pm = __ballot(leaf);
       if (take_m) {
-        const uint32_t pos = nm + (uint32_t)popc64(am & lt);
+        const uint32_t pos = nm + mask_rank(am);
         V4 e0;
         if constexpr (sizeof(R) == 4)
           e0 = make_float4((float)(gm.cx - fox), (float)(gm.cy - foy), (float)(gm.cz - foz), 0.0f);
@@ -372,7 +371,7 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
         lm[pos][0] = e0; lm[pos][1] = e1; lm[pos][2] = e2; lm[pos][3] = e3;
       }
       if (leaf) {
-        const uint32_t pos = np + (uint32_t)popc64(pm & lt);
+        const uint32_t pos = np + mask_rank(pm);
         V4 e;
         if constexpr (sizeof(R) == 4) {
           e = make_float4((float)(gm.cx - fox), (float)(gm.cy - foy), (float)(gm.cz - foz),
@@ -386,11 +385,11 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
       const uint32_t nma = (uint32_t)popc64(am), npa = (uint32_t)popc64(pm);
       nm += nma; np += npa; cnt_m2p += nma; cnt_p2p += npa;
 
-      const uint32_t nc = (twig && !acc_ok) ? lk.nchild : 0u;
-      const uint32_t incl = wave_inclusive_scan(nc);
-      const uint32_t tot = __shfl(incl, kWave - 1);
+      const uint32_t nc = (twig && !acc_ok) ? lk.nchild : 0u;   // at most 8
+      uint32_t tot;
+      const uint32_t excl = wave_exclusive_scan16(nc, &tot);
       if (sp + tot > (uint32_t)(kStackCap + kStackSlack)) { overflow = true; break; }
-      for (uint32_t k = 0; k < nc; ++k) stack[sp + incl - nc + k] = lk.first + k;
+      for (uint32_t k = 0; k < nc; ++k) stack[sp + excl + k] = lk.first + k;
       sp += tot;
       __syncthreads();
     }
@@ -426,6 +425,7 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
       a.np2p[tid] = cnt_p2p;
       a.nm2p[tid] = cnt_m2p;
     }
+    const unsigned long long nmac = SINK ? 0ull : wave_sum((unsigned long long)mac_lane);
     if (!SINK && lane == 0) {   // per-group record, summed by walk_stats_kernel
       uint4* gs = reinterpret_cast<uint4*>(a.gstat + 8 * (size_t)g);
       gs[0] = make_uint4((uint32_t)nmac, cnt_p2p, cnt_m2p, namb);
