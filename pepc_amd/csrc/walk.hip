@@ -54,7 +54,7 @@ namespace {
 
 constexpr int kStackCap = 512;      // entries the pop policy keeps below
 constexpr int kStackSlack = 192;    // >= 7 * 21 + 8: depth-first overrun bound
-constexpr int kListP = 128;
+constexpr int kListP = 64;
 
 template <typename R> struct Prec;
 template <> struct Prec<float> {
@@ -152,9 +152,14 @@ __device__ __forceinline__ bool mac_fails(double xt, double yt, double zt, doubl
 }
 
 // LM: capacity of the M2P list staged in LDS (the list tile); the P2P list
-// holds kListP entries.
+// holds kListP entries.  The default tiles (LM * sizeof(R) = 256 bytes per
+// float component: FP32 64, FP64 32) leave LDS for four one-wave workgroups
+// per SIMD; the register budget is held to the same four (at most 128 VGPRs).
+// The larger tiles are LDS-limited to three.
 template <typename R, int LM, bool PER, bool SINK>
-__global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
+__global__ __launch_bounds__(kWave)
+__attribute__((amdgpu_waves_per_eu(LM * sizeof(R) <= 256 ? 4 : 3)))
+void walk_kernel(WalkArgs a) {
   using P = Prec<R>;
   using V4 = typename P::V4;
   __shared__ uint32_t stack[kStackCap + kStackSlack];
@@ -326,29 +331,42 @@ __global__ __launch_bounds__(kWave) void walk_kernel(WalkArgs a) {
       }
       // Undecided nodes: exact per-target test.  Every slice holds all the
       // targets, so slice s tests the s-th undecided node of the batch and S
-      // nodes are settled per pass.
+      // nodes are settled per pass.  The nodes' centres and b^2 are staged in
+      // the free tail of the M2P list (entries nm .. nm + m - 1, which this
+      // step's appends overwrite only after the tests), so each slice reads
+      // its node with one broadcast load per pass.
       const bool amb_me = twig && !acc_ok && !rej;
-      unsigned long long amb = __ballot(amb_me);
-      namb += (uint32_t)popc64(amb);
-      const uint32_t my_rank = mask_rank(amb);
-      uint32_t base = 0;
-      while (amb) {
-        unsigned long long mk = amb;
-        for (uint32_t k = 0; k < s && mk; ++k) mk &= mk - 1ull;
-        const bool active = valid && mk != 0ull;
-        const int j = active ? __ffsll((long long)mk) - 1 : lane;
-        const double cx = __shfl(gm.cx, j), cy = __shfl(gm.cy, j), cz = __shfl(gm.cz, j);
-        const double b2 = __shfl(gm.b2, j);
-        const bool fail = active && mac_fails(sx, sy, sz, cx, cy, cz, b2, theta2);
-        const unsigned long long ball = __ballot(fail && valid);
-        if (amb_me && my_rank >= base && my_rank < base + S) {
-          const uint32_t r = my_rank - base;
-          const unsigned long long fm = (ball >> (r * T)) & (T == 64 ? ~0ull : ((1ull << T) - 1ull));
-          if (fm == 0ull) { acc_ok = true; macc = cnt; }
-          else { rej = true; macc = (uint32_t)__ffsll((long long)fm); }
+      const unsigned long long amb = __ballot(amb_me);
+      const uint32_t na = (uint32_t)popc64(amb);
+      namb += na;
+      if (na) {
+        const uint32_t my_rank = mask_rank(amb);
+        if (amb_me) {
+          double2* st = reinterpret_cast<double2*>(&lm[nm + my_rank][0]);
+          st[0] = make_double2(gm.cx, gm.cy);
+          st[1] = make_double2(gm.cz, gm.b2);
         }
-        for (uint32_t k = 0; k < S && amb; ++k) amb &= amb - 1ull;
-        base += S;
+        __syncthreads();
+        for (uint32_t base = 0; base < na; base += S) {
+          const uint32_t r = base + s;
+          const bool active = valid && r < na;
+          double2 c0 = make_double2(0.0, 0.0), c1 = c0;
+          if (active) {
+            const double2* st = reinterpret_cast<const double2*>(&lm[nm + r][0]);
+            c0 = st[0];
+            c1 = st[1];
+          }
+          const bool fail = active && mac_fails(sx, sy, sz, c0.x, c0.y, c1.x, c1.y, theta2);
+          const unsigned long long ball = __ballot(fail);
+          if (amb_me && my_rank >= base && my_rank < base + S) {
+            const uint32_t rr = my_rank - base;
+            const unsigned long long fm =
+                (ball >> (rr * T)) & (T == 64 ? ~0ull : ((1ull << T) - 1ull));
+            if (fm == 0ull) { acc_ok = true; macc = cnt; }
+            else { rej = true; macc = (uint32_t)__ffsll((long long)fm); }
+          }
+        }
+        __syncthreads();
       }
       mac_lane += twig ? macc : 0u;
 
