@@ -127,7 +127,8 @@ struct WalkArgs {
   uint32_t* np2p;
   uint32_t* nm2p;
   uint32_t* gstat;                   // 8 words per group: mac, p2p, m2p, amb, steps, count
-  uint32_t* queue;                   // work-queue head (own cache line)
+  uint32_t* queue;                   // work-queue heads, kQueueStride words apart
+  int nqueue = 1;                    // queues: one contiguous range of groups per XCD
   const uint32_t* sys;               // batched systems: system per particle (else nullptr)
   const uint32_t* sysroot;           // batched systems: root node per system
   int nbox;                          // periodic: lattice vectors walked per group (else 1)
@@ -139,7 +140,12 @@ struct WalkArgs {
   uint32_t nsink_groups = 0;
 };
 void launch_walk_reset(hipStream_t s, const WalkArgs& a);
-void launch_queue_reset(hipStream_t s, uint32_t* queue);   // work-queue head only
+// Work queues of the persistent walk: up to kMaxQueues heads, each on a
+// 64-byte line of its own (kQueueWords words in all).
+constexpr int kMaxQueues = 8;
+constexpr int kQueueStride = 16;
+constexpr int kQueueWords = kMaxQueues * kQueueStride;
+void launch_queue_reset(hipStream_t s, uint32_t* queue);   // work-queue heads only
 // list_tile: M2P list capacity in LDS (0 = default: 64 FP32, 32 FP64;
 // FP32 takes 64 or 128, FP64 32 or 64).
 bool walk_list_tile_ok(int precision, int list_tile);

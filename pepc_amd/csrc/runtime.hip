@@ -29,6 +29,7 @@ struct Ctx {
   int64_t n = 0, ncap = 0, mcap = 0, io_cap = 0;
   bool grown = false, walked = false;
   int walk_blocks = 0;
+  int nxcd = 1;   // XCDs of the device: one walk work queue each
 
   // particles
   double *in[4] = {nullptr, nullptr, nullptr, nullptr};   // host-path input copies
@@ -366,7 +367,7 @@ int traverse_device_impl(Ctx* c, double* ex, double* ey, double* ez, double* pot
   a.gfirst = c->gfirst; a.gcount = c->gcount; a.sc = c->sc;
   a.theta2 = c->cfg.theta2; a.eps2 = c->cfg.eps2; a.sign = c->cfg.sign;
   a.ex = ex; a.ey = ey; a.ez = ez; a.pot = pot; a.work = work;
-  a.np2p = c->np2p; a.nm2p = c->nm2p; a.gstat = c->gstat; a.queue = c->queue;
+  a.np2p = c->np2p; a.nm2p = c->nm2p; a.gstat = c->gstat; a.queue = c->queue; a.nqueue = c->nxcd;
   a.sys = c->nsys > 1 ? c->sys : nullptr;
   a.sysroot = c->sysroot;
   if (c->periodic && c->nsys > 1)
@@ -459,7 +460,7 @@ int traverse_sinks_impl(Ctx* c, int64_t m, const double* x, const double* y, con
   a.geom = c->geom; a.link = c->link; a.wm32 = c->wm32; a.wm64 = c->wm64;
   a.sc = c->sc;
   a.theta2 = c->cfg.theta2; a.eps2 = c->cfg.eps2; a.sign = c->cfg.sign;
-  a.np2p = c->sk_p2p; a.nm2p = c->sk_m2p; a.gstat = nullptr; a.queue = c->queue;
+  a.np2p = c->sk_p2p; a.nm2p = c->sk_m2p; a.gstat = nullptr; a.queue = c->queue; a.nqueue = c->nxcd;
   a.sys = nullptr; a.sysroot = c->sysroot;
   a.nbox = c->periodic ? c->nbox : 1;
   a.vbox = c->periodic ? c->vbox : nullptr;
@@ -546,11 +547,19 @@ int pepc_gpu_create(void** out, const pepc_gpu_config* cfg) {
       return PEPC_GPU_EHIP;
     }
   if (dalloc(c, &c->box, 1) || dalloc(c, &c->sc, 1) || dalloc(c, &c->partial, 1024 * 6) ||
-      dalloc(c, &c->queue, 64) ||
+      dalloc(c, &c->queue, kQueueWords) ||
       hipHostMalloc((void**)&c->sc_host, sizeof(DevScalars), hipHostMallocDefault) != hipSuccess) {
     free_all(c);
     delete c;
     return PEPC_GPU_ENOMEM;
+  }
+  {
+    int nx = 1;
+    if (hipDeviceGetAttribute(&nx, hipDeviceAttributeNumberOfXccs, cfg->device) != hipSuccess) {
+      (void)hipGetLastError();   // attribute unknown to this runtime: one queue
+      nx = 1;
+    }
+    c->nxcd = nx < 1 ? 1 : (nx > kMaxQueues ? kMaxQueues : nx);
   }
   c->walk_blocks = cfg->walk_blocks > 0
                        ? cfg->walk_blocks

@@ -176,12 +176,26 @@ void walk_kernel(WalkArgs a) {
   const double* __restrict__ TY = SINK ? a.ty : a.y;
   const double* __restrict__ TZ = SINK ? a.tz : a.z;
 
-  // Work queue: the first group of each wave is its block index, later ones
-  // come from a counter on a cache line of its own.
-  bool first = true;
+  // Work queues, one per XCD: the groups (in key order) are cut into nqueue
+  // contiguous ranges and workgroup b, which the dispatcher places on XCD
+  // b mod nqueue, takes groups from range b mod nqueue first, so the waves
+  // sharing an XCD's L2 walk neighbouring groups and reuse the same nodes.
+  // An exhausted range sends the wave on to the next one.
+  const uint32_t nq = (uint32_t)a.nqueue;
+  uint32_t q = blockIdx.x % nq, qleft = nq;   // lane 0's position in the queues
   for (;;) {
-    if (lane == 0) gsh = first ? blockIdx.x : gridDim.x + atomicAdd(a.queue, 1u);
-    first = false;
+    if (lane == 0) {
+      uint32_t gg = ngroups;
+      while (qleft) {
+        const uint32_t lo = (uint32_t)((uint64_t)ngroups * q / nq);
+        const uint32_t hi = (uint32_t)((uint64_t)ngroups * (q + 1) / nq);
+        const uint32_t r = atomicAdd(a.queue + kQueueStride * q, 1u);
+        if (r < hi - lo) { gg = lo + r; break; }
+        q = q + 1 == nq ? 0u : q + 1;
+        --qleft;
+      }
+      gsh = gg;
+    }
     __syncthreads();
     const uint32_t g = gsh;
     __syncthreads();
@@ -491,8 +505,8 @@ __global__ __launch_bounds__(256) void walk_stats_kernel(const uint32_t* __restr
 }
 
 __global__ void walk_reset_kernel(DevScalars* sc, uint32_t* queue) {
+  if (threadIdx.x < kMaxQueues) queue[kQueueStride * threadIdx.x] = 0;
   if (threadIdx.x == 0) {
-    *queue = 0;
     sc->interactions = 0;
     sc->mac_evals = 0;
     sc->p2p = 0;
@@ -503,7 +517,7 @@ __global__ void walk_reset_kernel(DevScalars* sc, uint32_t* queue) {
 }
 
 __global__ void queue_reset_kernel(uint32_t* queue) {
-  if (threadIdx.x == 0) *queue = 0;
+  if (threadIdx.x < kMaxQueues) queue[kQueueStride * threadIdx.x] = 0;
 }
 
 // FP64 direct sum (src/utils/module_directsum.f90:34-172): all sources
