@@ -365,6 +365,8 @@ struct UpArgs {
   const DevBox* boxes;
   const uint32_t* sys;
   const uint32_t* nfirst;
+  const uint32_t* trank;      // twig rank of each node (exclusive count of twigs before it)
+  const uint32_t* node_rank;  // node index of each twig rank
   const DevScalars* sc;
 };
 
@@ -462,18 +464,22 @@ __device__ void twig_moments(int64_t i, const NodeLink lk, const UpArgs& u) {
   write_walk_forms(i, m, b2, u.prec, u.geom, u.wm32, u.wm64);
 }
 
-// One launch per level, deepest first, one thread per node of the level
-// (leaves skip).  The launch boundary orders a level's writes before the
-// next level's reads on every XCD; a single-launch scheme would need an
-// agent-scope release per node, which writes back the L2 of its XCD.
+// One launch per level, deepest first, one thread per twig of the level:
+// nodes are ordered by level, so a level's twigs are one contiguous run of
+// twig ranks, [trank[first node], trank[first node of the next level]).
+// The launch boundary orders a level's writes before the next level's reads
+// on every XCD; a single-launch scheme would need an agent-scope release per
+// node, which writes back the L2 of its XCD.
 __global__ __launch_bounds__(kT) void upward_level_kernel(int level, UpArgs u) {
   if (tree_broken(u.sc)) return;
+  const int64_t nn = (int64_t)u.sc->nnodes, nt = (int64_t)u.sc->ntwig;
   const int64_t b = u.sc->level_offset[level], e = u.sc->level_offset[level + 1];
-  for (int64_t i = b + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < e;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const NodeLink lk = u.link[i];
-    if (lk.nchild == 0u) continue;
-    twig_moments(i, lk, u);
+  const int64_t tb = b < nn ? (int64_t)u.trank[b] : nt;
+  const int64_t te = e < nn ? (int64_t)u.trank[e] : nt;
+  for (int64_t r = tb + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < te;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = u.node_rank[r];
+    twig_moments(i, u.link[i], u);
   }
 }
 
@@ -685,9 +691,10 @@ void launch_upward(hipStream_t s, const TreeBuffers& t) {
   u.x = t.x; u.y = t.y; u.z = t.z; u.q = t.q;
   u.mom = t.mom64; u.geom = t.geom; u.wm32 = t.wm32; u.wm64 = t.wm64;
   u.boxes = t.box; u.sys = t.sys; u.nfirst = t.nfirst;
+  u.trank = t.flag; u.node_rank = t.node_rank;
   u.sc = t.sc;
   for (int l = kMaxLevel; l >= 0; --l)
-    upward_level_kernel<<<grid_for(t.cap / 4), kT, 0, s>>>(l, u);
+    upward_level_kernel<<<grid_for(t.cap / 8), kT, 0, s>>>(l, u);
 }
 
 void launch_tree_check(hipStream_t s, const NodeLink* link, const uint32_t* parent, int64_t n,
