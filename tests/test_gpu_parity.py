@@ -74,7 +74,7 @@ def test_keys_permutation_tree_bitexact(gpu, orc, name):
     sol.close()
 
 
-@pytest.mark.parametrize("G,pack", [(1, 0), (16, 0), (64, 0), (16, 64), (64, 64)])
+@pytest.mark.parametrize("G,pack", [(1, 0), (16, 0), (64, 0), (16, 32), (16, 64), (64, 64)])
 @pytest.mark.parametrize("name", ["mini", "plummer64k"])
 def test_groups_and_counts_bitexact(gpu, orc, name, G, pack):
     s, d, t, w = _oracle(orc, name, G, pack)
