@@ -154,36 +154,43 @@ def host_cpu():
     return None
 
 
-def cpu_baseline(sys_, group, stride, nthreads, pack=0):
+def cpu_baseline(sys_, group, stride, nthreads, pack=0, pp_stride=8):
     """The oracle (CPU restatement of the reference path, FP64) on this host:
-    grow timed whole on one thread (one reference rank), then the group walk
-    on every host thread at the reference's default tile rule (module_tree_walk_simple
-    TILE_SIZE = 16, no packing; the value) and at the GPU run's rule."""
+    grow timed whole (keys on every host thread, presort, tree and upward pass
+    serial as on one reference rank), then the group walk on every host thread
+    at the reference's default tile rule (module_tree_walk_simple TILE_SIZE =
+    16, no packing; the value), at the GPU run's rule, and per particle (the
+    pthreads walk's one target per traversal, module_tree_walk_pthreads.f90)
+    on every pp_stride-th particle, scaled."""
     from oracle import oracle as orc
     t0 = time.perf_counter()
     d = orc.domain_decompose(sys_.x, sys_.y, sys_.z, sys_.q)
     t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
     t_grow = time.perf_counter() - t0
 
-    def walk(G, P):
+    def walk(G, P, k):
         tiles = orc.walk_tiles(d.keys, G, P)
         t2 = time.perf_counter()
         w = orc.walk_group(d.x, d.y, d.z, t, sys_.theta2, sys_.eps2, tile_size=G, sign=sys_.sign,
-                           tile_stride=stride, nthreads=nthreads, tiles=tiles)
-        return (time.perf_counter() - t2) * stride, len(tiles[0]), w.num_interactions
+                           tile_stride=k, nthreads=nthreads, tiles=tiles)
+        return (time.perf_counter() - t2) * k, len(tiles[0]), w.num_interactions
 
-    t_ref, nt_ref, ni_ref = walk(16, 0)
-    t_same, nt_same, ni_same = walk(group, pack)
+    t_ref, nt_ref, ni_ref = walk(16, 0, stride)
+    t_same, nt_same, ni_same = walk(group, pack, stride)
+    t_pp, nt_pp, ni_pp = walk(1, 0, stride * pp_stride)
     return {"value": sys_.n / (t_grow + t_ref), "unit": "particle updates/s", "cores": nthreads,
             "kind": "port",
             "sample": (f"oracle (CPU restatement, FP64) on the same {sys_.n}-particle system: grow "
-                       f"(box, keys, reference MSD presort, tree, upward pass) timed whole on 1 "
-                       f"thread ({t_grow:.2f} s); walk at the reference's default tile rule "
-                       f"(G=16, {nt_ref} tiles, every {stride}-th, scaled) on {nthreads} threads "
-                       f"({t_ref:.2f} s)"),
+                       f"(box, keys on {nthreads} threads, reference MSD presort, tree, upward pass "
+                       f"serial) timed whole ({t_grow:.2f} s); walk at the reference's default "
+                       f"tile rule (G=16, {nt_ref} tiles, every {stride}-th, scaled) on "
+                       f"{nthreads} threads ({t_ref:.2f} s)"),
             "t_grow_s": t_grow, "t_walk_s": t_ref, "interactions": ni_ref * stride,
             "same_rule_as_gpu": {"group_size": group, "group_pack": pack, "t_walk_s": t_same,
                                  "value": sys_.n / (t_grow + t_same), "tiles": nt_same},
+            "per_particle": {"group_size": 1, "t_walk_s": t_pp, "value": sys_.n / (t_grow + t_pp),
+                             "sample": f"every {stride * pp_stride}-th of {nt_pp} particles, "
+                                       f"scaled", "interactions": ni_pp * stride * pp_stride},
             "host_cpu": host_cpu(), "host_threads_visible": os.cpu_count()}
 
 
