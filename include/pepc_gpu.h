@@ -55,7 +55,9 @@ typedef struct {
     double eps2;            /* Plummer softening squared (default 0) */
     double sign;            /* +1 Coulomb; -1 attractive (gravity-like) */
     double node_factor;     /* node capacity per particle (np_mult analogue, default 2.5) */
-    int32_t mac_select;     /* 0 = Barnes-Hut (the only value supported) */
+    int32_t mac_select;     /* 0 = Barnes-Hut theta2 d^2 > boxlength2(level); 1 = Bmax,
+                               theta2 d^2 > min(bmax^2, 3 boxlength2); any other value
+                               never accepts (N^2), as coulomb/module_interaction_specific.f90:256-275 */
     int32_t group_size;     /* walk tile size G, 1..64 (16 = module_tree_walk_simple TILE_SIZE) */
     int32_t precision;      /* PEPC_GPU_FP32_KAHAN or PEPC_GPU_FP64 */
     int32_t force_cubic_domain;  /* module_box.f90:29 */

@@ -75,6 +75,10 @@ def lib():
                                     C.c_int64, _i64, _i64, C.c_int64, C.c_int,
                                     _f64, _f64, _f64, _f64, _f64, _i32, _i32, _i64, _i64]
         L.or_walk_group.restype = C.c_int
+        L.or_set_mac_select.argtypes = [C.c_int]
+        L.or_set_mac_select.restype = None
+        L.or_get_mac_select.argtypes = []
+        L.or_get_mac_select.restype = C.c_int
         L.or_p2p.argtypes = [_f64, _f64, C.c_double, _f64, _f64]
         L.or_m2p.argtypes = [_f64, _f64, C.c_double, _f64, _f64]
         L.or_direct.argtypes = [C.c_int64, _f64, _f64, _f64, _f64, C.c_int64, _i64,
@@ -221,6 +225,24 @@ def tree_build(keys, x, y, z, q, b: Box, cap=None) -> Tree:
     out = {k: v[:nn] for k, v in arr.items() if k != "mom"}
     return Tree(nn, int(counts[1]), int(counts[2]), mom=arr["mom"][:nn * MOM].reshape(nn, MOM),
                 node_leaf=node_leaf, b2=boxlength2(b), **out)
+
+
+class mac_select:
+    """Context manager: MAC of every oracle walk inside the block
+    (coulomb/module_interaction_specific.f90:256-275): 0 Barnes-Hut, 1 Bmax
+    min(bmax^2, 3 boxlength2), other values never accept (N^2)."""
+
+    def __init__(self, value):
+        self.value = int(value)
+
+    def __enter__(self):
+        self.prev = lib().or_get_mac_select()
+        lib().or_set_mac_select(self.value)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_mac_select(self.prev)
+        return False
 
 
 def walk_tiles(keys, tile_size, pack=0):

@@ -487,6 +487,27 @@ void or_m2p(const double *t, const double d[3], double dist2, double exyz[3], do
 }
 
 /* ------------------------------------------------------------------------- */
+/* MAC selection, coulomb/module_interaction_specific.f90:256-275:            */
+/*   0: Barnes-Hut, accept iff theta2 * dist2 > boxlength2(level);            */
+/*   1: Bmax, accept iff theta2 * dist2 > min(bmax**2, 3.0 * boxlength2);     */
+/*   any other value: never accept (the reference's N^2 default case).        */
+/* ------------------------------------------------------------------------- */
+static int g_mac_select = 0;
+
+void or_set_mac_select(int mac_select) { g_mac_select = mac_select; }
+int or_get_mac_select(void) { return g_mac_select; }
+
+static inline int or_mac(double theta2, double dist2, double boxlength2, const double *m)
+{
+    if (g_mac_select == 0) return theta2 * dist2 > boxlength2;
+    if (g_mac_select == 1) {
+        double bm = m[OR_BMAX] * m[OR_BMAX], c = 3.0 * boxlength2;
+        return theta2 * dist2 > (bm < c ? bm : c);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Walk tiles: module_tree_walk_simple.f90:135-212 (maximal key cells holding */
 /* at most TILE_SIZE particles; TILE_SIZE=16 at :40 is the reference rule).   */
 /* ------------------------------------------------------------------------- */
@@ -538,7 +559,7 @@ int64_t or_walk_tiles(int64_t n, const uint64_t *keys, int64_t tile_size, int64_
 /* children in key order; leaf -> P2P for every tile particle (self when the  */
 /* separation is zero); twig -> MAC per particle in tile order, the first     */
 /* failure opens the node for the whole tile; all pass -> M2P for all.        */
-/* MAC: coulomb/module_interaction_specific.f90:256-275 (mac_select=0).        */
+/* MAC: coulomb/module_interaction_specific.f90:256-275 (or_mac above).       */
 /* ------------------------------------------------------------------------- */
 typedef struct {
     const double *x, *y, *z;
@@ -583,7 +604,7 @@ static void walk_aux(wk_ctx *w, int64_t node)
         d[ip][1] = (w->y[p] - w->vb[1]) - m[OR_COCY];
         d[ip][2] = (w->z[p] - w->vb[2]) - m[OR_COCZ];
         d2[ip] = d[ip][0] * d[ip][0] + d[ip][1] * d[ip][1] + d[ip][2] * d[ip][2];
-        if (!(w->theta2 * d2[ip] > w->b2[w->nlevel[node]])) {
+        if (!or_mac(w->theta2, d2[ip], w->b2[w->nlevel[node]], m)) {
             for (int64_t c = w->nfc[node]; c >= 0; c = w->nns[c]) walk_aux(w, c);
             return;
         }
@@ -734,7 +755,7 @@ static void lists_aux(wl_ctx *w, int64_t node)
         int64_t p = w->t0 + ip;
         double d0 = w->x[p] - m[OR_COCX], d1 = w->y[p] - m[OR_COCY], d2 = w->z[p] - m[OR_COCZ];
         double dd = d0 * d0 + d1 * d1 + d2 * d2;
-        if (!(w->theta2 * dd > w->b2[w->nlevel[node]])) {
+        if (!or_mac(w->theta2, dd, w->b2[w->nlevel[node]], m)) {
             for (int64_t c = w->nfc[node]; c >= 0; c = w->nns[c]) lists_aux(w, c);
             return;
         }

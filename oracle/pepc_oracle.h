@@ -25,6 +25,10 @@
  *                           over each node's particles (parallel-axis form)
  *   or_walk_group           theta=0 -> direct sum and N leaves per target;
  *                           G=1 vs per-particle recursion; FP64 direct sum
+ *   or_set_mac_select       mac_select=2 (never accept) -> direct sum, N
+ *                           leaves per target at any theta; mac_select=1 on a
+ *                           hand-built pair + far target: accepted exactly
+ *                           when theta^2 d^2 > bmax^2 while Barnes-Hut opens it
  *   or_p2p / or_m2p         closed-form Plummer pair; M2P with zero dip/quad
  *                           equals P2P; finite-difference E = -grad(phi)
  *   or_direct               textbook pair sum (numpy float64)
@@ -101,6 +105,10 @@ int64_t or_tree_build(int64_t n, const uint64_t *keys, const double *x, const do
 void or_boxlength2(const or_box *b, double b2[OR_MAXLEVEL + 1]);
 
 /* ---- tiles and walks ------------------------------------------------------- */
+/* MAC of every walk (coulomb/module_interaction_specific.f90:256-275): 0 =
+ * Barnes-Hut (default), 1 = min(bmax^2, 3 boxlength2), other = never accept. */
+void or_set_mac_select(int mac_select);
+int or_get_mac_select(void);
 int64_t or_walk_tiles(int64_t n, const uint64_t *keys, int64_t tile_size, int64_t pack,
                       int64_t *tile_first, int64_t *tile_count);
 int or_walk_group(int64_t n, const double *x, const double *y, const double *z,

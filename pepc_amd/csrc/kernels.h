@@ -86,6 +86,7 @@ struct TreeBuffers {
   DevBox* box;                       // one box per system
   DevScalars* sc;
   int prec;                          // walk record form to write (WalkPrecision)
+  int mac_select;                    // 0 Barnes-Hut, 1 Bmax, other: never accept
   // batched systems (sys == nullptr for a single system)
   const uint32_t* sys;               // system index per sorted particle
   const int64_t* soff;               // system ranges, nsys + 1 entries

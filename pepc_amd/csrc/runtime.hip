@@ -340,6 +340,7 @@ int grow_device_impl(Ctx* c, int64_t n, const double* x, const double* y, const 
   t.gmark = c->gmark; t.gfirst = c->gfirst; t.gcount = c->gcount;
   t.hist = c->hist; t.bsum = c->bsum; t.box = c->box; t.sc = c->sc;
   t.prec = c->cfg.precision == PEPC_GPU_FP64 ? kWalkFP64 : kWalkFP32Kahan;
+  t.mac_select = c->cfg.mac_select;
   t.sys = sys;
   t.soff = nsys > 1 ? c->soff : nullptr;
   t.sysroot = c->sysroot;
@@ -522,7 +523,7 @@ void pepc_gpu_config_default(pepc_gpu_config* cfg) {
 int pepc_gpu_create(void** out, const pepc_gpu_config* cfg) {
   if (!out || !cfg) return PEPC_GPU_EINVAL;
   *out = nullptr;
-  if (cfg->mac_select != 0 || cfg->group_size < 1 || cfg->group_size > 64 ||
+  if (cfg->group_size < 1 || cfg->group_size > 64 ||
       cfg->group_pack < 0 || cfg->group_pack > 64 ||
       !(cfg->theta2 >= 0.0) || !(cfg->eps2 >= 0.0) || !(cfg->node_factor >= 1.0) ||
       (cfg->precision != PEPC_GPU_FP32_KAHAN && cfg->precision != PEPC_GPU_FP64) ||

@@ -325,6 +325,19 @@ __global__ void root_parent_kernel(uint32_t* __restrict__ parent, const DevScala
 // Moment record slots, reference layout (t_tree_node_interaction_data order).
 enum { M_CX = 0, M_CY, M_CZ, M_Q, M_AQ, M_PX, M_PY, M_PZ, M_QX, M_QY, M_QZ, M_XY, M_YZ, M_ZX, M_BMAX };
 
+// The walk accepts a twig for a target iff theta2 * dist2 > b2 with the b2
+// written here, so the MAC selection (coulomb/module_interaction_specific.f90:
+// 256-275) is fixed per node: 0 Barnes-Hut, boxlength2(level); 1 Bmax,
+// min(bmax**2, 3.0 * boxlength2); any other value never accepts (N^2).
+__device__ __forceinline__ double walk_b2(const double* m, double boxlength2, int mac_select) {
+  if (mac_select == 0) return boxlength2;
+  if (mac_select == 1) {
+    const double bm = m[M_BMAX] * m[M_BMAX], c = 3.0 * boxlength2;
+    return bm < c ? bm : c;
+  }
+  return INFINITY;
+}
+
 __device__ __forceinline__ void write_walk_forms(int64_t i, const double* m, double b2, int prec,
                                                  NodeGeom* geom, NodeMoments<float>* wm32,
                                                  NodeMoments<double>* wm64) {
@@ -356,6 +369,7 @@ __device__ __forceinline__ void write_walk_forms(int64_t i, const double* m, dou
 // higher moments and bmax 0), a twig child's come from its record.
 struct UpArgs {
   int prec;
+  int mac_select;
   NodeLink* link;
   const double *x, *y, *z, *q;
   double* mom;
@@ -461,7 +475,7 @@ __device__ void twig_moments(int64_t i, const NodeLink lk, const UpArgs& u) {
   double2* out = reinterpret_cast<double2*>(u.mom + i * 16);
 #pragma unroll
   for (int k = 0; k < 8; ++k) out[k] = make_double2(m[2 * k], m[2 * k + 1]);
-  write_walk_forms(i, m, b2, u.prec, u.geom, u.wm32, u.wm64);
+  write_walk_forms(i, m, walk_b2(m, b2, u.mac_select), u.prec, u.geom, u.wm32, u.wm64);
 }
 
 // One launch per level, deepest first, one thread per twig of the level:
@@ -687,7 +701,7 @@ void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size, int 
 
 void launch_upward(hipStream_t s, const TreeBuffers& t) {
   UpArgs u;
-  u.prec = t.prec; u.link = t.link;
+  u.prec = t.prec; u.mac_select = t.mac_select; u.link = t.link;
   u.x = t.x; u.y = t.y; u.z = t.z; u.q = t.q;
   u.mom = t.mom64; u.geom = t.geom; u.wm32 = t.wm32; u.wm64 = t.wm64;
   u.boxes = t.box; u.sys = t.sys; u.nfirst = t.nfirst;

@@ -303,14 +303,15 @@ contains
             error stop 1
          end if
       end if
-      if (force_law /= 3 .or. mac_select /= 0) then
-         write (*, *) 'PEPC-GPU: only force_law = 3 (3-D Coulomb) and mac_select = 0 (BH)'
+      if (force_law /= 3) then
+         write (*, *) 'PEPC-GPU: only force_law = 3 (3-D Coulomb)'
          error stop 1
       end if
       if (c_associated(ctx)) call check(pepc_gpu_destroy(ctx), 'pepc_prepare')
       call pepc_gpu_config_default(cfg)
       cfg%theta2 = theta2
       cfg%eps2 = eps2
+      cfg%mac_select = mac_select
       cfg%sign = interaction_sign
       cfg%node_factor = max(1.0_c_double, real(abs(np_mult), c_double))
       cfg%group_size = group_size

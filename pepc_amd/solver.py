@@ -31,7 +31,7 @@ class Solver:
 
     def __init__(self, theta2=0.36, eps2=0.0, group_size=16, precision="fp32",
                  sign=1.0, node_factor=2.5, force_cubic_domain=False, walk_blocks=0,
-                 device=0, group_pack=0, list_tile=0):
+                 device=0, group_pack=0, list_tile=0, mac_select=0):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("pepc_amd.Solver needs a GPU (no HIP device visible)")
@@ -44,7 +44,8 @@ class Solver:
                                        sign=float(sign), node_factor=float(node_factor),
                                        force_cubic_domain=int(bool(force_cubic_domain)),
                                        walk_blocks=int(walk_blocks), device=int(device),
-                                       group_pack=int(group_pack), list_tile=int(list_tile))
+                                       group_pack=int(group_pack), list_tile=int(list_tile),
+                                       mac_select=int(mac_select))
         self.device = torch.device("cuda", device)
         h = C.c_void_p()
         rc = self.lib.pepc_gpu_create(C.byref(h), C.byref(self.cfg))

@@ -37,7 +37,7 @@ def test_create_rejects_bad_config_before_touching_device():
     from pepc_amd import _lib
     L = _lib.load()
     h = ctypes.c_void_p()
-    for kw in ({"group_size": 0}, {"group_size": 65}, {"mac_select": 1}, {"sign": 2.0},
+    for kw in ({"group_size": 0}, {"group_size": 65}, {"sign": 2.0},
                {"theta2": -1.0}, {"precision": 7}, {"list_tile": 32}, {"list_tile": 100},
                {"precision": 1, "list_tile": 128}, {"group_pack": 65}):
         cfg = _lib.default_config(**kw)
