@@ -17,16 +17,44 @@
 !                                             ! MPI-IO and binary checkpoints of
 !                                             ! step 7 under ./particles, read the
 !                                             ! MPI-IO one back into out.bin (no GPU)
+!   pepc_mini_gpu --vtk in.bin name           ! write the particles of in.bin as a
+!                                             ! three-step VTK series (steps 0, 1, 2;
+!                                             ! times 0, 0.5, 1) with work = 1 + step,
+!                                             ! pot = q (step + 1), field = (x (step + 1),
+!                                             ! -y, z + step); step 1 adds "L2 error" =
+!                                             ! 2 pot, step 2 scales coordinates by 2
+!                                             ! (no GPU)
 ! in.bin : int64 n, then float64 x(n), y(n), z(n), q(n)
 ! out.bin: int64 n, int64 label(n) (0-based input index), float64 x,y,z(n),
 !          ex,ey,ez,pot,work(n), all in the grown (key) order.
 ! sinks.bin: int64 m, float64 x(m), y(m), z(m); sinks_out.bin: int64 m, float64
 !          ex,ey,ez,pot,work(m) in the sinks' order (work starts at 1).
+!> A frontend's VTK data-array callback (module procedure: an internal one
+!> passed as an argument would need an executable stack for its trampoline).
+module pepc_mini_vtk
+   use module_interaction_specific_types
+   use module_vtk
+   use module_vtk_helpers
+   implicit none
+contains
+   !> The default arrays plus one more.
+   subroutine with_l2(d, r, vtkf)
+      type(t_particle_data), intent(in) :: d(:)
+      type(t_particle_results), intent(in) :: r(:)
+      type(vtkfile_unstructured_grid), intent(inout) :: vtkf
+      call vtk_write_particle_data_results(d, r, vtkf)
+      call vtkf%write_data_array("L2 error", 2.0_8 * r(:)%pot)
+   end subroutine with_l2
+end module pepc_mini_vtk
+
 program pepc_mini_gpu
    use module_pepc_kinds
    use module_pepc_types
    use module_pepc
    use module_checkpoint
+   use module_vtk
+   use module_vtk_helpers
+   use pepc_mini_vtk
    implicit none
 
    type(t_particle), allocatable :: particles(:), sinks(:)
@@ -43,6 +71,10 @@ program pepc_mini_gpu
       call get_command_argument(1, fin)
       if (trim(fin) == '--checkpoint') then
          call checkpoint_roundtrip()
+         stop
+      end if
+      if (trim(fin) == '--vtk') then
+         call vtk_series()
          stop
       end if
    end if
@@ -177,6 +209,41 @@ contains
       write (12) back
       close (12)
    end subroutine checkpoint_roundtrip
+
+   subroutine vtk_series()
+      integer :: k, vstep
+      call get_command_argument(2, fin)
+      call get_command_argument(3, fout)
+      open (11, file=trim(fin), access='stream', form='unformatted', status='old')
+      read (11) n
+      allocate (particles(n), a(n))
+      read (11) a; particles(:)%x(1) = a
+      read (11) a; particles(:)%x(2) = a
+      read (11) a; particles(:)%x(3) = a
+      read (11) a; particles(:)%data%q = a
+      close (11)
+      do i = 1, n
+         particles(i)%label = i - 1
+      end do
+      do k = 0, 2
+         particles(:)%work = 1.0_8 + k
+         particles(:)%results%pot = particles(:)%data%q * (k + 1)
+         particles(:)%results%e(1) = particles(:)%x(1) * (k + 1)
+         particles(:)%results%e(2) = -particles(:)%x(2)
+         particles(:)%results%e(3) = particles(:)%x(3) + k
+         vstep = VTK_STEP_NORMAL
+         if (k == 0) vstep = VTK_STEP_FIRST
+         if (k == 2) vstep = VTK_STEP_LAST
+         select case (k)
+         case (1)
+            call vtk_write_particles(trim(fout), 0, k, 0.5_8 * k, vstep, particles, with_l2)
+         case (2)
+            call vtk_write_particles(trim(fout), 0, k, 0.5_8 * k, vstep, particles, coord_scale=2.0_8)
+         case default
+            call vtk_write_particles(trim(fout), 0, k, 0.5_8 * k, vstep, particles)
+         end select
+      end do
+   end subroutine vtk_series
 
    subroutine init_particles(p)
       type(t_particle), intent(inout) :: p(:)
