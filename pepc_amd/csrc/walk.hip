@@ -259,19 +259,36 @@ void walk_kernel(WalkArgs a) {
       }
       fold(acc);
     };
+    // self term: the reference's rule (a zero separation from the leaf's
+    // particle) for sinks, the particle index for the grown particles
+    auto p2p_self = [&](uint32_t k) -> bool {
+      if constexpr (SINK) {
+        const uint32_t j = lpi[k];
+        return sx == a.x[j] && sy == a.y[j] && sz == a.z[j];
+      } else {
+        return lpi[k] == tid;   // mirror-box entries hold ~0u
+      }
+    };
+    // FP32: two entries per pass as for M2P (same per-lane order of terms)
     auto flush_p2p = [&]() {
-      for (uint32_t kk = 0; kk < np; kk += S) {
+      uint32_t kk = 0;
+      constexpr bool kPair = sizeof(R) == 4;
+      const uint32_t n2 = (kPair && np >= 2 * S) ? np - 2 * S + 1 : 0;
+      if (kPair && valid) {
+        for (; kk < n2; kk += 2 * S) {
+          const uint32_t k = kk + s, k2 = k + S;
+          const V4 e = lp[k], f = lp[k2];
+          const bool self1 = p2p_self(k), self2 = p2p_self(k2);
+          p2p<R>(tx - (R)e.x, ty - (R)e.y, tz - (R)e.z, (R)e.w, eps2, self1, acc.p);
+          p2p<R>(tx - (R)f.x, ty - (R)f.y, tz - (R)f.z, (R)f.w, eps2, self2, acc.p);
+        }
+      }
+      kk = n2 ? ((n2 - 1) / (2 * S) + 1) * 2 * S : 0;
+      for (; kk < np; kk += S) {
         const uint32_t k = kk + s;
         if (valid && k < np) {
           const V4 e = lp[k];
-          bool self;
-          if constexpr (SINK) {
-            const uint32_t j = lpi[k];
-            self = sx == a.x[j] && sy == a.y[j] && sz == a.z[j];
-          } else {
-            self = lpi[k] == tid;   // mirror-box entries hold ~0u
-          }
-          p2p<R>(tx - (R)e.x, ty - (R)e.y, tz - (R)e.z, (R)e.w, eps2, self, acc.p);
+          p2p<R>(tx - (R)e.x, ty - (R)e.y, tz - (R)e.z, (R)e.w, eps2, p2p_self(k), acc.p);
         }
       }
       fold(acc);
