@@ -300,7 +300,7 @@ def main():
         log(f"cpu baseline: {cpu['value']:.3e} updates/s ({cpu['sample']})")
 
     rec = {
-        "metric": "Coulomb particle updates/s, N=4M theta=0.6, FP32c vs FP64 at rel force err <=1e-4",
+        "metric": "Coulomb particle updates/s, N=4M \u03b8=0.6, FP32c vs FP64 at rel force err \u22641e-4",
         "value": s.n / (ms * 1e-3),
         "unit": "particle updates/s",
         "n_gpus": 1,
