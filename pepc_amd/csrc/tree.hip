@@ -707,8 +707,12 @@ void launch_upward(hipStream_t s, const TreeBuffers& t) {
   u.boxes = t.box; u.sys = t.sys; u.nfirst = t.nfirst;
   u.trank = t.flag; u.node_rank = t.node_rank;
   u.sc = t.sc;
+  // A fixed grid near the resident capacity (six 256-thread blocks per CU of
+  // MI355X) with a twig-stride loop: the large levels keep every CU busy and
+  // the many small levels launch few idle blocks.
+  constexpr unsigned kUpBlocks = 1536;
   for (int l = kMaxLevel; l >= 0; --l)
-    upward_level_kernel<<<grid_for(t.cap / 8), kT, 0, s>>>(l, u);
+    upward_level_kernel<<<kUpBlocks, kT, 0, s>>>(l, u);
 }
 
 void launch_tree_check(hipStream_t s, const NodeLink* link, const uint32_t* parent, int64_t n,
