@@ -39,6 +39,20 @@ module module_interaction_specific_types
 
    type(t_particle_results), parameter :: EMPTY_PARTICLE_RESULTS = &
       t_particle_results([0._kind_physics, 0._kind_physics, 0._kind_physics], 0._kind_physics)
+
+   !> Node moments in the reference's layout (read back from the device by
+   !> pepc_gpu_read_tree; declared for the frontends' VTK callbacks).
+   type t_tree_node_interaction_data
+      real(kind_physics) :: coc(3)
+      real(kind_physics) :: charge
+      real(kind_physics) :: abs_charge
+      real(kind_physics) :: dip(3)
+      real(kind_physics) :: quad(3)
+      real(kind_physics) :: xyquad
+      real(kind_physics) :: yzquad
+      real(kind_physics) :: zxquad
+      real(kind_physics) :: bmax
+   end type t_tree_node_interaction_data
 end module module_interaction_specific_types
 
 module module_pepc_types
@@ -54,6 +68,14 @@ module module_pepc_types
       type(t_particle_data) :: data
       type(t_particle_results) :: results
    end type t_particle
+
+   !> The host-side tree of the reference (module_tree.f90).  The tree of this
+   !> build lives on the device, so the host record is never allocated; the
+   !> frontends' tree output sees it as such and skips (as the reference does
+   !> for an unallocated tree).
+   type t_tree
+      logical :: allocated = .false.
+   end type t_tree
 end module module_pepc_types
 
 !> iso_c_binding interface of include/pepc_gpu.h
@@ -183,6 +205,8 @@ module module_pepc
    public :: pepc_timber_tree, pepc_restore_particles, pepc_grow_and_traverse
    public :: pepc_grow_and_traverse_for_others, pepc_check_sanity
    public :: pepc_statistics, pepc_last_stats, pepc_last_timings
+   !> module_pepc.f90's public tree; never allocated on the host here.
+   type(t_tree), public, target :: global_tree
 
    ! &calc_force_coulomb (coulomb/module_interaction_specific.f90:35-43)
    logical, public :: include_far_field_if_periodic = .true.

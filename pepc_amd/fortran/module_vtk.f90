@@ -15,6 +15,7 @@ module module_vtk
    integer, public, parameter :: VTK_STEP_NORMAL = 0
    integer, public, parameter :: VTK_STEP_LAST = 1      !< closes the time series
    integer, public, parameter :: VTK_VERTEX = 1
+   integer, public, parameter :: VTK_POLY_LINE = 4
 
    character(*), parameter :: dir_series = "./"
    character(*), parameter :: dir_pieces = "./vtk/"
@@ -49,7 +50,10 @@ module module_vtk
       procedure, private :: da_int8
       procedure, private :: da_int4
       procedure, private :: da_int4_repeat
-      generic :: write_data_array => da_real8, da_real8_3, da_int8, da_int4, da_int4_repeat
+      procedure, private :: da_int4_scalar
+      procedure, private :: da_int8_scalar
+      generic :: write_data_array => da_real8, da_real8_3, da_int8, da_int4, da_int4_repeat, &
+         da_int4_scalar, da_int8_scalar
    end type vtkfile_unstructured_grid
 
    interface
@@ -408,6 +412,21 @@ contains
       write (vtk%fh, '(a)') '</DataArray>'
    end subroutine da_int4
 
+   !> A one-element array (e.g. the offset or type of a single cell).
+   subroutine da_int4_scalar(vtk, name, value)
+      class(vtkfile_unstructured_grid) :: vtk
+      character(*) :: name
+      integer(c_int32_t) :: value
+      call da_int4(vtk, name, [value])
+   end subroutine da_int4_scalar
+
+   subroutine da_int8_scalar(vtk, name, value)
+      class(vtkfile_unstructured_grid) :: vtk
+      character(*) :: name
+      integer(c_int64_t) :: value
+      call da_int8(vtk, name, [value])
+   end subroutine da_int8_scalar
+
    !> n copies of one value (e.g. the writing rank per point).
    subroutine da_int4_repeat(vtk, name, n, value)
       class(vtkfile_unstructured_grid) :: vtk
@@ -428,7 +447,8 @@ module module_vtk_helpers
    use module_interaction_specific_types
    implicit none
    private
-   public :: vtk_write_particles, vtk_write_particle_data_results
+   public :: vtk_write_particles, vtk_write_particle_data_results, vtk_write_spacecurve
+   public :: vtk_write_branches, vtk_write_leaves
 
 contains
 
@@ -485,6 +505,85 @@ contains
       call vtk%write_final()
       call vtk%close()
    end subroutine vtk_write_particles
+
+   !> The space-filling curve: the particles, in the order given (the key
+   !> order after a grow), joined by one poly-line cell
+   !> (utils/module_vtk_helpers.f90:724-764).
+   subroutine vtk_write_spacecurve(step, tsim, vtk_step, particles, coord_scale)
+      integer, intent(in) :: step
+      integer, intent(in) :: vtk_step
+      real*8, intent(in) :: tsim
+      type(t_particle), intent(in) :: particles(:)
+      real(kind_physics), intent(in), optional :: coord_scale
+      type(vtkfile_unstructured_grid) :: vtk
+      integer(kind_particle) :: i, npp
+      npp = size(particles, kind=kind_particle)
+      call vtk%create_parallel("spacecurve", step, 0, 1, tsim, vtk_step)
+      call vtk%write_headers(npp, 1_kind_particle)
+      call vtk%startpoints()
+      call vtk%write_data_array("xyz", particles(:)%x(1), particles(:)%x(2), particles(:)%x(3), coord_scale)
+      call vtk%finishpoints()
+      call vtk%startpointdata()
+      call vtk%finishpointdata()
+      call vtk%startcells()
+      call vtk%write_data_array("connectivity", [(i, i=0, npp - 1)])
+      call vtk%write_data_array("offsets", npp)
+      call vtk%write_data_array("types", int(VTK_POLY_LINE, c_int32_t))
+      call vtk%finishcells()
+      call vtk%startcelldata()
+      call vtk%write_data_array("processor", 0_c_int32_t)
+      call vtk%finishcelldata()
+      call vtk%write_final()
+      call vtk%close()
+   end subroutine vtk_write_spacecurve
+
+   !> Branch and leaf nodes as boxes (utils/module_vtk_helpers.f90:553-700)
+   !> read the host tree.  This build keeps the tree on the device and never
+   !> allocates the host record, so these take the reference's path for an
+   !> unallocated tree: a message, no output.
+   subroutine vtk_write_branches(step, tsim, vtk_step, t, helper_func, coord_scale)
+      integer, intent(in) :: step
+      integer, intent(in) :: vtk_step
+      real*8, intent(in) :: tsim
+      type(t_tree), intent(in) :: t
+      interface
+         subroutine helper_func(d, vtkf)
+            use module_interaction_specific_types, only: t_tree_node_interaction_data
+            use module_vtk, only: vtkfile_unstructured_grid
+            implicit none
+            type(t_tree_node_interaction_data), intent(in) :: d(:)
+            type(vtkfile_unstructured_grid), intent(inout) :: vtkf
+         end subroutine helper_func
+      end interface
+      optional :: helper_func
+      real(kind_physics), intent(in), optional :: coord_scale
+      if (.not. t%allocated) then
+         write (*, *) 'vtk_write_branches(): tree is not allocated, aborting branch output.'
+         return
+      end if
+   end subroutine vtk_write_branches
+
+   subroutine vtk_write_leaves(step, tsim, vtk_step, t, helper_func, coord_scale)
+      integer, intent(in) :: step
+      integer, intent(in) :: vtk_step
+      real*8, intent(in) :: tsim
+      type(t_tree), intent(in) :: t
+      interface
+         subroutine helper_func(d, vtkf)
+            use module_interaction_specific_types, only: t_tree_node_interaction_data
+            use module_vtk, only: vtkfile_unstructured_grid
+            implicit none
+            type(t_tree_node_interaction_data), intent(in) :: d(:)
+            type(vtkfile_unstructured_grid), intent(inout) :: vtkf
+         end subroutine helper_func
+      end interface
+      optional :: helper_func
+      real(kind_physics), intent(in), optional :: coord_scale
+      if (.not. t%allocated) then
+         write (*, *) 'vtk_write_leaves(): tree is not allocated, aborting leaves output.'
+         return
+      end if
+   end subroutine vtk_write_leaves
 
    subroutine vtk_write_particle_data_results(d, r, vtkf)
       type(t_particle_data), intent(in) :: d(:)

@@ -22,7 +22,9 @@
 !                                             ! times 0, 0.5, 1) with work = 1 + step,
 !                                             ! pot = q (step + 1), field = (x (step + 1),
 !                                             ! -y, z + step); step 1 adds "L2 error" =
-!                                             ! 2 pot, step 2 scales coordinates by 2
+!                                             ! 2 pot, step 2 scales coordinates by 2;
+!                                             ! plus the space curve and the tree
+!                                             ! writers (which skip: no host tree)
 !                                             ! (no GPU)
 ! in.bin : int64 n, then float64 x(n), y(n), z(n), q(n)
 ! out.bin: int64 n, int64 label(n) (0-based input index), float64 x,y,z(n),
@@ -234,6 +236,9 @@ contains
          vstep = VTK_STEP_NORMAL
          if (k == 0) vstep = VTK_STEP_FIRST
          if (k == 2) vstep = VTK_STEP_LAST
+         call vtk_write_spacecurve(k, 0.5_8 * k, vstep, particles)
+         call vtk_write_branches(k, 0.5_8 * k, vstep, global_tree)
+         call vtk_write_leaves(k, 0.5_8 * k, vstep, global_tree)
          select case (k)
          case (1)
             call vtk_write_particles(trim(fout), 0, k, 0.5_8 * k, vstep, particles, with_l2)

@@ -34,7 +34,7 @@ def _arrays(path):
     assert root.get("type") == "UnstructuredGrid" and root.get("byte_order") == "BigEndian"
     piece = root.find("UnstructuredGrid/Piece")
     out = {}
-    for sec in ("Points", "PointData", "Cells"):
+    for sec in ("Points", "PointData", "Cells", "CellData"):
         for da in piece.find(sec).findall("DataArray"):
             assert da.get("format") == "binary"
             v = _decode(da.text, da.get("type"))
@@ -88,3 +88,33 @@ def test_particle_series(tmp_path):
     visit = (tmp_path / "parts.timeseries.visit").read_text().split()
     assert visit[:2] == ["!NBLOCKS", "1"]
     assert visit[2:] == [f"./vtk/parts_{k:06d}.000000.vtu" for k in range(3)]
+
+
+def test_spacecurve_and_tree_writers(tmp_path):
+    """vtk_write_spacecurve (module_vtk_helpers.f90:724-764): the particles in
+    order joined by one poly-line cell (VTK type 4) tagged with the rank.
+    vtk_write_branches / vtk_write_leaves see an unallocated host tree (the
+    tree lives on the device) and skip with the reference's message."""
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran frontend not built (amdflang absent)")
+    n = 40
+    r = np.random.default_rng(8)
+    x, y, z = r.random((3, n))
+    with open(tmp_path / "in.bin", "wb") as f:
+        np.array([n], np.int64).tofile(f)
+        for a in (x, y, z, r.normal(size=n)):
+            a.tofile(f)
+    res = subprocess.run([EXE, "--vtk", "in.bin", "parts"], cwd=tmp_path, capture_output=True,
+                         text=True, timeout=60)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert res.stdout.count("tree is not allocated, aborting branch output") == 3
+    assert res.stdout.count("tree is not allocated, aborting leaves output") == 3
+    for k in range(3):
+        piece, a = _arrays(tmp_path / "vtk" / f"spacecurve_{k:06d}.000000.vtu")
+        assert piece.get("NumberOfPoints") == str(n) and piece.get("NumberOfCells") == "1"
+        assert np.array_equal(a["xyz"], np.stack([x, y, z], 1))
+        assert np.array_equal(a["connectivity"], np.arange(n))
+        assert a["offsets"].tolist() == [n] and a["types"].tolist() == [4]
+        assert a["processor"].tolist() == [0]
+    sets = ET.parse(tmp_path / "spacecurve.timeseries.pvd").getroot().findall("Collection/DataSet")
+    assert [d.get("file") for d in sets] == [f"./vtk/spacecurve_{k:06d}.pvtu" for k in range(3)]
