@@ -205,6 +205,14 @@ int pepc_gpu_direct_device(void *ctx, int64_t n, const double *x, const double *
                            const int64_t *testidx, double eps2, double sign, double *ex,
                            double *ey, double *ez, double *pot);
 
+/* Host form of pepc_gpu_direct_device (directforce, src/utils/
+ * module_directsum.f90:34-172, for open boundaries): host arrays in and
+ * out, indices checked against n (EINVAL), synchronous; device memory for
+ * the sources is taken for the call and released after it. */
+int pepc_gpu_direct(void *ctx, int64_t n, const double *x, const double *y, const double *z,
+                    const double *q, int64_t ntest, const int64_t *testidx, double eps2,
+                    double sign, double *ex, double *ey, double *ez, double *pot);
+
 /* Device timing of the last grow and traverse in milliseconds (HIP events
  * on the context's stream, recorded by the *_device and host forms). */
 int pepc_gpu_timings(void *ctx, double *t_grow_ms, double *t_walk_ms);

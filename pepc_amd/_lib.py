@@ -30,7 +30,7 @@ EXPORTS = [
     "pepc_gpu_replay_device", "pepc_gpu_restore_device", "pepc_gpu_periodic_default",
     "pepc_gpu_set_periodic", "pepc_gpu_lattice_get", "pepc_gpu_constrain_periodic_device",
     "pepc_gpu_traverse_sinks_device", "pepc_gpu_traverse_sinks", "pepc_gpu_read_sink_counts",
-    "pepc_gpu_check_tree",
+    "pepc_gpu_check_tree", "pepc_gpu_direct",
 ]
 
 SINKS_PER_PARTICLE = 0
@@ -114,6 +114,8 @@ def load(path: str = LIB_PATH):
     L.pepc_gpu_read_walk.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp, C.POINTER(_i64)]
     L.pepc_gpu_direct_device.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp,
                                          C.c_double, C.c_double, _vp, _vp, _vp, _vp]
+    L.pepc_gpu_direct.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp,
+                                  C.c_double, C.c_double, _vp, _vp, _vp, _vp]
     L.pepc_gpu_timings.argtypes = [_vp, _dp, _dp]
     L.pepc_gpu_grow_batch_device.argtypes = [_vp, _i64, _vp, _vp, _vp, _vp, _vp]
     L.pepc_gpu_state_set.argtypes = [_vp, _i64, _i64, _vp] + [_vp] * 8 + [_i64]

@@ -894,6 +894,48 @@ int pepc_gpu_direct_device(void* p, int64_t n, const double* x, const double* y,
   return 0;
 }
 
+int pepc_gpu_direct(void* p, int64_t n, const double* x, const double* y, const double* z,
+                    const double* q, int64_t ntest, const int64_t* testidx, double eps2,
+                    double sign, double* ex, double* ey, double* ez, double* pot) {
+  Ctx* c = as_ctx(p);
+  if (!c || !x || !y || !z || !q || !testidx || !ex || !ey || !ez || !pot || n < 1 || ntest < 0)
+    return fail(c, PEPC_GPU_EINVAL, "direct: bad argument");
+  for (int64_t j = 0; j < ntest; ++j)
+    if (testidx[j] < 0 || testidx[j] >= n) return fail(c, PEPC_GPU_EINVAL, "direct: test index out of range");
+  if (ntest == 0) return 0;
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  // a diagnostic, not a solve: one temporary block for sources, indices and results
+  const size_t nb = sizeof(double) * (size_t)(4 * n + 5 * ntest);
+  double* d = nullptr;
+  if (hipMalloc((void**)&d, nb) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, PEPC_GPU_ENOMEM, "direct: no device memory for the sources");
+  }
+  double* dx = d; double* dy = dx + n; double* dz = dy + n; double* dq = dz + n;
+  int64_t* dt = reinterpret_cast<int64_t*>(dq + n);
+  double* r = reinterpret_cast<double*>(dt + ntest);
+  hipStream_t s = c->stream;
+  hipError_t e = hipSuccess;
+  const double* src[4] = {x, y, z, q};
+  double* dst[4] = {dx, dy, dz, dq};
+  for (int k = 0; k < 4 && e == hipSuccess; ++k)
+    e = hipMemcpyAsync(dst[k], src[k], sizeof(double) * n, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dt, testidx, sizeof(int64_t) * ntest, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    launch_direct(s, n, dx, dy, dz, dq, ntest, dt, eps2, sign, r, r + ntest, r + 2 * ntest,
+                  r + 3 * ntest);
+    e = hipGetLastError();
+  }
+  double* out[4] = {ex, ey, ez, pot};
+  for (int k = 0; k < 4 && e == hipSuccess; ++k)
+    e = hipMemcpyAsync(out[k], r + k * ntest, sizeof(double) * ntest, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(c, PEPC_GPU_EHIP, std::string("direct: ") + hipGetErrorString(e));
+  return 0;
+}
+
 int pepc_gpu_timings(void* p, double* t_grow_ms, double* t_walk_ms) {
   Ctx* c = as_ctx(p);
   if (!c) return PEPC_GPU_EINVAL;

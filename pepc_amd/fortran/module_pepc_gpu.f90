@@ -186,6 +186,17 @@ module pepc_gpu_c
          integer(c_int32_t), value :: mode
          real(c_double), intent(out) :: ex(*), ey(*), ez(*), pot(*), work(*)
       end function
+
+      integer(c_int) function pepc_gpu_direct(ctx, n, x, y, z, q, ntest, testidx, eps2, sign, &
+                                              ex, ey, ez, pot) bind(C, name="pepc_gpu_direct")
+         import :: c_int, c_ptr, c_int64_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: n, ntest
+         real(c_double), intent(in) :: x(*), y(*), z(*), q(*)
+         integer(c_int64_t), intent(in) :: testidx(*)
+         real(c_double), value :: eps2, sign
+         real(c_double), intent(out) :: ex(*), ey(*), ez(*), pot(*)
+      end function
    end interface
 end module pepc_gpu_c
 
@@ -204,7 +215,7 @@ module module_pepc
    public :: pepc_particleresults_clear, pepc_grow_tree, pepc_traverse_tree
    public :: pepc_timber_tree, pepc_restore_particles, pepc_grow_and_traverse
    public :: pepc_grow_and_traverse_for_others, pepc_check_sanity
-   public :: pepc_statistics, pepc_last_stats, pepc_last_timings
+   public :: pepc_statistics, pepc_last_stats, pepc_last_timings, pepc_direct_force
    !> module_pepc.f90's public tree; never allocated on the host here.
    type(t_tree), public, target :: global_tree
 
@@ -509,6 +520,42 @@ contains
       if (dealloc) call pepc_timber_tree()
    end subroutine pepc_grow_and_traverse_for_others
 
+   !> directforce (src/utils/module_directsum.f90:34-172) on the device, open
+   !> boundaries: field and potential at particles(testidx(1:ntest)) from all
+   !> particles, self excluded by index, with this module's eps2 and
+   !> interaction sign.  The periodic form (mirror boxes and the lattice term)
+   !> is not provided and stops with an error.
+   subroutine pepc_direct_force(particles, testidx, ntest, directresults)
+      type(t_particle), intent(in) :: particles(:)
+      integer(kind_particle), intent(in) :: testidx(:)
+      integer(kind_particle), intent(in) :: ntest
+      type(t_particle_results), intent(out) :: directresults(:)
+      real(c_double), allocatable :: x(:), y(:), z(:), q(:), ex(:), ey(:), ez(:), pot(:)
+      integer(c_int64_t), allocatable :: idx(:)
+      if (any(periodicity)) then
+         write (*, *) 'PEPC-GPU: directforce sums open boundaries only (no mirror boxes or lattice term)'
+         error stop 1
+      end if
+      if (.not. c_associated(ctx)) then
+         write (*, *) 'PEPC-GPU: directforce needs pepc_prepare first'
+         error stop 1
+      end if
+      if (ntest < 1) return
+      x = particles(:)%x(1)
+      y = particles(:)%x(2)
+      z = particles(:)%x(3)
+      q = particles(:)%data%q
+      idx = int(testidx(1:ntest) - 1, c_int64_t)
+      allocate (ex(ntest), ey(ntest), ez(ntest), pot(ntest))
+      call check(pepc_gpu_direct(ctx, size(particles, kind=c_int64_t), x, y, z, q, &
+                                 int(ntest, c_int64_t), idx, real(eps2, c_double), &
+                                 real(interaction_sign, c_double), ex, ey, ez, pot), 'directforce')
+      directresults(1:ntest)%e(1) = ex
+      directresults(1:ntest)%e(2) = ey
+      directresults(1:ntest)%e(3) = ez
+      directresults(1:ntest)%pot = pot
+   end subroutine pepc_direct_force
+
    !> module_pepc.f90:504-520: aborts if the grown tree is inconsistent;
    !> dump prints the tree's counters (the reference dumps its hash table)
    subroutine pepc_check_sanity(caller, dump, particles)
@@ -737,3 +784,27 @@ contains
       if (comm == comm) continue
    end subroutine read_particles_mpiio_from_filename
 end module module_checkpoint
+
+!> The frontends' direct-sum check (src/utils/module_directsum.f90:34-172).
+module module_directsum
+   use module_pepc_kinds
+   use module_pepc_types
+   use module_interaction_specific_types
+   use module_pepc, only: pepc_direct_force
+   implicit none
+   private
+   public :: directforce
+
+contains
+
+   !> Fields and potentials at particles(testidx(1:ntest)) from all particles
+   !> (open boundaries; one process, so comm is accepted and unused).
+   subroutine directforce(particles, testidx, ntest, directresults, comm)
+      type(t_particle), intent(in) :: particles(:)
+      integer(kind_particle), dimension(:), intent(in) :: testidx
+      integer(kind_particle), intent(in) :: ntest
+      type(t_particle_results), dimension(:), intent(out) :: directresults
+      integer, intent(in) :: comm
+      call pepc_direct_force(particles, testidx, ntest, directresults)
+   end subroutine directforce
+end module module_directsum

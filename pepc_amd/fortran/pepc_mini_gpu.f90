@@ -17,6 +17,10 @@
 !                                             ! MPI-IO and binary checkpoints of
 !                                             ! step 7 under ./particles, read the
 !                                             ! MPI-IO one back into out.bin (no GPU)
+!   pepc_mini_gpu --direct params in.bin out.bin
+!                                             ! directforce for every particle of
+!                                             ! in.bin (input order): out.bin holds
+!                                             ! int64 n, float64 ex,ey,ez,pot(n)
 !   pepc_mini_gpu --vtk in.bin name           ! write the particles of in.bin as a
 !                                             ! three-step VTK series (steps 0, 1, 2;
 !                                             ! times 0, 0.5, 1) with work = 1 + step,
@@ -54,6 +58,7 @@ program pepc_mini_gpu
    use module_pepc_types
    use module_pepc
    use module_checkpoint
+   use module_directsum
    use module_vtk
    use module_vtk_helpers
    use pepc_mini_vtk
@@ -77,6 +82,10 @@ program pepc_mini_gpu
       end if
       if (trim(fin) == '--vtk') then
          call vtk_series()
+         stop
+      end if
+      if (trim(fin) == '--direct') then
+         call direct_all()
          stop
       end if
    end if
@@ -211,6 +220,35 @@ contains
       write (12) back
       close (12)
    end subroutine checkpoint_roundtrip
+
+   subroutine direct_all()
+      type(t_particle_results), allocatable :: res(:)
+      integer(kind_particle), allocatable :: idx(:)
+      call get_command_argument(2, fin)
+      call pepc_initialize("pepc-mini-gpu", my_rank, n_ranks, .true.)
+      call pepc_read_parameters_from_file_name(trim(fin))
+      call pepc_prepare(3_kind_dim)
+      call get_command_argument(3, fin)
+      call get_command_argument(4, fout)
+      open (11, file=trim(fin), access='stream', form='unformatted', status='old')
+      read (11) n
+      allocate (particles(n), a(n), res(n), idx(n))
+      read (11) a; particles(:)%x(1) = a
+      read (11) a; particles(:)%x(2) = a
+      read (11) a; particles(:)%x(3) = a
+      read (11) a; particles(:)%data%q = a
+      close (11)
+      idx = [(i, i=1, n)]
+      call directforce(particles, idx, n, res, 0)
+      open (12, file=trim(fout), access='stream', form='unformatted', status='replace')
+      write (12) n
+      write (12) res(:)%e(1)
+      write (12) res(:)%e(2)
+      write (12) res(:)%e(3)
+      write (12) res(:)%pot
+      close (12)
+      call pepc_finalize()
+   end subroutine direct_all
 
    subroutine vtk_series()
       integer :: k, vstep

@@ -42,3 +42,4 @@ def test_create_rejects_bad_config_before_touching_device():
                {"precision": 1, "list_tile": 128}, {"group_pack": 65}):
         cfg = _lib.default_config(**kw)
         assert L.pepc_gpu_create(ctypes.byref(h), ctypes.byref(cfg)) == _lib.EINVAL
+

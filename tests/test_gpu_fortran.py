@@ -99,3 +99,31 @@ def test_fortran_timestep_loop_runs(tmp_path):
     assert len(lines) == 4
     ekin = [float(l.split("Ekin")[1].split()[0]) for l in lines]
     assert ekin[0] > 0 and np.all(np.isfinite(ekin))
+
+
+def test_fortran_directforce_matches_oracle(orc, tmp_path):
+    """module_directsum::directforce (src/utils/module_directsum.f90:34-172)
+    through the shim: every particle against all others, self excluded by
+    index, equals the oracle's FP64 direct sum to 1e-12."""
+    if not os.path.exists(EXE):
+        pytest.fail("pepc_mini_gpu not built (python -m pepc_amd.build)")
+    r = np.random.default_rng(43)
+    n = 3000
+    x, y, z = r.random((3, n))
+    q = r.uniform(-1.0, 1.0, n)
+    fin, fout, par = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "params"
+    with open(fin, "wb") as f:
+        np.array([n], np.int64).tofile(f)
+        for a in (x, y, z, q):
+            a.tofile(f)
+    _params(par, theta2=0.36, eps2=1e-4, G=16, prec="fp64")
+    res = subprocess.run([EXE, "--direct", str(par), str(fin), str(fout)], capture_output=True,
+                         text=True, timeout=120)
+    print(res.stdout[-2000:], res.stderr[-2000:], flush=True)
+    assert res.returncode == 0
+    raw = open(fout, "rb").read()
+    assert int(np.frombuffer(raw[:8], np.int64)[0]) == n
+    f = np.frombuffer(raw[8:], np.float64).reshape(4, n)
+    e_ref, p_ref = orc.direct(x, y, z, q, np.arange(n), 1e-4)
+    assert np.max(np.abs(f[:3].T - e_ref)) <= 1e-12 * np.abs(e_ref).max()
+    assert np.max(np.abs(f[3] - p_ref)) <= 1e-12 * np.abs(p_ref).max()

@@ -186,3 +186,27 @@ def test_tree_check_sound(gpu, name):
     sol.grow_batch(s.x, s.y, s.z, s.q, off)
     assert sol.check_tree() == 0
     sol.close()
+
+
+def test_direct_host_form(gpu, orc):
+    """pepc_gpu_direct (host arrays) equals the oracle's direct sum and
+    rejects test indices outside [0, n) with EINVAL before any device work."""
+    import ctypes as C
+    from pepc_amd import _lib
+    s = workloads.mini(n=2048, seed=5)
+    sol = gpu(theta2=s.theta2, eps2=s.eps2)
+    src = [np.ascontiguousarray(a, dtype=np.float64) for a in (s.x, s.y, s.z, s.q)]
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    for idx, want in ((np.array([0, 7, 2047, 1000], np.int64), _lib.OK),
+                      (np.array([0, 2048], np.int64), _lib.EINVAL),
+                      (np.array([-1], np.int64), _lib.EINVAL)):
+        out = [np.zeros(len(idx)) for _ in range(4)]
+        rc = sol.lib.pepc_gpu_direct(sol.h, s.n, *[vp(a) for a in src], len(idx), vp(idx),
+                                     s.eps2, 1.0, *[vp(a) for a in out])
+        assert rc == want
+        if rc == _lib.OK:
+            e_ref, p_ref = orc.direct(s.x, s.y, s.z, s.q, idx, s.eps2)
+            e = np.stack(out[:3], 1)
+            assert np.max(np.abs(e - e_ref)) <= 1e-12 * np.abs(e_ref).max()
+            assert np.max(np.abs(out[3] - p_ref)) <= 1e-12 * np.abs(p_ref).max()
+    sol.close()
