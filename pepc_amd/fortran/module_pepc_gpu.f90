@@ -200,12 +200,229 @@ module pepc_gpu_c
    end interface
 end module pepc_gpu_c
 
+!> Timers of the library and the frontends (lpepcsrc/module_timings.f90): the
+!> same ids and semantics (a running timer holds minus its start time) and
+!> timings_GatherAndOutput writing the timing/*.dat tables; this shim runs as
+!> one process, so maximum, average and minimum coincide.  The library fills
+!> t_fields_tree (grow), t_fields_passes (walk) and t_all (both) with the
+!> device times of each solve.
+module module_timings
+   use iso_c_binding
+   implicit none
+   private
+   integer, parameter, public :: t_domains = 1, t_allocate = 2, t_restore = 7, t_walk = 8, &
+      t_deallocate = 11, t_all = 12, t_local = 13, t_exchange_branches = 14, t_global = 15, &
+      t_lattice = 16, t_fields_tree = 18, t_fields_passes = 20, t_fields_stats = 21, &
+      t_domains_keys = 23, t_domains_sort = 24, t_domains_sort_pure = 25, t_domains_ship = 26, &
+      t_domains_bound = 27, t_grow_tree = 29, t_build_pure = 34, t_props_leaves = 37, &
+      t_tot = 41, t_direct_force = 49, t_direct_comm = 50, &
+      t_userdefined_first = 60, t_userdefined_last = 90
+   integer, parameter :: numtimings = t_userdefined_last
+   real*8 :: tim(numtimings) = 0.0d0
+   public :: timer_start, timer_stop, timer_resume, timer_add, timer_read, timer_reset
+   public :: timer_reset_all, timings_GatherAndOutput
+
+   interface
+      integer(c_int) function timing_mkdir(path, mode) bind(C, name='mkdir')
+         import :: c_int, c_char
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int), value :: mode
+      end function timing_mkdir
+   end interface
+
+contains
+
+   real*8 function wtime()
+      integer(8) :: count, rate
+      call system_clock(count, rate)
+      wtime = real(count, 8) / real(rate, 8)
+   end function wtime
+
+   real*8 function timer_read(id)
+      integer, intent(in) :: id
+      timer_read = tim(id)
+   end function timer_read
+
+   subroutine timer_reset(id)
+      integer, intent(in) :: id
+      tim(id) = 0.0d0
+   end subroutine timer_reset
+
+   subroutine timer_reset_all()
+      tim = 0.0d0
+   end subroutine timer_reset_all
+
+   subroutine timer_start(id)
+      integer, intent(in) :: id
+      tim(id) = -wtime()
+   end subroutine timer_start
+
+   subroutine timer_stop(id)
+      integer, intent(in) :: id
+      tim(id) = tim(id) + wtime()
+   end subroutine timer_stop
+
+   subroutine timer_resume(id)
+      integer, intent(in) :: id
+      tim(id) = tim(id) - wtime()
+   end subroutine timer_resume
+
+   subroutine timer_add(id, val)
+      integer, intent(in) :: id
+      real*8, intent(in) :: val
+      tim(id) = tim(id) + val
+   end subroutine timer_add
+
+   subroutine table(itime, flag, t, filename, header)
+      integer, intent(in) :: itime, flag
+      real*8, intent(in) :: t(numtimings)
+      character(*), intent(in) :: filename
+      logical, intent(in) :: header
+      integer :: u, i
+      character(40) :: fmt
+      open (newunit=u, file=filename, status='unknown', position='append')
+      if (header) then
+         write (fmt, '(a,i0,a)') '(2(1x,a20),', numtimings, '(1x,i20))'
+         write (u, fmt) "#timestep", "userflag", [(i, i=1, numtimings)]
+      end if
+      write (fmt, '(a,i0,a)') '(2(1x,i20),', numtimings, '(1x,e20.5))'
+      write (u, fmt) itime, flag, t
+      close (u)
+   end subroutine table
+
+   !> Appends this step's timers to timing/timing_{max,avg,min,dev_abs,dev_rel}.dat
+   !> (a header when itime <= 1 unless printheader says otherwise) and prints
+   !> the total.
+   subroutine timings_GatherAndOutput(itime, iuserflag, printheader)
+      integer, intent(in) :: itime
+      integer, optional, intent(in) :: iuserflag
+      logical, optional, intent(in) :: printheader
+      integer :: flag
+      logical :: header
+      integer(c_int) :: rc
+      real*8 :: zero(numtimings)
+      flag = 0
+      if (present(iuserflag)) flag = iuserflag
+      header = itime <= 1
+      if (present(printheader)) header = printheader
+      rc = timing_mkdir("timing"//c_null_char, int(o'755', c_int))
+      zero = 0.0d0
+      call table(itime, flag, tim, 'timing/timing_max.dat', header)
+      call table(itime, flag, tim, 'timing/timing_avg.dat', header)
+      call table(itime, flag, tim, 'timing/timing_min.dat', header)
+      call table(itime, flag, zero, 'timing/timing_dev_abs.dat', header)
+      call table(itime, flag, zero, 'timing/timing_dev_rel.dat', header)
+      write (*, '(a20,f16.10," s")') "t_all = ", tim(t_all)
+   end subroutine timings_GatherAndOutput
+end module module_timings
+
+!> Debug switches of the library (lpepcsrc/module_debug.f90:34-58): the
+!> bitmask debug_level (from pepc_initialize's db_level_in or &libpepc) and
+!> dbg(flag) to test it.
+module module_debug
+   implicit none
+   private
+   integer, public :: debug_level = 0
+   integer, parameter, public :: DBG_STATUS = 1, DBG_TREE = 2, DBG_BUILD = 4, DBG_DOMAIN = 8, &
+      DBG_BRANCH = 16, DBG_STATS = 32, DBG_WALKSUMMARY = 64, DBG_DUMPTREE = 128, &
+      DBG_TIMINGFILE = 256, DBG_WALK = 1024, DBG_PERIODIC = 2048
+   public :: dbg
+contains
+   logical function dbg(flag)
+      integer, intent(in) :: flag
+      dbg = iand(debug_level, flag) /= 0
+   end function dbg
+end module module_debug
+
+!> The Coulomb backend's parameters (coulomb/module_interaction_specific.f90:
+!> 35-43, the &calc_force_coulomb namelist).
+module module_interaction_specific
+   use module_pepc_kinds
+   implicit none
+   private
+   logical, public :: include_far_field_if_periodic = .true.
+   integer, public :: force_law = 3
+   integer, public :: mac_select = 0
+   real(kind_physics), public :: theta2 = 0.36_kind_physics
+   real(kind_physics), public :: eps2 = 0.0_kind_physics
+   ! read for the Kelbg law (force_law 6), which this backend does not offer
+   real(kind_physics), public :: kelbg_invsqrttemp = 0.0_kind_physics
+end module module_interaction_specific
+
+!> Periodic boundaries (module_mirror_boxes.f90:30-36, 174-210): the
+!> settings, the interaction cutoff (never cuts: huge, the reference's
+!> default) and constrain_periodic for a host particle array.
+module module_mirror_boxes
+   use module_pepc_kinds
+   use module_pepc_types
+   implicit none
+   private
+   logical, public :: periodicity(3) = [.false., .false., .false.]
+   integer, public :: mirror_box_layers = 1
+   real(kind_physics), public :: t_lattice_1(3) = [1, 0, 0]
+   real(kind_physics), public :: t_lattice_2(3) = [0, 1, 0]
+   real(kind_physics), public :: t_lattice_3(3) = [0, 0, 1]
+   real(kind_physics), public :: LatticeOrigin(3) = [0, 0, 0]
+   real(kind_physics), public :: spatial_interaction_cutoff(3) = huge(0._kind_physics)
+   public :: constrain_periodic
+contains
+   !> Wraps positions into the central box (module_mirror_boxes.f90:178-215).
+   !> A lattice with positive axes along x, y, z (module_mirror_boxes.f90:
+   !> 232-244): per periodic axis, Fortran MODULO relative to LatticeOrigin.
+   !> Any other lattice: a particle with a lattice coordinate outside [0, 1]
+   !> has all three wrapped, and only its periodic Cartesian components take
+   !> the wrapped position.
+   subroutine constrain_periodic(particles)
+      type(t_particle), intent(inout) :: particles(:)
+      real(kind_physics) :: a(3, 3), ainv(3, 3), det, c(3), r(3)
+      integer(kind_particle) :: i
+      integer :: k
+      logical :: simple
+      if (.not. any(periodicity)) return
+      a(:, 1) = t_lattice_1; a(:, 2) = t_lattice_2; a(:, 3) = t_lattice_3
+      simple = t_lattice_1(1) > 0 .and. t_lattice_1(2) == 0 .and. t_lattice_1(3) == 0 .and. &
+               t_lattice_2(1) == 0 .and. t_lattice_2(2) > 0 .and. t_lattice_2(3) == 0 .and. &
+               t_lattice_3(1) == 0 .and. t_lattice_3(2) == 0 .and. t_lattice_3(3) > 0
+      if (simple) then
+         do i = 1, size(particles, kind=kind_particle)
+            do k = 1, 3
+               if (periodicity(k)) particles(i)%x(k) = LatticeOrigin(k) + &
+                  modulo(particles(i)%x(k) - LatticeOrigin(k), a(k, k))
+            end do
+         end do
+         return
+      end if
+      det = a(1, 1) * (a(2, 2) * a(3, 3) - a(3, 2) * a(2, 3)) - a(1, 2) * (a(2, 1) * a(3, 3) - &
+            a(3, 1) * a(2, 3)) + a(1, 3) * (a(2, 1) * a(3, 2) - a(3, 1) * a(2, 2))
+      ainv(1, 1) = (a(2, 2) * a(3, 3) - a(3, 2) * a(2, 3)) / det
+      ainv(1, 2) = (a(1, 3) * a(3, 2) - a(1, 2) * a(3, 3)) / det
+      ainv(1, 3) = (a(1, 2) * a(2, 3) - a(1, 3) * a(2, 2)) / det
+      ainv(2, 1) = (a(2, 3) * a(3, 1) - a(2, 1) * a(3, 3)) / det
+      ainv(2, 2) = (a(1, 1) * a(3, 3) - a(1, 3) * a(3, 1)) / det
+      ainv(2, 3) = (a(2, 1) * a(1, 3) - a(1, 1) * a(2, 3)) / det
+      ainv(3, 1) = (a(2, 1) * a(3, 2) - a(3, 1) * a(2, 2)) / det
+      ainv(3, 2) = (a(3, 1) * a(1, 2) - a(1, 1) * a(3, 2)) / det
+      ainv(3, 3) = (a(1, 1) * a(2, 2) - a(2, 1) * a(1, 2)) / det
+      do i = 1, size(particles, kind=kind_particle)
+         c = matmul(ainv, particles(i)%x - LatticeOrigin)
+         if (any(c > 1.0_kind_physics) .or. any(c < 0.0_kind_physics)) then
+            r = LatticeOrigin + matmul(a, modulo(c, 1.0_kind_physics))
+            where (periodicity) particles(i)%x = r
+         end if
+      end do
+   end subroutine constrain_periodic
+end module module_mirror_boxes
+
 !> The frontend API of module_pepc.f90:32-56 on top of the GPU library.
 module module_pepc
    use iso_c_binding
    use module_pepc_kinds
    use module_pepc_types
    use pepc_gpu_c
+   use module_timings
+   use module_debug
+   use module_interaction_specific
+   use module_mirror_boxes
    implicit none
    private
 
@@ -219,12 +436,10 @@ module module_pepc
    !> module_pepc.f90's public tree; never allocated on the host here.
    type(t_tree), public, target :: global_tree
 
-   ! &calc_force_coulomb (coulomb/module_interaction_specific.f90:35-43)
-   logical, public :: include_far_field_if_periodic = .true.
-   integer, public :: force_law = 3
-   integer, public :: mac_select = 0
-   real(kind_physics), public :: theta2 = 0.36_kind_physics
-   real(kind_physics), public :: eps2 = 0.0_kind_physics
+   ! &calc_force_coulomb lives in module_interaction_specific, the periodic
+   ! settings in module_mirror_boxes (re-exported here)
+   public :: include_far_field_if_periodic, force_law, mac_select, theta2, eps2, kelbg_invsqrttemp
+   public :: periodicity, mirror_box_layers, t_lattice_1, t_lattice_2, t_lattice_3, LatticeOrigin
    ! &libpepc entries that apply (module_libpepc_main.f90:44, module_box.f90:29)
    logical, public :: force_cubic_domain = .false.
    real, public :: np_mult = 2.5
@@ -234,18 +449,17 @@ module module_pepc
    integer, public :: list_tile = 0           ! LDS M2P list entries (0 = default)
    character(len=8), public :: precision = 'fp32'
    real(kind_physics), public :: interaction_sign = 1.0_kind_physics
-   logical, public :: print_stats = .false.   ! the reference's DBG_STATS
-   ! periodic boundaries: &libpepc periodicity, mirror_box_layers and the
-   ! lattice of module_mirror_boxes.f90:30-36 (set before pepc_prepare)
-   logical, public :: periodicity(3) = [.false., .false., .false.]
-   integer, public :: mirror_box_layers = 1
-   real(kind_physics), public :: t_lattice_1(3) = [1, 0, 0]
-   real(kind_physics), public :: t_lattice_2(3) = [0, 1, 0]
-   real(kind_physics), public :: t_lattice_3(3) = [0, 0, 1]
-   real(kind_physics), public :: LatticeOrigin(3) = [0, 0, 0]
+   logical, public :: print_stats = .false.   ! also: debug_level with DBG_STATS
+   ! &libpepc entries read for compatibility: curve_type must be 1 (Hilbert,
+   ! as the reference requires), idim 3; the load-balancing weights, list
+   ! length factor and thread count have no role on one GPU
+   integer :: curve_type = 1, weighted = 1, interaction_list_length_factor = 1, num_threads = 3
+   integer :: idim = 3
 
-   namelist /calc_force_coulomb/ force_law, mac_select, include_far_field_if_periodic, theta2, eps2
-   namelist /libpepc/ periodicity, force_cubic_domain, np_mult, mirror_box_layers, print_stats
+   namelist /calc_force_coulomb/ force_law, mac_select, include_far_field_if_periodic, theta2, eps2, &
+      kelbg_invsqrttemp
+   namelist /libpepc/ debug_level, periodicity, np_mult, curve_type, force_cubic_domain, weighted, &
+      interaction_list_length_factor, mirror_box_layers, num_threads, idim, print_stats
    namelist /walk_para_gpu/ group_size, group_pack, list_tile, precision, interaction_sign
 
    type(c_ptr) :: ctx = c_null_ptr
@@ -283,18 +497,41 @@ contains
       n_cpu = 1
       my_rank_ = 0
       write (*, '(a,a)') ' PEPC-GPU frontend: ', trim(frontendname)
-      if (present(db_level_in) .or. present(comm) .or. init_mpi) continue
+      if (present(db_level_in)) debug_level = db_level_in   ! module_pepc.f90:162-164
+      if (present(comm) .or. init_mpi) continue
    end subroutine pepc_initialize
 
+   !> module_pepc.f90:230-244.  A group the file lacks keeps its defaults; a
+   !> group with an unknown entry or a bad value stops the run, as the
+   !> reference's plain namelist reads do.
    subroutine pepc_read_parameters(filehandle)
       integer, intent(in) :: filehandle
       integer :: ios
+      character(len=256) :: msg
       rewind (filehandle)
-      read (filehandle, NML=libpepc, iostat=ios)
+      read (filehandle, NML=libpepc, iostat=ios, iomsg=msg)
+      call bad_group('libpepc')
       rewind (filehandle)
-      read (filehandle, NML=calc_force_coulomb, iostat=ios)
+      read (filehandle, NML=calc_force_coulomb, iostat=ios, iomsg=msg)
+      call bad_group('calc_force_coulomb')
       rewind (filehandle)
-      read (filehandle, NML=walk_para_gpu, iostat=ios)
+      read (filehandle, NML=walk_para_gpu, iostat=ios, iomsg=msg)
+      call bad_group('walk_para_gpu')
+      if (curve_type /= 1) then       ! module_libpepc_main.f90:57-60
+         write (*, *) 'PEPC-GPU: only curve_type = 1 (Hilbert) is supported'
+         error stop 1
+      end if
+      if (idim /= 3) then
+         write (*, *) 'PEPC-GPU: only idim = 3 is supported'
+         error stop 1
+      end if
+   contains
+      subroutine bad_group(name)
+         character(*), intent(in) :: name
+         if (ios <= 0) return
+         write (*, '(a,a,a,a)') ' PEPC-GPU: error reading namelist &', name, ': ', trim(msg)
+         error stop 1
+      end subroutine bad_group
    end subroutine pepc_read_parameters
 
    !> module_pepc.f90:250-262: the namelists this library reads
@@ -459,7 +696,21 @@ contains
          end if
       end do
       deallocate (ex, ey, ez, pot, work)
+      call record_timers()
    end subroutine pepc_traverse_tree
+
+   !> The library's timers for this solve from the device clock: tree build
+   !> (t_fields_tree), walk (t_fields_passes) and both (t_all), in seconds.
+   subroutine record_timers()
+      real(c_double) :: tg, tw
+      call check(pepc_gpu_timings(ctx, tg, tw), 'pepc_traverse_tree')
+      call timer_reset(t_fields_tree)
+      call timer_reset(t_fields_passes)
+      call timer_reset(t_all)
+      if (tg > 0) call timer_add(t_fields_tree, real(tg, 8) * 1.0d-3)
+      if (tw > 0) call timer_add(t_fields_passes, real(tw, 8) * 1.0d-3)
+      call timer_add(t_all, timer_read(t_fields_tree) + timer_read(t_fields_passes))
+   end subroutine record_timers
 
    !> module_pepc.f90:538
    subroutine pepc_timber_tree()
@@ -492,7 +743,7 @@ contains
       if (present(no_restore)) restore = .not. no_restore
       call pepc_grow_tree(particles)
       call pepc_traverse_tree(particles)
-      if (print_stats) call pepc_statistics(itime)
+      if (print_stats .or. dbg(DBG_STATS)) call pepc_statistics(itime)
       if (restore) call pepc_restore_particles(particles)
       if (dealloc) call pepc_timber_tree()
    end subroutine pepc_grow_and_traverse
@@ -515,7 +766,7 @@ contains
       else
          call pepc_traverse_tree(particles_source)
       end if
-      if (print_stats) call pepc_statistics(itime)
+      if (print_stats .or. dbg(DBG_STATS)) call pepc_statistics(itime)
       if (restore) call pepc_restore_particles(particles_source)
       if (dealloc) call pepc_timber_tree()
    end subroutine pepc_grow_and_traverse_for_others

@@ -30,6 +30,17 @@
 !                                             ! plus the space curve and the tree
 !                                             ! writers (which skip: no host tree)
 !                                             ! (no GPU)
+!   pepc_mini_gpu --library params in.bin out.bin
+!                                             ! the library modules without a GPU:
+!                                             ! read params, wrap the particles of
+!                                             ! in.bin with constrain_periodic
+!                                             ! (out.bin: int64 n, float64 x,y,z(n)),
+!                                             ! time two steps into timing/*.dat
+!                                             ! (user timer t_userdefined_first + 1
+!                                             ! gets 1.25 s, then 0.5 s more) and
+!                                             ! print the settings read; the
+!                                             ! lattice comes from &lattice_test
+!                                             ! (t1, t2, t3, origin) if present
 ! in.bin : int64 n, then float64 x(n), y(n), z(n), q(n)
 ! out.bin: int64 n, int64 label(n) (0-based input index), float64 x,y,z(n),
 !          ex,ey,ez,pot,work(n), all in the grown (key) order.
@@ -54,6 +65,9 @@ contains
 end module pepc_mini_vtk
 
 program pepc_mini_gpu
+   use module_timings
+   use module_debug
+   use module_mirror_boxes
    use module_pepc_kinds
    use module_pepc_types
    use module_pepc
@@ -86,6 +100,10 @@ program pepc_mini_gpu
       end if
       if (trim(fin) == '--direct') then
          call direct_all()
+         stop
+      end if
+      if (trim(fin) == '--library') then
+         call library_modules()
          stop
       end if
    end if
@@ -183,7 +201,9 @@ program pepc_mini_gpu
       write (*, '(a,i5,a,es12.4,a,es12.4,a,f9.3,a,f9.3,a,f9.3)') ' step', step, '  Epot', epot, &
          '  Ekin', ekin, '  grow+walk [ms]', 1e3*real(t1 - t0, 8)/real(rate, 8), '  dev grow', tg, &
          '  dev walk', tw
+      if (dbg(DBG_STATS)) call pepc_statistics(step)
       call pepc_timber_tree()
+      call timings_GatherAndOutput(step, 0)      ! pepc-mini/pepc.f90:99
    end do
    call pepc_finalize()
 
@@ -249,6 +269,53 @@ contains
       close (12)
       call pepc_finalize()
    end subroutine direct_all
+
+   subroutine library_modules()
+      integer :: k, ios
+      real(kind_physics) :: t1(3), t2(3), t3(3), origin(3)
+      namelist /lattice_test/ t1, t2, t3, origin
+      call pepc_initialize("pepc-mini-gpu", my_rank, n_ranks, .true., db_level_in=0)
+      call get_command_argument(2, fin)
+      call pepc_read_parameters_from_file_name(trim(fin))
+      ! the frontend's lattice (frontends set module_mirror_boxes' variables)
+      t1 = t_lattice_1; t2 = t_lattice_2; t3 = t_lattice_3; origin = LatticeOrigin
+      open (13, file=trim(fin), status='old', action='read')
+      read (13, NML=lattice_test, iostat=ios)
+      close (13)
+      t_lattice_1 = t1; t_lattice_2 = t2; t_lattice_3 = t3; LatticeOrigin = origin
+      call get_command_argument(3, fin)
+      call get_command_argument(4, fout)
+      open (11, file=trim(fin), access='stream', form='unformatted', status='old')
+      read (11) n
+      allocate (particles(n), a(n))
+      do k = 1, 3
+         read (11) a
+         particles(:)%x(k) = a
+      end do
+      close (11)
+      call constrain_periodic(particles)
+      open (12, file=trim(fout), access='stream', form='unformatted', status='replace')
+      write (12) n
+      do k = 1, 3
+         write (12) particles(:)%x(k)
+      end do
+      close (12)
+      call timer_reset_all()
+      call timer_start(t_userdefined_first)
+      call timer_stop(t_userdefined_first)
+      call timer_add(t_userdefined_first + 1, 1.25_8)
+      call timings_GatherAndOutput(1)
+      call timer_add(t_userdefined_first + 1, 0.5_8)
+      call timings_GatherAndOutput(2, 7)
+      write (*, '(a,l1)') 'dbg_stats ', dbg(DBG_STATS)
+      write (*, '(a,i0)') 'debug_level ', debug_level
+      write (*, '(a,3l1)') 'periodicity ', periodicity
+      write (*, '(a,i0)') 'mirror_box_layers ', mirror_box_layers
+      write (*, '(a,i0)') 'mac_select ', mac_select
+      write (*, '(a,es24.16)') 'theta2 ', theta2
+      write (*, '(a,es24.16)') 'eps2 ', eps2
+      write (*, '(a,es24.16)') 'kelbg_invsqrttemp ', kelbg_invsqrttemp
+   end subroutine library_modules
 
    subroutine vtk_series()
       integer :: k, vstep

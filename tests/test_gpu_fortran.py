@@ -92,13 +92,26 @@ def test_fortran_traverse_other_particles(tmp_path):
 def test_fortran_timestep_loop_runs(tmp_path):
     par = tmp_path / "params"
     _params(par, theta2=0.36, eps2=1e-4, G=64, prec="fp32", tnp=20000, nt=3)
-    r = subprocess.run([EXE, str(par)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([EXE, str(par)], capture_output=True, text=True, timeout=120,
+                       cwd=tmp_path)
     print(r.stdout[-3000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0
     lines = [l for l in r.stdout.splitlines() if l.strip().startswith("step")]
     assert len(lines) == 4
     ekin = [float(l.split("Ekin")[1].split()[0]) for l in lines]
     assert ekin[0] > 0 and np.all(np.isfinite(ekin))
+    # timings_GatherAndOutput per step: the library's device times in
+    # t_fields_tree (18) and t_fields_passes (20), their sum in t_all (12),
+    # matching the driver's own "dev grow"/"dev walk" figures (ms)
+    rows = np.loadtxt(tmp_path / "timing" / "timing_avg.dat", comments="#", ndmin=2)
+    assert rows.shape == (4, 92) and list(rows[:, 0]) == [0, 1, 2, 3]
+    tree, passes, total = rows[:, 1 + 18], rows[:, 1 + 20], rows[:, 1 + 12]
+    assert np.all(tree > 0) and np.all(passes > 0)
+    np.testing.assert_allclose(total, tree + passes, rtol=2e-4)   # e20.5: 5 digits
+    dev = np.array([[float(l.split("dev grow")[1].split()[0]),
+                     float(l.split("dev walk")[1].split()[0])] for l in lines])
+    np.testing.assert_allclose(tree * 1e3, dev[:, 0], rtol=2e-3, atol=2e-3)
+    np.testing.assert_allclose(passes * 1e3, dev[:, 1], rtol=2e-3, atol=2e-3)
 
 
 def test_fortran_directforce_matches_oracle(orc, tmp_path):
