@@ -941,13 +941,21 @@ int pepc_gpu_timings(void* p, double* t_grow_ms, double* t_walk_ms) {
   if (!c) return PEPC_GPU_EINVAL;
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   float a = 0.f, b = 0.f;
+  // an interval whose events were not recorded reads -1; the failed query's
+  // error is cleared so that the next call's launch check does not see it
   if (t_grow_ms) {
     *t_grow_ms = -1.0;
-    if (c->grown && hipEventElapsedTime(&a, c->ev[0], c->ev[1]) == hipSuccess) *t_grow_ms = a;
+    if (c->grown) {
+      if (hipEventElapsedTime(&a, c->ev[0], c->ev[1]) == hipSuccess) *t_grow_ms = a;
+      else (void)hipGetLastError();
+    }
   }
   if (t_walk_ms) {
     *t_walk_ms = -1.0;
-    if (c->walked && hipEventElapsedTime(&b, c->ev[2], c->ev[3]) == hipSuccess) *t_walk_ms = b;
+    if (c->walked) {
+      if (hipEventElapsedTime(&b, c->ev[2], c->ev[3]) == hipSuccess) *t_walk_ms = b;
+      else (void)hipGetLastError();
+    }
   }
   return 0;
 }
