@@ -190,6 +190,10 @@ void drop_graph(Ctx* c) {
 int ensure_particles(Ctx* c, int64_t n) {
   if (n <= c->ncap) return 0;
   const int64_t nc = n + n / 8 + 64;
+  // node indices (links, parents, stacks) are 32-bit, with 0xFFFFFFFF as the
+  // no-parent mark: the pool must stay below it
+  if (c->cfg.node_factor * (double)nc + 64.0 >= 4294967295.0)
+    return fail(c, PEPC_GPU_EINVAL, "node pool of node_factor * n nodes exceeds 32-bit node indices");
   ALLOC(c, c->k0, nc); ALLOC(c, c->k1, nc); ALLOC(c, c->oldkeys, nc);
   ALLOC(c, c->v0, nc); ALLOC(c, c->v1, nc);
   ALLOC(c, c->xs, nc); ALLOC(c, c->ys, nc); ALLOC(c, c->zs, nc); ALLOC(c, c->qs, nc);

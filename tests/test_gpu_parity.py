@@ -171,6 +171,12 @@ def test_error_paths(gpu):
         sol.grow(s.x, s.y, s.z, s.q)
     assert ei.value.code == _lib.EOVERFLOW
     sol.close()
+    # a node pool beyond 32-bit node indices is refused before any allocation
+    sol = gpu(node_factor=1e6)
+    with pytest.raises(_lib.PepcGpuError) as ei:
+        sol.grow(s.x, s.y, s.z, s.q)
+    assert ei.value.code == _lib.EINVAL and "32-bit" in str(ei.value)
+    sol.close()
 
 
 @pytest.mark.parametrize("name", list(SYSTEMS))
