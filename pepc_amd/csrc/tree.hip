@@ -381,15 +381,16 @@ struct UpArgs {
   const uint32_t* nfirst;
   const uint32_t* trank;      // twig rank of each node (exclusive count of twigs before it)
   const uint32_t* node_rank;  // node index of each twig rank
+  const uint32_t* fc_rank;    // first child of each twig rank (+1 sentinel)
   const DevScalars* sc;
 };
 
-__device__ void twig_moments(int64_t i, const NodeLink lk, const UpArgs& u) {
-  const uint32_t c0 = lk.first, nc = lk.nchild;
-  const double b2 = u.boxes[u.sys ? u.sys[u.nfirst[i]] : 0u].b2[lk.level];
+__device__ void twig_moments(int64_t i, uint32_t c0, uint32_t nc, int level, const UpArgs& u) {
+  const double b2 = u.boxes[u.sys ? u.sys[u.nfirst[i]] : 0u].b2[level];
   // Every child's charge, |charge| and centre are requested together (a leaf
-  // child's from its particle, a twig child's from the first 48 bytes of its
-  // record), so the loads of all children overlap.
+  // child's from its geometry record, which holds its particle's position
+  // and charge and is loaded with its link; a twig child's from the first 48
+  // bytes of its moment record), so the loads of all children overlap.
   double cq[8], ca[8], cx[8], cy[8], cz[8];
   uint32_t leafmask = 0, count = 0;
 #pragma unroll
@@ -397,11 +398,11 @@ __device__ void twig_moments(int64_t i, const NodeLink lk, const UpArgs& u) {
     cq[j] = ca[j] = cx[j] = cy[j] = cz[j] = 0.0;
     if (j < nc) {
       const NodeLink cl = u.link[c0 + j];
+      const NodeGeom g = u.geom[c0 + j];
       count += cl.count;
       if (cl.nchild == 0) {
         leafmask |= 1u << j;
-        const uint32_t p = cl.first;
-        cx[j] = u.x[p]; cy[j] = u.y[p]; cz[j] = u.z[p]; cq[j] = u.q[p];
+        cx[j] = g.cx; cy[j] = g.cy; cz[j] = g.cz; cq[j] = g.b2;
         ca[j] = fabs(cq[j]);
       } else {
         const double2* r = reinterpret_cast<const double2*>(u.mom + (int64_t)(c0 + j) * 16);
@@ -481,6 +482,8 @@ __device__ void twig_moments(int64_t i, const NodeLink lk, const UpArgs& u) {
 // One launch per level, deepest first, one thread per twig of the level:
 // nodes are ordered by level, so a level's twigs are one contiguous run of
 // twig ranks, [trank[first node], trank[first node of the next level]).
+// A twig's children come from the rank-indexed first-child array (children
+// of consecutive twigs are consecutive), read together with its node index.
 // The launch boundary orders a level's writes before the next level's reads
 // on every XCD; a single-launch scheme would need an agent-scope release per
 // node, which writes back the L2 of its XCD.
@@ -493,7 +496,8 @@ __global__ __launch_bounds__(kT) void upward_level_kernel(int level, UpArgs u) {
   for (int64_t r = tb + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < te;
        r += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = u.node_rank[r];
-    twig_moments(i, u.link[i], u);
+    const uint32_t c0 = u.fc_rank[r];
+    twig_moments(i, c0, u.fc_rank[r + 1] - c0, level, u);
   }
 }
 
@@ -705,7 +709,7 @@ void launch_upward(hipStream_t s, const TreeBuffers& t) {
   u.x = t.x; u.y = t.y; u.z = t.z; u.q = t.q;
   u.mom = t.mom64; u.geom = t.geom; u.wm32 = t.wm32; u.wm64 = t.wm64;
   u.boxes = t.box; u.sys = t.sys; u.nfirst = t.nfirst;
-  u.trank = t.flag; u.node_rank = t.node_rank;
+  u.trank = t.flag; u.node_rank = t.node_rank; u.fc_rank = t.fc_rank;
   u.sc = t.sc;
   // A fixed grid near the resident capacity (six 256-thread blocks per CU of
   // MI355X) with a twig-stride loop: the large levels keep every CU busy and
