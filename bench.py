@@ -142,6 +142,21 @@ def force_error(sol, out, sys_, nsamp):
     return float(np.median(rel)), float(np.max(rel)), float(np.median(prel))
 
 
+def tree_error(out, ref):
+    """Relative difference of two solves of the same grown tree (same order):
+    the precision the FP32 + Kahan path loses against the FP64 tree at the
+    same theta and expansion order, over every target."""
+    e = (out.e - ref.e).norm(dim=0).cpu().numpy()
+    en = ref.e.norm(dim=0).cpu().numpy()
+    pd = (out.pot - ref.pot).abs().cpu().numpy()
+    pn = ref.pot.abs().cpu().numpy()
+    ok, okp = en > 0, pn > 0
+    rel, prel = e[ok] / en[ok], pd[okp] / pn[okp]
+    return {"targets": int(ok.sum()), "median_rel_force_err": float(np.median(rel)),
+            "p999_rel_force_err": float(np.quantile(rel, 0.999)), "max_rel_force_err": float(rel.max()),
+            "median_rel_pot_err": float(np.median(prel))}
+
+
 def host_cpu():
     """Model name of this host's CPU (from /proc/cpuinfo), or None."""
     try:
@@ -275,9 +290,10 @@ def main():
                                                       1, dev_inputs, args.pack)
         med64, mx64, _ = force_error(sol64, out64, s, min(args.err_samples, 16384))
         fp64 = {"ms_per_step": ms64, "t_walk_ms": w64, "t_grow_ms": g64,
-                "median_rel_force_err": med64, "speedup_fp32_over_fp64": ms64 / ms}
+                "median_rel_force_err": med64, "speedup_fp32_over_fp64": ms64 / ms,
+                "fp32_vs_fp64_tree": tree_error(out, out64)}
         log(f"fp64: {ms64:.3f} ms/step walk {w64:.3f} ms; median err {med64:.3e}; "
-            f"fp32 speedup {ms64 / ms:.2f}x")
+            f"fp32 speedup {ms64 / ms:.2f}x; fp32 vs fp64 tree {json.dumps(fp64['fp32_vs_fp64_tree'])}")
         sol64.close()
 
     ts_ms = ts_drift = None
