@@ -52,7 +52,8 @@ namespace pepc {
 
 namespace {
 
-constexpr int kStackCap = 512;      // entries the pop policy keeps below
+constexpr int kStackCap = 512;      // entries the pop policy keeps below (FP64)
+constexpr int kStackCap32 = 768;    // the same for FP32, whose lists leave LDS for it
 constexpr int kStackSlack = 192;    // >= 7 * 21 + 8: depth-first overrun bound
 constexpr int kListP = 64;
 
@@ -162,7 +163,12 @@ __attribute__((amdgpu_waves_per_eu(LM * sizeof(R) <= 256 ? 4 : 3)))
 void walk_kernel(WalkArgs a) {
   using P = Prec<R>;
   using V4 = typename P::V4;
-  __shared__ uint32_t stack[kStackCap + kStackSlack];
+  // A step pops at most (cap - sp) / 7 nodes (each may push 8 children), so a
+  // deep stack means small steps; wide near fields (small theta) fill it.
+  // Four one-wave workgroups per SIMD leave 10 KB of LDS each: FP32 lists
+  // take 5.4 KB, FP64 lists 6.4 KB, so only FP32 has room for the larger cap.
+  constexpr int kCap = sizeof(R) == 4 ? kStackCap32 : kStackCap;
+  __shared__ uint32_t stack[kCap + kStackSlack];
   __shared__ V4 lm[LM][4];
   __shared__ V4 lp[kListP];
   __shared__ uint32_t lpi[kListP];
@@ -310,7 +316,7 @@ void walk_kernel(WalkArgs a) {
     sp = 1;
     __syncthreads();
     while (sp > 0) {
-      int room = ((int)kStackCap - (int)sp) / 7;
+      int room = ((int)kCap - (int)sp) / 7;
       room = room < 1 ? 1 : room;
       uint32_t m = sp < (uint32_t)room ? sp : (uint32_t)room;
       // at most min(64, LM) nodes per step, so one step's appends always fit
@@ -437,7 +443,7 @@ void walk_kernel(WalkArgs a) {
       const uint32_t nc = (twig && !acc_ok) ? lk.nchild : 0u;   // at most 8
       uint32_t tot;
       const uint32_t excl = wave_exclusive_scan16(nc, &tot);
-      if (sp + tot > (uint32_t)(kStackCap + kStackSlack)) { overflow = true; break; }
+      if (sp + tot > (uint32_t)(kCap + kStackSlack)) { overflow = true; break; }
       for (uint32_t k = 0; k < nc; ++k) stack[sp + excl + k] = lk.first + k;
       sp += tot;
       __syncthreads();
