@@ -316,20 +316,18 @@ void walk_kernel(WalkArgs a) {
     sp = 1;
     __syncthreads();
     while (sp > 0) {
-      int room = ((int)kCap - (int)sp) / 7;
-      room = room < 1 ? 1 : room;
-      uint32_t m = sp < (uint32_t)room ? sp : (uint32_t)room;
-      // at most min(64, LM) nodes per step, so one step's appends always fit
-      // an emptied list (positions nm + rank < LM)
+      // Up to min(64, LM) nodes are read from the top of the stack (so one
+      // step's appends always fit an emptied list, positions nm + rank < LM);
+      // after the box test only the longest run of them whose pushes fit
+      // below the cap is visited, the rest stay on the stack.
       constexpr uint32_t kPop = (uint32_t)(LM < kWave ? LM : kWave);
       static_assert(kListP >= kWave, "P2P list must hold one full step");
-      m = m > kPop ? kPop : m;
+      const uint32_t m = sp < kPop ? sp : kPop;
       if (nm + m > (uint32_t)LM) { __syncthreads(); flush_m2p(); nm = 0; }
       if (np + m > (uint32_t)kListP) { __syncthreads(); flush_p2p(); np = 0; }
       const bool has = (uint32_t)lane < m;
       const uint32_t node = has ? stack[sp - 1 - lane] : 0u;
       __syncthreads();
-      sp -= m;
       ++nsteps;
 
       // All of a node's data is requested at once, before any decision, so a
@@ -346,14 +344,11 @@ void walk_kernel(WalkArgs a) {
         else src = reinterpret_cast<const V4*>(&a.wm64[node]);
         w0 = src[0]; w1 = src[1]; w2 = src[2];
       }
-      const bool leaf = has && lk.nchild == 0u;
-      const bool twig = has && lk.nchild != 0u;
-
       // Decide from the group's bounding box when possible (margins cover
       // every rounding of the exact per-target test below).
       bool acc_ok = false, rej = false;
       uint32_t macc = 0;
-      if (twig) {
+      if (has && lk.nchild != 0u) {
         const double ex = fmax(fmax(blox - gm.cx, gm.cx - bhix), 0.0);
         const double ey = fmax(fmax(bloy - gm.cy, gm.cy - bhiy), 0.0);
         const double ez = fmax(fmax(bloz - gm.cz, gm.cz - bhiz), 0.0);
@@ -366,6 +361,28 @@ void walk_kernel(WalkArgs a) {
         rej = !acc_ok && (theta2 * dmax2) * (1.0 + 1e-12) <= gm.b2;
         macc = acc_ok ? cnt : (rej ? 1u : 0u);
       }
+      // Visit lanes 0 .. k-1 for the largest k whose step leaves at most kCap
+      // entries: sp - k + (children of lanes < k, counted for every twig the
+      // box did not accept) <= kCap.  k >= 1 always, so the stack passes the
+      // cap only by the one-node steps of a depth-first descent (within
+      // kStackSlack), and k is never below the worst-case count
+      // (kCap - sp) / 7.  Nodes k .. m-1 stay on the stack unvisited: none of
+      // their decisions is counted.
+      // While the worst case fits (7 net pushes per node), all m are visited
+      // without the search.
+      uint32_t k = m;
+      if ((int)(7u * m) > (int)kCap - (int)sp) {
+        const uint32_t pmax = (has && lk.nchild != 0u && !acc_ok) ? lk.nchild : 0u;
+        uint32_t ptot;
+        const uint32_t pex = wave_exclusive_scan16(pmax, &ptot);
+        const bool fits = has && (int)(pex + pmax) - (lane + 1) <= (int)kCap - (int)sp;
+        const unsigned long long fm = __ballot(fits);
+        k = fm ? 64u - (uint32_t)__clzll((long long)fm) : 1u;
+      }
+      const bool keep = (uint32_t)lane < k;
+      sp -= k;
+      const bool leaf = keep && lk.nchild == 0u;
+      const bool twig = keep && lk.nchild != 0u;
       // Undecided nodes: exact per-target test.  Every slice holds all the
       // targets, so slice s tests the s-th undecided node of the batch and S
       // nodes are settled per pass.  The nodes' centres and b^2 are staged in
