@@ -107,7 +107,11 @@ int pepc_gpu_set_stream(void *ctx, void *hip_stream);
  * module_tree_grow.f90:37-138), host arrays.  Inputs x,y,z,q[n] in any
  * order; outputs (each may be NULL): keys_out[n] sorted keys, perm_out[n],
  * xs/ys/zs[n] sorted positions after the duplicate-key fix-up
- * (module_domains.f90:299-305 rewrites moved particles).  Synchronous. */
+ * (module_domains.f90:299-305 rewrites moved particles).  Synchronous.
+ * When the tree needs more nodes than the pool holds, this host form
+ * enlarges the pool to the count the build found and grows again, so it
+ * never returns PEPC_GPU_EOVERFLOW (the device forms do, and leave the pool
+ * size to node_factor and pepc_gpu_reserve_nodes). */
 int pepc_gpu_grow(void *ctx, int64_t n, const double *x, const double *y, const double *z,
                   const double *q, uint64_t *keys_out, int64_t *perm_out,
                   double *xs, double *ys, double *zs);
@@ -161,6 +165,13 @@ int pepc_gpu_check_tree(void *ctx, int64_t *nbad);
  * and reports errors raised by the kernels. */
 int pepc_gpu_sync(void *ctx, pepc_gpu_stats *st);
 
+/* Lower bound of the node pool: at least `nodes` entries from the next grow
+ * on (the pool is node_factor * n otherwise).  tree_create's table size
+ * (module_tree.f90:177-186: max(30 nintmax + 4 N, 10^4) for np_mult > 0,
+ * |np_mult| 10^4 otherwise, at least 2^15) maps here; the Fortran shim
+ * passes it before each grow.  Enlarging the pool drops the current tree. */
+int pepc_gpu_reserve_nodes(void *ctx, int64_t nodes);
+
 /* Replaces pepc_timber_tree (module_pepc.f90:538): releases nothing but
  * marks the tree invalid (device buffers are kept for the next step). */
 int pepc_gpu_timber(void *ctx);
@@ -192,6 +203,12 @@ int pepc_gpu_read_sorted(void *ctx, uint64_t *keys, int64_t *perm, double *x, do
  * node count in *nnodes. */
 int pepc_gpu_read_tree(void *ctx, int64_t cap, uint32_t *link, uint32_t *first, double *mom,
                        int64_t *nnodes);
+
+/* Bounding box of system sys (0 for a single system) of the last grow, as
+ * box_create computes it (module_box.f90:48-93, t_box :33-38): boxmin[3],
+ * boxmax[3], boxsize[3] (each may be NULL).  The tree writers place node
+ * boxes with it (key_to_coord, module_spacefilling.f90:445-471). */
+int pepc_gpu_read_box(void *ctx, int64_t sys, double *boxmin, double *boxmax, double *boxsize);
 
 /* Copies the walk groups and per-target counts of the last traverse:
  * gfirst/gcount/gmac[ngroups] and p2p/m2p[n] (sorted order). */

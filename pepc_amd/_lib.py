@@ -30,7 +30,7 @@ EXPORTS = [
     "pepc_gpu_replay_device", "pepc_gpu_restore_device", "pepc_gpu_periodic_default",
     "pepc_gpu_set_periodic", "pepc_gpu_lattice_get", "pepc_gpu_constrain_periodic_device",
     "pepc_gpu_traverse_sinks_device", "pepc_gpu_traverse_sinks", "pepc_gpu_read_sink_counts",
-    "pepc_gpu_check_tree", "pepc_gpu_direct",
+    "pepc_gpu_check_tree", "pepc_gpu_direct", "pepc_gpu_reserve_nodes", "pepc_gpu_read_box",
 ]
 
 SINKS_PER_PARTICLE = 0
@@ -139,6 +139,8 @@ def load(path: str = LIB_PATH):
     L.pepc_gpu_traverse_sinks.argtypes = [_vp, _i64, _vp, _vp, _vp, C.c_int32] + [_vp] * 5
     L.pepc_gpu_read_sink_counts.argtypes = [_vp, _i64, _vp, _vp]
     L.pepc_gpu_check_tree.argtypes = [_vp, C.POINTER(_i64)]
+    L.pepc_gpu_reserve_nodes.argtypes = [_vp, _i64]
+    L.pepc_gpu_read_box.argtypes = [_vp, _i64, _vp, _vp, _vp]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is not None and name not in ("pepc_gpu_config_default",

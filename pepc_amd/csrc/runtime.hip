@@ -27,6 +27,7 @@ struct Ctx {
   bool own_stream = false;
   std::string err;
   int64_t n = 0, ncap = 0, mcap = 0, io_cap = 0;
+  int64_t node_min = 0;   // lower bound of the node pool (pepc_gpu_reserve_nodes)
   bool grown = false, walked = false;
   int walk_blocks = 0;
   int nxcd = 1;   // XCDs of the device: one walk work queue each
@@ -187,24 +188,14 @@ void drop_graph(Ctx* c) {
   c->graph = nullptr;
 }
 
-int ensure_particles(Ctx* c, int64_t n) {
-  if (n <= c->ncap) return 0;
-  const int64_t nc = n + n / 8 + 64;
+// Node arrays for mc nodes, and the radix histogram and scan blocks sized
+// for the larger of the particle and node arrays.  Drops the current tree.
+int alloc_nodes(Ctx* c, int64_t mc) {
   // node indices (links, parents, stacks) are 32-bit, with 0xFFFFFFFF as the
   // no-parent mark: the pool must stay below it
-  if (c->cfg.node_factor * (double)nc + 64.0 >= 4294967295.0)
-    return fail(c, PEPC_GPU_EINVAL, "node pool of node_factor * n nodes exceeds 32-bit node indices");
-  ALLOC(c, c->k0, nc); ALLOC(c, c->k1, nc); ALLOC(c, c->oldkeys, nc);
-  ALLOC(c, c->v0, nc); ALLOC(c, c->v1, nc);
-  ALLOC(c, c->xs, nc); ALLOC(c, c->ys, nc); ALLOC(c, c->zs, nc); ALLOC(c, c->qs, nc);
-  ALLOC(c, c->prec4, nc);
-  ALLOC(c, c->lo, nc); ALLOC(c, c->hi, nc);
-  ALLOC(c, c->cnt, nc); ALLOC(c, c->gmark, nc); ALLOC(c, c->dflag, nc); ALLOC(c, c->dlist, nc);
-  ALLOC(c, c->kd, nc);
-  ALLOC(c, c->gfirst, nc); ALLOC(c, c->gcount, nc); ALLOC(c, c->gstat, 8 * nc);
-  ALLOC(c, c->sys, nc);
-  ALLOC(c, c->np2p, nc); ALLOC(c, c->nm2p, nc);
-  const int64_t mc = (int64_t)(c->cfg.node_factor * (double)nc) + 64;
+  if ((double)mc >= 4294967295.0)
+    return fail(c, PEPC_GPU_EINVAL, "node pool exceeds 32-bit node indices");
+  const int64_t nc = c->ncap;
   ALLOC(c, c->dkey[0], mc); ALLOC(c, c->dkey[1], mc);
   ALLOC(c, c->dval[0], mc); ALLOC(c, c->dval[1], mc);
   ALLOC(c, c->dpart, mc); ALLOC(c, c->inv, mc); ALLOC(c, c->flag, mc);
@@ -218,11 +209,37 @@ int ensure_particles(Ctx* c, int64_t n) {
   int64_t bc = scan_blocks(hc > mc ? hc : mc) + 64;
   if (bc < 512) bc = 512;   // also holds the 256 digit totals of a radix pass
   ALLOC(c, c->bsum, bc);
-  c->ncap = nc;
   c->mcap = mc;
+  c->grown = false;
+  c->walked = false;
+  c->frozen = false;
+  drop_graph(c);
+  return 0;
+}
+
+int ensure_particles(Ctx* c, int64_t n) {
+  if (n <= c->ncap) return 0;
+  const int64_t nc = n + n / 8 + 64;
+  int64_t mc = (int64_t)(c->cfg.node_factor * (double)nc) + 64;
+  if (c->cfg.node_factor * (double)nc + 64.0 >= 4294967295.0)
+    return fail(c, PEPC_GPU_EINVAL, "node pool of node_factor * n nodes exceeds 32-bit node indices");
+  if (mc < c->node_min) mc = c->node_min;
+  if (mc < nc) mc = nc;   // every particle is a leaf (and the grow borrows node arrays of n entries)
+  ALLOC(c, c->k0, nc); ALLOC(c, c->k1, nc); ALLOC(c, c->oldkeys, nc);
+  ALLOC(c, c->v0, nc); ALLOC(c, c->v1, nc);
+  ALLOC(c, c->xs, nc); ALLOC(c, c->ys, nc); ALLOC(c, c->zs, nc); ALLOC(c, c->qs, nc);
+  ALLOC(c, c->prec4, nc);
+  ALLOC(c, c->lo, nc); ALLOC(c, c->hi, nc);
+  ALLOC(c, c->cnt, nc); ALLOC(c, c->gmark, nc); ALLOC(c, c->dflag, nc); ALLOC(c, c->dlist, nc);
+  ALLOC(c, c->kd, nc);
+  ALLOC(c, c->gfirst, nc); ALLOC(c, c->gcount, nc); ALLOC(c, c->gstat, 8 * nc);
+  ALLOC(c, c->sys, nc);
+  ALLOC(c, c->np2p, nc); ALLOC(c, c->nm2p, nc);
+  c->ncap = nc;
+  int r = alloc_nodes(c, mc);
+  if (r) return r;
   c->io_cap = 0;        // staging and state arrays follow the particle capacity
   c->has_state = false;
-  drop_graph(c);
   return 0;
 }
 
@@ -325,8 +342,12 @@ int grow_device_impl(Ctx* c, int64_t n, const double* x, const double* y, const 
     launch_box(s, n, x, y, z, c->cfg.force_cubic_domain, c->partial, c->box, c->sc);
   launch_keys(s, n, x, y, z, q, c->box, sys, c->k0, c->v0, c->prec4);
   launch_sort_keys(s, c->k0, c->v0, c->k1, c->v1, n, c->hist, c->bsum, sys, nsys);
-  launch_gather_rec(s, n, c->v0, c->prec4, c->xs, c->ys, c->zs, c->qs);
   launch_dup_detect(s, n, c->k0, sys, c->sc);
+  // equal keys into the reference presort's order (the sort's ping-pong
+  // buffers and the build's node-sized index buffers are free here)
+  launch_tie_order(s, n, nsys, c->soff, c->k0, c->v0, c->k1, c->v1, c->dval[0],
+                   reinterpret_cast<uint16_t*>(c->dval[1]), c->sc);
+  launch_gather_rec(s, n, c->v0, c->prec4, c->xs, c->ys, c->zs, c->qs);
   launch_dupfix(s, n, c->k0, c->xs, c->ys, c->zs, c->box, sys, nsys > 1 ? c->soff : nullptr, c->kd,
                 c->oldkeys, c->dflag, c->dlist, c->bsum, c->sc);
   TreeBuffers t;
@@ -639,6 +660,15 @@ int pepc_gpu_sync(void* p, pepc_gpu_stats* st) {
   return sync_impl(c, st);
 }
 
+int pepc_gpu_reserve_nodes(void* p, int64_t nodes) {
+  Ctx* c = as_ctx(p);
+  if (!c || nodes < 0) return fail(c, PEPC_GPU_EINVAL, "reserve_nodes: bad argument");
+  HIPCHECK(c, hipSetDevice(c->cfg.device));
+  c->node_min = nodes;
+  if (c->ncap > 0 && nodes > c->mcap) return alloc_nodes(c, nodes);
+  return 0;
+}
+
 int pepc_gpu_grow(void* p, int64_t n, const double* x, const double* y, const double* z,
                   const double* q, uint64_t* keys_out, int64_t* perm_out, double* xs, double* ys,
                   double* zs) {
@@ -655,6 +685,16 @@ int pepc_gpu_grow(void* p, int64_t n, const double* x, const double* y, const do
   r = grow_device_impl(c, n, c->in[0], c->in[1], c->in[2], c->in[3], 1, nullptr);
   if (r) return r;
   r = sync_impl(c, nullptr);
+  if (r == PEPC_GPU_EOVERFLOW) {
+    // the build counted its nodes before it stopped: enlarge the pool to
+    // that count and grow again (the inputs are still staged)
+    const int64_t need = (int64_t)c->sc_host->nnodes;
+    r = alloc_nodes(c, need + need / 16 + 64);
+    if (r) return r;
+    r = grow_device_impl(c, n, c->in[0], c->in[1], c->in[2], c->in[3], 1, nullptr);
+    if (r) return r;
+    r = sync_impl(c, nullptr);
+  }
   if (r) return r;
   if (keys_out)
     HIPCHECK(c, hipMemcpy(keys_out, c->k0, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
@@ -857,6 +897,22 @@ int pepc_gpu_read_tree(void* p, int64_t cap, uint32_t* link, uint32_t* first, do
   if (link) HIPCHECK(c, hipMemcpy(link, c->link, sizeof(NodeLink) * m, hipMemcpyDeviceToHost));
   if (first) HIPCHECK(c, hipMemcpy(first, c->nfirst, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
   if (mom) HIPCHECK(c, hipMemcpy(mom, c->mom64, sizeof(double) * 16 * m, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pepc_gpu_read_box(void* p, int64_t sys, double* boxmin, double* boxmax, double* boxsize) {
+  Ctx* c = as_ctx(p);
+  if (!c || !c->grown) return fail(c, PEPC_GPU_EINVAL, "read_box: no tree");
+  if (sys < 0 || sys >= c->nsys) return fail(c, PEPC_GPU_EINVAL, "read_box: no such system");
+  int r = sync_impl(c, nullptr);
+  if (r) return r;
+  DevBox b;
+  HIPCHECK(c, hipMemcpy(&b, c->box + sys, sizeof(DevBox), hipMemcpyDeviceToHost));
+  for (int d = 0; d < 3; ++d) {
+    if (boxmin) boxmin[d] = b.boxmin[d];
+    if (boxmax) boxmax[d] = b.boxmax[d];
+    if (boxsize) boxsize[d] = b.boxsize[d];
+  }
   return 0;
 }
 

@@ -309,7 +309,240 @@ __global__ __launch_bounds__(256) void scatter_back_kernel(int64_t n, const uint
   out[perm[i]] = in[i];
 }
 
+// ---------------------------------------------------------------- tie order
+// Order of equal keys as the reference's local presort leaves it.
+//
+// sort_radix_ip / rs_rec_ip (sl_pepc sort_radix.c:33-151) is an in-place MSD
+// radix over 10-bit digits from bit 63 down.  A pass over a segment moves
+// every element whose digit class k differs from the class of the region it
+// sits in along permutation cycles; buckets above 256 elements recurse on
+// the next digit, smaller ones get a stable insertion sort (sort_insert.c:
+// 14-37).  Equal keys share every bucket, so their final order is the order
+// in which the passes over the buckets that contain them leave them.
+//
+// In one pass an element already in its class's region stays where it is;
+// the other slots of region k are filled, in increasing position, by the
+// class-k elements in the order the cycles carry them there; and the cycles
+// of the pass visit regions as a rotor walk: leaving region r always takes
+// r's next not-yet-taken foreign slot (positions in increasing order), and
+// the cycles started from region i (phase i, i = 0, 1, ...) end on their
+// first return to i.  The walk is replayed here literally.  It is serial by
+// nature (every step depends on the class read by the previous one), so
+// one thread walks while the block prepares classes and counts in parallel.
+//
+// Buckets without equal keys need nothing: for distinct keys the stable LSD
+// order is the only sorted order.  So the replay descends only into buckets
+// that hold a tie, and writes the final order of each such bucket into the
+// permutation.  Runs only when the sort left equal adjacent keys (dup_count
+// from dup_detect); one block per system.
+constexpr int kTieThreads = 256;
+constexpr int kTieWidth = 10;           // sl_tune_intern.h:22 (sort_radix_width_default)
+constexpr int kTieThreshold = 256;      // sl_tune_intern.h:30 (sort_radix_threshold_rec)
+constexpr int kTieClasses = 1 << kTieWidth;
+constexpr int kTieDepth = 8;            // 64 bits in 10-bit digits: at most 7 passes
+
+// Next foreign slot of region c (positions relative to the segment), or -1
+// when region c has none left.  rot[c] is the region's rotor.
+__device__ __forceinline__ int64_t tie_next(int c, uint32_t* rot, const uint32_t* reg,
+                                            const uint16_t* __restrict__ cls) {
+  uint32_t p = rot[c];
+  const uint32_t end = reg[c + 1];
+  while (p < end && cls[p] == (uint16_t)c) ++p;
+  if (p >= end) {
+    rot[c] = end;
+    return -1;
+  }
+  rot[c] = p + 1;
+  return (int64_t)p;
+}
+
+__global__ __launch_bounds__(kTieThreads) void tie_order_kernel(
+    int64_t n, const int64_t* __restrict__ soff, const unsigned long long* __restrict__ ks,
+    uint32_t* __restrict__ perm, unsigned long long* __restrict__ kin, uint32_t* __restrict__ ord,
+    uint32_t* __restrict__ ord2, uint16_t* __restrict__ cls, DevScalars* __restrict__ sc) {
+  if (sc->dup_count == 0) return;
+  const int tid = threadIdx.x;
+  const int64_t b = soff ? soff[blockIdx.x] : 0;
+  const int64_t e = soff ? soff[blockIdx.x + 1] : n;
+  if (e - b < 2) return;
+
+  __shared__ uint32_t reg[kTieClasses + 1];   // region starts of the current pass
+  __shared__ uint32_t rot[kTieClasses];       // rotors
+  __shared__ unsigned long long skey[kTieThreshold];
+  __shared__ int64_t fs[kTieDepth], fe[kTieDepth], fcur[kTieDepth];
+  __shared__ int frh[kTieDepth], fnext[kTieDepth];
+  __shared__ int depth, any;
+  __shared__ int64_t found;
+
+  if (tid == 0) any = 0;
+  __syncthreads();
+  for (int64_t i = b + tid; i + 1 < e; i += kTieThreads)
+    if (ks[i + 1] == ks[i]) any = 1;
+  __syncthreads();
+  if (!any) return;
+
+  // keys in input order (the sort's input of this system is [b, e) too) and
+  // the initial order of the top segment
+  for (int64_t i = b + tid; i < e; i += kTieThreads) {
+    kin[perm[i]] = ks[i];
+    ord[i] = (uint32_t)i;
+  }
+  if (tid == 0) {
+    depth = 1;
+    fs[0] = b;
+    fe[0] = e;
+    frh[0] = 63;
+    fnext[0] = -2;   // pass not run yet
+  }
+  __syncthreads();
+
+  while (depth > 0) {
+    const int d = depth - 1;
+    const int64_t s0 = fs[d], e0 = fe[d];
+    const int64_t m = e0 - s0;
+    if (fnext[d] == -2) {
+      // ---- one pass of rs_rec_ip over [s0, e0) on bits rhigh..sh
+      const int rhigh = frh[d];
+      const int cw = rhigh + 1 < kTieWidth ? rhigh + 1 : kTieWidth;
+      const int sh = rhigh - cw + 1;
+      const int K = 1 << cw;
+      const unsigned long long mask = (unsigned long long)(K - 1);
+      for (int c = tid; c <= K; c += kTieThreads) reg[c] = 0;
+      __syncthreads();
+      for (int64_t t = tid; t < m; t += kTieThreads) {
+        const uint32_t el = ord[s0 + t];
+        const int c = (int)((kin[el] >> sh) & mask);
+        cls[s0 + t] = (uint16_t)c;
+        ord2[s0 + t] = el;
+        atomicAdd(&reg[c + 1], 1u);
+      }
+      __syncthreads();
+      if (tid == 0) {
+        for (int c = 0; c < K; ++c) reg[c + 1] += reg[c];
+        for (int c = 0; c < K; ++c) rot[c] = reg[c];
+        const uint16_t* cl = cls + s0;
+        const uint32_t* o1 = ord + s0;
+        uint32_t* o2 = ord2 + s0;
+        for (int i = 0; i < K; ++i) {
+          for (;;) {
+            const int64_t p0 = tie_next(i, rot, reg, cl);   // the cycle's start slot
+            if (p0 < 0) break;
+            uint32_t el = o1[p0];
+            int c = cl[p0];
+            for (;;) {
+              const int64_t p = tie_next(c, rot, reg, cl);
+              if (p < 0) {   // cannot happen: every region has as many foreign slots as it sends
+                atomicOr(&sc->flags, kFlagDupKeys);
+                break;
+              }
+              o2[p] = el;
+              el = o1[p];
+              const int c2 = cl[p];
+              if (c2 == i) {
+                o2[p0] = el;
+                break;
+              }
+              c = c2;
+            }
+          }
+        }
+        fnext[d] = sh - 1;
+        fcur[d] = s0;
+      }
+      __syncthreads();
+      for (int64_t t = tid; t < m; t += kTieThreads) ord[s0 + t] = ord2[s0 + t];
+      __syncthreads();
+      if (fnext[d] < 0) {
+        // all 64 bits consumed: the segment holds one key, its order is final
+        for (int64_t t = tid; t < m; t += kTieThreads) perm[s0 + t] = ord[s0 + t];
+        __syncthreads();
+        if (tid == 0) depth = d;
+        __syncthreads();
+        continue;
+      }
+    }
+    // ---- next bucket of this pass that holds a tie
+    const int sh = fnext[d] + 1;   // low bit of this pass's digit
+    if (tid == 0) found = -1;
+    __syncthreads();
+    for (int64_t base = fcur[d]; base + 1 < e0; base += kTieThreads) {
+      const int64_t i = base + tid;
+      if (i + 1 < e0 && ks[i + 1] == ks[i]) atomicMin((unsigned long long*)&found, (unsigned long long)i);
+      __syncthreads();
+      if (found >= 0) break;
+      __syncthreads();
+    }
+    __syncthreads();
+    const int64_t t0 = found;
+    if (t0 < 0) {
+      if (tid == 0) depth = d;
+      __syncthreads();
+      continue;
+    }
+    // bucket = positions of [s0, e0) whose key shares ks[t0]'s bits above sh
+    __shared__ int64_t cs, ce;
+    if (tid == 0) {
+      const unsigned long long pre = ks[t0] >> sh;
+      int64_t lo = s0, hi = t0;   // first position with the prefix
+      while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if ((ks[mid] >> sh) < pre) lo = mid + 1; else hi = mid;
+      }
+      cs = lo;
+      lo = t0 + 1;
+      hi = e0;                    // first position past the prefix
+      while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if ((ks[mid] >> sh) <= pre) lo = mid + 1; else hi = mid;
+      }
+      ce = lo;
+      fcur[d] = ce;
+    }
+    __syncthreads();
+    const int64_t c0 = cs, c1 = ce, cm = c1 - c0;
+    if (cm > kTieThreshold) {
+      __syncthreads();
+      if (tid == 0) {
+        fs[d + 1] = c0;
+        fe[d + 1] = c1;
+        frh[d + 1] = fnext[d];
+        fnext[d + 1] = -2;
+        depth = d + 2;
+      }
+      __syncthreads();
+      continue;
+    }
+    // stable insertion sort of a small bucket (sort_insert.c:14-37): rank
+    // by key, equal keys in their current order
+    uint32_t el = 0;
+    unsigned long long k = 0;
+    if (tid < cm) {
+      el = ord[c0 + tid];
+      k = kin[el];
+      skey[tid] = k;
+    }
+    __syncthreads();
+    if (tid < cm) {
+      int r = 0;
+      for (int u = 0; u < (int)cm; ++u) {
+        const unsigned long long ku = skey[u];
+        r += (ku < k || (ku == k && u < tid)) ? 1 : 0;
+      }
+      perm[c0 + r] = el;
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
+
+void launch_tie_order(hipStream_t s, int64_t n, int64_t nsys, const int64_t* soff,
+                      const unsigned long long* ks, uint32_t* perm, unsigned long long* kin,
+                      uint32_t* ord, uint32_t* ord2, uint16_t* cls, DevScalars* sc) {
+  if (n < 2) return;
+  tie_order_kernel<<<(unsigned)(nsys > 1 ? nsys : 1), kTieThreads, 0, s>>>(
+      n, nsys > 1 ? soff : nullptr, ks, perm, kin, ord, ord2, cls, sc);
+}
 
 int64_t scan_blocks(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
 int64_t sort_blocks(int64_t n) { return (n + kSortTile - 1) / kSortTile; }

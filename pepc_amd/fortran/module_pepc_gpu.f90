@@ -22,6 +22,7 @@ module module_pepc_kinds
    integer, parameter :: kind_key = c_int64_t
    integer, parameter :: kind_node = c_int64_t
    integer, parameter :: kind_default = c_int32_t
+   integer, parameter :: kind_pe = kind_default
    integer, parameter :: kind_dim = 4
 end module module_pepc_kinds
 
@@ -69,14 +70,100 @@ module module_pepc_types
       type(t_particle_results) :: results
    end type t_particle
 
-   !> The host-side tree of the reference (module_tree.f90).  The tree of this
-   !> build lives on the device, so the host record is never allocated; the
-   !> frontends' tree output sees it as such and skips (as the reference does
-   !> for an unallocated tree).
+   !> module_box.f90:33-38
+   type t_box
+      real(kind_physics) :: boxmin(3) = 0.0_kind_physics
+      real(kind_physics) :: boxmax(3) = 0.0_kind_physics
+      real(kind_physics) :: boxsize(3) = 0.0_kind_physics
+   end type t_box
+
+   !> The host-side record of the reference's tree (module_tree.f90:80-106).
+   !> The nodes of this build live on the device: after a grow the record
+   !> holds the tree's counts and bounding box and is marked allocated until
+   !> pepc_timber_tree; it has no node array (the tree writers read the nodes
+   !> from the device).
    type t_tree
       logical :: allocated = .false.
+      integer(kind_particle) :: npart = 0, npart_me = 0
+      integer(kind_node) :: nleaf = 0, ntwig = 0, nleaf_me = 0, ntwig_me = 0
+      integer(kind_node) :: nbranch = 0, nbranch_me = 0
+      type(t_box) :: bounding_box
    end type t_tree
 end module module_pepc_types
+
+!> Key algebra the frontends and the tree writers use
+!> (module_spacefilling.f90:47-57, 445-565): level of a key and the inverse
+!> Hilbert map to the cell's integer and real coordinates.
+module module_spacefilling
+   use module_pepc_kinds
+   use module_pepc_types, only: t_box
+   implicit none
+   private
+   public :: level_from_key, key_to_coord, key_to_intcoord_hilbert
+
+   integer, parameter :: maxlevel = 20
+   ! state tables of the inverse map: child value -> cell bits, exchange and
+   ! reverse codes (5: swap x and z, 6: swap y and z)
+   integer, parameter :: cval_of(0:7) = [0, 1, 3, 2, 6, 7, 5, 4]
+   integer, parameter :: exch_of(0:7) = [5, 6, 0, 5, 5, 0, 6, 5]
+   integer, parameter :: rev_of(0:7) = [0, 0, 0, 5, 0, 0, 6, 5]
+
+contains
+
+   !> position of the placeholder bit / 3
+   integer function level_from_key(key)
+      integer(kind_key), intent(in) :: key
+      level_from_key = (63 - leadz(key)) / 3
+   end function level_from_key
+
+   !> integer coordinates (finest-cell units) of the lower corner of the
+   !> key's cell
+   subroutine key_to_intcoord_hilbert(key, ic)
+      integer(kind_key), intent(in) :: key
+      integer(kind_key), intent(out) :: ic(3)
+      integer :: lev, i, j, k, h, cv, rv, ex
+      integer(kind_key) :: t
+      lev = level_from_key(key)
+      ic = 0
+      do i = 0, lev - 1
+         h = int(iand(ishft(key, -3 * i), 7_kind_key))
+         cv = cval_of(h)
+         rv = rev_of(h)
+         ex = exch_of(h)
+         if (i > 0) then
+            do j = 0, 2
+               if (btest(rv, j)) ic(j + 1) = iand(not(ic(j + 1)), ishft(1_kind_key, i) - 1)
+            end do
+            do j = 0, 1
+               if (.not. btest(ex, j)) cycle
+               do k = j + 1, 2
+                  if (btest(ex, k)) then
+                     t = ic(j + 1)
+                     ic(j + 1) = ic(k + 1)
+                     ic(k + 1) = t
+                  end if
+               end do
+            end do
+         end if
+         do j = 0, 2
+            if (btest(cv, j)) ic(j + 1) = ior(ic(j + 1), ishft(1_kind_key, i))
+         end do
+      end do
+      ic = ishft(ic, maxlevel - lev)
+   end subroutine key_to_intcoord_hilbert
+
+   !> module_spacefilling.f90:445-471: ((ic + 1/2) s + boxmin, s = boxsize / 2^20
+   subroutine key_to_coord(b, key, x)
+      type(t_box), intent(in) :: b
+      integer(kind_key), intent(in) :: key
+      real(kind_physics), intent(inout) :: x(3)
+      integer(kind_key) :: ic(3)
+      real(kind_physics) :: s(3)
+      call key_to_intcoord_hilbert(key, ic)
+      s = b%boxsize / 2_kind_key**maxlevel
+      x = (real(ic, kind_physics) + 0.5_kind_physics) * s + b%boxmin
+   end subroutine key_to_coord
+end module module_spacefilling
 
 !> iso_c_binding interface of include/pepc_gpu.h
 module pepc_gpu_c
@@ -126,6 +213,12 @@ module pepc_gpu_c
          integer(c_size_t), value :: len
       end function
 
+      integer(c_int) function pepc_gpu_reserve_nodes(ctx, nodes) bind(C, name="pepc_gpu_reserve_nodes")
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: nodes
+      end function
+
       integer(c_int) function pepc_gpu_grow(ctx, n, x, y, z, q, keys, perm, xs, ys, zs) &
          bind(C, name="pepc_gpu_grow")
          import :: c_int, c_ptr, c_int64_t, c_double
@@ -141,6 +234,29 @@ module pepc_gpu_c
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: ctx
          real(c_double), intent(out) :: ex(*), ey(*), ez(*), pot(*), work(*)
+      end function
+
+      integer(c_int) function pepc_gpu_read_tree(ctx, cap, link, first, mom, nnodes) &
+         bind(C, name="pepc_gpu_read_tree")
+         import :: c_int, c_ptr, c_int64_t
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: cap
+         type(c_ptr), value :: link, first, mom
+         integer(c_int64_t), intent(out) :: nnodes
+      end function
+
+      integer(c_int) function pepc_gpu_read_sorted(ctx, keys, perm, x, y, z, q) &
+         bind(C, name="pepc_gpu_read_sorted")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, keys, perm, x, y, z, q
+      end function
+
+      integer(c_int) function pepc_gpu_read_box(ctx, sys, boxmin, boxmax, boxsize) &
+         bind(C, name="pepc_gpu_read_box")
+         import :: c_int, c_ptr, c_int64_t, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: sys
+         real(c_double), intent(out) :: boxmin(3), boxmax(3), boxsize(3)
       end function
 
       integer(c_int) function pepc_gpu_sync(ctx, st) bind(C, name="pepc_gpu_sync")
@@ -428,12 +544,14 @@ module module_pepc
 
    public :: pepc_initialize, pepc_finalize, pepc_prepare
    public :: pepc_read_parameters, pepc_read_parameters_from_file_name, pepc_write_parameters
-   public :: pepc_read_parameters_from_first_argument
+   public :: pepc_read_parameters_from_first_argument, pepc_get_para_file
    public :: pepc_particleresults_clear, pepc_grow_tree, pepc_traverse_tree
    public :: pepc_timber_tree, pepc_restore_particles, pepc_grow_and_traverse
    public :: pepc_grow_and_traverse_for_others, pepc_check_sanity
    public :: pepc_statistics, pepc_last_stats, pepc_last_timings, pepc_direct_force
-   !> module_pepc.f90's public tree; never allocated on the host here.
+   public :: pepc_device_tree
+   !> module_pepc.f90's public tree: counts and box of the grown tree
+   !> (its nodes stay on the device).
    type(t_tree), public, target :: global_tree
 
    ! &calc_force_coulomb lives in module_interaction_specific, the periodic
@@ -451,8 +569,9 @@ module module_pepc
    real(kind_physics), public :: interaction_sign = 1.0_kind_physics
    logical, public :: print_stats = .false.   ! also: debug_level with DBG_STATS
    ! &libpepc entries read for compatibility: curve_type must be 1 (Hilbert,
-   ! as the reference requires), idim 3; the load-balancing weights, list
-   ! length factor and thread count have no role on one GPU
+   ! as the reference requires), idim 3; np_mult and the list length factor
+   ! size the node pool as tree_create does (module_tree.f90:177-191); the
+   ! load-balancing weights and the thread count have no role on one GPU
    integer :: curve_type = 1, weighted = 1, interaction_list_length_factor = 1, num_threads = 3
    integer :: idim = 3
 
@@ -564,6 +683,51 @@ contains
       end if
    end subroutine pepc_read_parameters_from_first_argument
 
+   !> module_pepc.f90:333-367: the first command-line argument names the
+   !> parameter file (rank 0 reads it; one rank here, so nothing to broadcast)
+   subroutine pepc_get_para_file(available, file_name, my_rank, comm)
+      logical, intent(out) :: available
+      character(len=255), intent(out) :: file_name
+      integer(kind_pe), intent(in) :: my_rank
+      integer, intent(in), optional :: comm
+      available = .false.
+      file_name = ' '
+      if (my_rank == 0) then
+         if (command_argument_count() /= 0) then
+            call get_command_argument(1, file_name)
+            available = .true.
+            write (*, *) "found parameter file: ", file_name
+         end if
+      end if
+      if (present(comm)) continue
+   end subroutine pepc_get_para_file
+
+   !> Node table size of tree_create (module_tree.f90:172-191) for n particles:
+   !> max(30 nintmax + 4 n, 10^4) for np_mult > 0, else |np_mult| 10^4, with
+   !> nintmax the list factor times the Hernquist estimate of
+   !> get_number_of_interactions_per_particle
+   !> (coulomb/module_interaction_specific.f90:231-243).  Stops, as the
+   !> reference does, when the table could not hold one node per particle;
+   !> returns the table's entry count (at least 2^15).
+   integer(c_int64_t) function reference_node_table(n) result(entries)
+      integer(c_int64_t), intent(in) :: n
+      integer(c_int64_t) :: nintmax, maxaddress
+      real(kind_physics) :: inv
+      inv = max(theta2 / (35.0_8 * log(1.0_8 * real(max(n, 1_c_int64_t), 8))), 1.0_8 / real(max(n, 1_c_int64_t), 8))
+      nintmax = int(1.0_8 / inv, c_int64_t) * interaction_list_length_factor
+      if (np_mult > 0) then
+         maxaddress = max(30_c_int64_t * nintmax + 4_c_int64_t * n, 10000_c_int64_t)
+      else
+         maxaddress = int(abs(real(np_mult, 8)) * 10000.0_8, c_int64_t)
+      end if
+      if (maxaddress <= n) then
+         write (*, '(a,i0,a,i0,a)') ' PEPC-GPU: maxaddress = ', maxaddress, ' .le. npart = ', n, &
+            '. You should increase np_mult.'
+         error stop 1
+      end if
+      entries = max(maxaddress, 32768_c_int64_t)
+   end function reference_node_table
+
    !> module_pepc.f90:269-288: creates the GPU context with the parameters read
    subroutine pepc_prepare(idim)
       integer(kind_dim), optional, intent(in) :: idim
@@ -641,6 +805,7 @@ contains
          z(i) = particles(i)%x(3)
          q(i) = particles(i)%data%q
       end do
+      call check(pepc_gpu_reserve_nodes(ctx, reference_node_table(n)), 'pepc_grow_tree')
       call check(pepc_gpu_grow(ctx, n, x, y, z, q, keys, last_perm, xs, ys, zs), 'pepc_grow_tree')
       call move_alloc(particles, tmp)
       allocate (particles(n))
@@ -654,7 +819,61 @@ contains
       grown_addr = c_null_ptr
       if (n > 0) grown_addr = c_loc(particles(1))
       deallocate (tmp, x, y, z, q, xs, ys, zs, keys)
+      call fill_global_tree(n)
    end subroutine pepc_grow_tree
+
+   !> The host record of the grown tree: counts (tree_stats,
+   !> module_tree.f90:622-699), the branch nodes of one rank (the root's
+   !> children: find_local_branches, module_tree_grow.f90:283-363, with the
+   !> whole key range as the rank's domain) and the bounding box.
+   subroutine fill_global_tree(n)
+      integer(c_int64_t), intent(in) :: n
+      type(pepc_gpu_stats) :: st
+      call check(pepc_gpu_sync(ctx, st), 'pepc_grow_tree')
+      global_tree%npart = n
+      global_tree%npart_me = n
+      global_tree%nleaf = st%nleaf
+      global_tree%nleaf_me = st%nleaf
+      global_tree%ntwig = st%ntwig
+      global_tree%ntwig_me = st%ntwig
+      global_tree%nbranch = int(st%level_offset(3), kind_node) - int(st%level_offset(2), kind_node)
+      global_tree%nbranch_me = global_tree%nbranch
+      call check(pepc_gpu_read_box(ctx, 0_c_int64_t, global_tree%bounding_box%boxmin, &
+                                   global_tree%bounding_box%boxmax, global_tree%bounding_box%boxsize), &
+                 'pepc_grow_tree')
+      global_tree%allocated = .true.
+   end subroutine fill_global_tree
+
+   !> The device tree of the last grow on the host, for the tree writers (the
+   !> reference's writers walk its host nodes): per node i (BFS order, a
+   !> twig's children contiguous) link(:, i) = {first child (twig) or
+   !> particle (leaf), number of children (0 for a leaf), particles below,
+   !> level}, all 0-based; first(i) its first particle; mom(:, i) the 16
+   !> moments in the reference layout {coc(3), charge, abs_charge, dip(3),
+   !> quad(3), xy, yz, zx, bmax, 0}; nkey(i) the node's key (its first
+   !> particle's key cut to the node's level).
+   subroutine pepc_device_tree(link, first, mom, nkey)
+      integer(c_int32_t), allocatable, target, intent(out) :: link(:, :), first(:)
+      real(c_double), allocatable, target, intent(out) :: mom(:, :)
+      integer(kind_key), allocatable, intent(out) :: nkey(:)
+      integer(c_int64_t), allocatable, target :: keys(:)
+      integer(c_int64_t) :: nn, i, m
+      if (.not. global_tree%allocated) then
+         write (*, *) 'PEPC-GPU: pepc_device_tree without a grown tree'
+         error stop 1
+      end if
+      nn = global_tree%nleaf + global_tree%ntwig
+      m = global_tree%npart
+      allocate (link(4, nn), first(nn), mom(16, nn), nkey(nn), keys(m))
+      call check(pepc_gpu_read_tree(ctx, nn, c_loc(link), c_loc(first), c_loc(mom), i), &
+                 'pepc_device_tree')
+      call check(pepc_gpu_read_sorted(ctx, c_loc(keys), c_null_ptr, c_null_ptr, c_null_ptr, &
+                                      c_null_ptr, c_null_ptr), 'pepc_device_tree')
+      do i = 1, nn
+         nkey(i) = ishft(keys(first(i) + 1), -3 * (20 - link(4, i)))
+      end do
+      deallocate (keys)
+   end subroutine pepc_device_tree
 
    !> module_pepc.f90:459-468.  The grown particles (in their grown order) take
    !> the grown groups; any other array is walked particle by particle, as the
@@ -689,8 +908,11 @@ contains
       do i = 1, n
          particles(i)%results%e = particles(i)%results%e + [ex(i), ey(i), ez(i)]
          particles(i)%results%pot = particles(i)%results%pot + pot(i)
+         ! the walk adds one per leaf and accepted twig to %work
+         ! (module_tree_walk_simple.f90:335, :380); the grown particles' work
+         ! from the library starts at the grow's 1
          if (grown) then
-            particles(i)%work = work(i)
+            particles(i)%work = particles(i)%work + (work(i) - 1.0_8)
          else
             particles(i)%work = particles(i)%work + work(i)
          end if
@@ -715,6 +937,7 @@ contains
    !> module_pepc.f90:538
    subroutine pepc_timber_tree()
       if (c_associated(ctx)) call check(pepc_gpu_timber(ctx), 'pepc_timber_tree')
+      global_tree%allocated = .false.
    end subroutine pepc_timber_tree
 
    !> module_pepc.f90:525 -> domain_restore, module_domains.f90:372-408
@@ -878,7 +1101,6 @@ module module_checkpoint
    integer, parameter :: filehandle = 91
    character(12), parameter :: directory = './particles'
    integer(c_int64_t), parameter :: max_header_size = 128
-   integer, parameter :: kind_pe = kind_default
 
    public write_particles_ascii, write_particles_binary, write_particles_mpiio
    public read_particles_mpiio, read_particles_mpiio_from_filename

@@ -16,6 +16,7 @@ module module_vtk
    integer, public, parameter :: VTK_STEP_LAST = 1      !< closes the time series
    integer, public, parameter :: VTK_VERTEX = 1
    integer, public, parameter :: VTK_POLY_LINE = 4
+   integer, public, parameter :: VTK_VOXEL = 11
 
    character(*), parameter :: dir_series = "./"
    character(*), parameter :: dir_pieces = "./vtk/"
@@ -439,16 +440,19 @@ contains
    end subroutine da_int4_repeat
 end module module_vtk
 
-!> Particle output of the frontends (utils/module_vtk_helpers.f90:49-113,
+!> Particle and tree output of the frontends (utils/module_vtk_helpers.f90:49-113, 233-723,
 !> coulomb/module_interaction_specific_types_XYZQ.f90:63-75).
 module module_vtk_helpers
+   use iso_c_binding
    use module_vtk
    use module_pepc_types
    use module_interaction_specific_types
+   use module_spacefilling, only: key_to_coord
+   use module_pepc, only: pepc_device_tree
    implicit none
    private
    public :: vtk_write_particles, vtk_write_particle_data_results, vtk_write_spacecurve
-   public :: vtk_write_branches, vtk_write_leaves
+   public :: vtk_write_branches, vtk_write_leaves, vtk_write_node_interaction_data
 
 contains
 
@@ -537,32 +541,9 @@ contains
       call vtk%close()
    end subroutine vtk_write_spacecurve
 
-   !> Branch and leaf nodes as boxes (utils/module_vtk_helpers.f90:553-700)
-   !> read the host tree.  This build keeps the tree on the device and never
-   !> allocates the host record, so these take the reference's path for an
-   !> unallocated tree: a message, no output.
-   subroutine vtk_write_branches(step, tsim, vtk_step, t, helper_func, coord_scale)
-      integer, intent(in) :: step
-      integer, intent(in) :: vtk_step
-      real*8, intent(in) :: tsim
-      type(t_tree), intent(in) :: t
-      interface
-         subroutine helper_func(d, vtkf)
-            use module_interaction_specific_types, only: t_tree_node_interaction_data
-            use module_vtk, only: vtkfile_unstructured_grid
-            implicit none
-            type(t_tree_node_interaction_data), intent(in) :: d(:)
-            type(vtkfile_unstructured_grid), intent(inout) :: vtkf
-         end subroutine helper_func
-      end interface
-      optional :: helper_func
-      real(kind_physics), intent(in), optional :: coord_scale
-      if (.not. t%allocated) then
-         write (*, *) 'vtk_write_branches(): tree is not allocated, aborting branch output.'
-         return
-      end if
-   end subroutine vtk_write_branches
-
+   !> Leaves of the grown tree as boxes (utils/module_vtk_helpers.f90:553-620):
+   !> one voxel per leaf, in key order (the order of the reference's
+   !> depth-first collection), from the device tree.
    subroutine vtk_write_leaves(step, tsim, vtk_step, t, helper_func, coord_scale)
       integer, intent(in) :: step
       integer, intent(in) :: vtk_step
@@ -579,11 +560,190 @@ contains
       end interface
       optional :: helper_func
       real(kind_physics), intent(in), optional :: coord_scale
+      integer(c_int32_t), allocatable :: link(:, :), first(:)
+      real(c_double), allocatable :: mom(:, :)
+      integer(kind_key), allocatable :: nkey(:)
+      integer(kind_node), allocatable :: sel(:)
+      integer(kind_node) :: i
       if (.not. t%allocated) then
          write (*, *) 'vtk_write_leaves(): tree is not allocated, aborting leaves output.'
          return
       end if
+      call pepc_device_tree(link, first, mom, nkey)
+      allocate (sel(t%nleaf_me))
+      do i = 1, size(first, kind=kind_node)
+         if (link(2, i) == 0) sel(first(i) + 1) = i     ! one leaf per particle, key order
+      end do
+      call write_nodes_as_boxes("leaves", step, tsim, vtk_step, t, link, mom, nkey, sel, &
+                                helper_func, coord_scale)
    end subroutine vtk_write_leaves
+
+   !> Branch nodes of the grown tree as boxes (utils/module_vtk_helpers.f90:
+   !> 625-700).  On one rank the branches are the root's children
+   !> (find_local_branches, module_tree_grow.f90:283-363, with the whole key
+   !> range as the rank's domain), in key order.
+   subroutine vtk_write_branches(step, tsim, vtk_step, t, helper_func, coord_scale)
+      integer, intent(in) :: step
+      integer, intent(in) :: vtk_step
+      real*8, intent(in) :: tsim
+      type(t_tree), intent(in) :: t
+      interface
+         subroutine helper_func(d, vtkf)
+            use module_interaction_specific_types, only: t_tree_node_interaction_data
+            use module_vtk, only: vtkfile_unstructured_grid
+            implicit none
+            type(t_tree_node_interaction_data), intent(in) :: d(:)
+            type(vtkfile_unstructured_grid), intent(inout) :: vtkf
+         end subroutine helper_func
+      end interface
+      optional :: helper_func
+      real(kind_physics), intent(in), optional :: coord_scale
+      integer(c_int32_t), allocatable :: link(:, :), first(:)
+      real(c_double), allocatable :: mom(:, :)
+      integer(kind_key), allocatable :: nkey(:)
+      integer(kind_node), allocatable :: sel(:)
+      integer(kind_node) :: i, j, nb, tmp
+      if (.not. t%allocated) then
+         write (*, *) 'vtk_write_branches(): tree is not allocated, aborting branch output.'
+         return
+      end if
+      call pepc_device_tree(link, first, mom, nkey)
+      allocate (sel(t%nbranch))
+      nb = 0
+      do i = 1, size(first, kind=kind_node)
+         if (link(4, i) == 1) then
+            nb = nb + 1
+            sel(nb) = i
+         end if
+      end do
+      do i = 2, nb                                     ! key order (at most eight)
+         j = i
+         do while (j > 1)
+            if (nkey(sel(j - 1)) <= nkey(sel(j))) exit
+            tmp = sel(j); sel(j) = sel(j - 1); sel(j - 1) = tmp
+            j = j - 1
+         end do
+      end do
+      call write_nodes_as_boxes("branches", step, tsim, vtk_step, t, link, mom, nkey, sel(1:nb), &
+                                helper_func, coord_scale)
+   end subroutine vtk_write_branches
+
+   !> vtk_write_nodes_as_boxes (utils/module_vtk_helpers.f90:233-373) for the
+   !> nodes sel of the device tree: eight corners per node (the cell's
+   !> key_to_coord point shifted by the node's box size along the voxel's
+   !> axes), voxel cells, and per node the rank, level, leaves, descendants
+   !> and the interaction data (helper_func, or charge and abs_charge).
+   subroutine write_nodes_as_boxes(fname, step, tsim, vtk_step, t, link, mom, nkey, sel, &
+                                   helper_func, coord_scale)
+      character(*), intent(in) :: fname
+      integer, intent(in) :: step, vtk_step
+      real*8, intent(in) :: tsim
+      type(t_tree), intent(in) :: t
+      integer(c_int32_t), intent(in) :: link(:, :)
+      real(c_double), intent(in) :: mom(:, :)
+      integer(kind_key), intent(in) :: nkey(:)
+      integer(kind_node), intent(in) :: sel(:)
+      interface
+         subroutine helper_func(d, vtkf)
+            use module_interaction_specific_types, only: t_tree_node_interaction_data
+            use module_vtk, only: vtkfile_unstructured_grid
+            implicit none
+            type(t_tree_node_interaction_data), intent(in) :: d(:)
+            type(vtkfile_unstructured_grid), intent(inout) :: vtkf
+         end subroutine helper_func
+      end interface
+      optional :: helper_func
+      real(kind_physics), intent(in), optional :: coord_scale
+      ! corner j of a voxel: offsets (x, y, z) in units of the box size
+      real, parameter :: box_shift(3, 8) = reshape([0., 0., 0., 0., 0., 1., 0., 1., 0., 0., 1., 1., &
+                                                    1., 0., 0., 1., 0., 1., 1., 1., 0., 1., 1., 1.], [3, 8])
+      real(kind_physics), allocatable :: cx(:), cy(:), cz(:)
+      integer(kind_node), allocatable :: cidx(:), coff(:), desc(:), leaves(:)
+      integer(c_int32_t), allocatable :: level(:), owner(:)
+      type(t_tree_node_interaction_data), allocatable :: d(:)
+      type(vtkfile_unstructured_grid) :: vtk
+      real(kind_physics) :: bx(3), bsize(3), bshift(3)
+      integer(kind_node) :: i, nn, nsel, k, c
+      integer :: j
+
+      ! descendants of every node: children plus their descendants
+      ! (tree_node_update_from_children, module_tree_grow.f90:791, 817);
+      ! a twig's children follow it in the device order
+      nn = size(link, 2, kind=kind_node)
+      allocate (desc(nn))
+      desc = 0
+      do i = nn, 1, -1
+         if (link(2, i) > 0) then
+            do k = 0, link(2, i) - 1
+               c = int(link(1, i), kind_node) + k + 1
+               desc(i) = desc(i) + 1 + desc(c)
+            end do
+         end if
+      end do
+
+      nsel = size(sel, kind=kind_node)
+      allocate (cx(8 * nsel), cy(8 * nsel), cz(8 * nsel), cidx(8 * nsel), coff(nsel), &
+                leaves(nsel), level(nsel), owner(nsel), d(nsel))
+      do i = 1, nsel
+         k = sel(i)
+         level(i) = link(4, k)
+         leaves(i) = link(3, k)
+         owner(i) = 0
+         d(i)%coc = mom(1:3, k)
+         d(i)%charge = mom(4, k)
+         d(i)%abs_charge = mom(5, k)
+         d(i)%dip = mom(6:8, k)
+         d(i)%quad = mom(9:11, k)
+         d(i)%xyquad = mom(12, k)
+         d(i)%yzquad = mom(13, k)
+         d(i)%zxquad = mom(14, k)
+         d(i)%bmax = mom(15, k)
+         bsize = t%bounding_box%boxsize / 2**level(i)
+         coff(i) = 8 * i
+         call key_to_coord(t%bounding_box, nkey(k), bx)
+         do j = 1, 8
+            cidx(8 * (i - 1) + j) = 8 * (i - 1) + j - 1
+            bshift = box_shift(:, j) * bsize
+            cx(8 * (i - 1) + j) = bx(1) + bshift(1)
+            cy(8 * (i - 1) + j) = bx(2) + bshift(2)
+            cz(8 * (i - 1) + j) = bx(3) + bshift(3)
+         end do
+      end do
+
+      call vtk%create_parallel(fname, step, 0, 1, tsim, vtk_step)
+      call vtk%write_headers(8 * nsel, nsel)
+      call vtk%startpoints()
+      call vtk%write_data_array("corners", cx, cy, cz, coord_scale)
+      call vtk%finishpoints()
+      call vtk%startpointdata()
+      call vtk%finishpointdata()
+      call vtk%startcells()
+      call vtk%write_data_array("connectivity", cidx)
+      call vtk%write_data_array("offsets", coff)
+      call vtk%write_data_array("types", int(nsel, c_int32_t), int(VTK_VOXEL, c_int32_t))
+      call vtk%finishcells()
+      call vtk%startcelldata()
+      call vtk%write_data_array("processor", owner)
+      call vtk%write_data_array("level", level)
+      call vtk%write_data_array("leaves", leaves)
+      call vtk%write_data_array("descendants", [(desc(sel(i)), i=1, nsel)])
+      if (present(helper_func)) then
+         call helper_func(d, vtk)
+      else
+         call vtk_write_node_interaction_data(d, vtk)
+      end if
+      call vtk%finishcelldata()
+      call vtk%write_final()
+      call vtk%close()
+   end subroutine write_nodes_as_boxes
+
+   !> coulomb/module_interaction_specific_types_XYZQ.f90:80-89
+   subroutine vtk_write_node_interaction_data(d, vtkf)
+      type(t_tree_node_interaction_data), intent(in) :: d(:)
+      type(vtkfile_unstructured_grid), intent(inout) :: vtkf
+      call vtkf%write_data_array("charge", d(:)%charge)
+      call vtkf%write_data_array("abs_charge", d(:)%abs_charge)
+   end subroutine vtk_write_node_interaction_data
 
    subroutine vtk_write_particle_data_results(d, r, vtkf)
       type(t_particle_data), intent(in) :: d(:)

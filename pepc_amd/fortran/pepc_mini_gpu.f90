@@ -30,6 +30,24 @@
 !                                             ! plus the space curve and the tree
 !                                             ! writers (which skip: no host tree)
 !                                             ! (no GPU)
+!   pepc_mini_gpu --twice params in.bin out.bin
+!                                             ! the parameter file through
+!                                             ! pepc_get_para_file (argument 1 is
+!                                             ! the flag, so the file is read by
+!                                             ! name), one grow, two traversals of
+!                                             ! the grown array: out.bin holds
+!                                             ! int64 n, label(n), float64
+!                                             ! x,y,z,ex,ey,ez,pot(n) after both,
+!                                             ! work(n) after the first and after
+!                                             ! the second traversal
+!   pepc_mini_gpu --solve-output params in.bin out.bin
+!                                             ! grow + traverse, then the outputs
+!                                             ! of a solved step 3: MPI-IO
+!                                             ! checkpoint (read back into out.bin:
+!                                             ! int64 n, itime, then the 96-byte
+!                                             ! records), branches and leaves as
+!                                             ! VTK boxes; after pepc_timber_tree
+!                                             ! the leaves writer skips (step 4)
 !   pepc_mini_gpu --library params in.bin out.bin
 !                                             ! the library modules without a GPU:
 !                                             ! read params, wrap the particles of
@@ -100,6 +118,14 @@ program pepc_mini_gpu
       end if
       if (trim(fin) == '--direct') then
          call direct_all()
+         stop
+      end if
+      if (trim(fin) == '--solve-output') then
+         call solve_output()
+         stop
+      end if
+      if (trim(fin) == '--twice') then
+         call traverse_twice()
          stop
       end if
       if (trim(fin) == '--library') then
@@ -240,6 +266,91 @@ contains
       write (12) back
       close (12)
    end subroutine checkpoint_roundtrip
+
+   subroutine solve_output()
+      type(t_particle), allocatable :: back(:)
+      character(len=255) :: f1, f3
+      integer(kind_default) :: itime
+      integer(kind_particle) :: ntot
+      call pepc_initialize("pepc-mini-gpu", my_rank, n_ranks, .true.)
+      call get_command_argument(2, fin)
+      call pepc_read_parameters_from_file_name(trim(fin))
+      call pepc_prepare(3_kind_dim)
+      call get_command_argument(3, fin)
+      call get_command_argument(4, fout)
+      open (11, file=trim(fin), access='stream', form='unformatted', status='old')
+      read (11) n
+      allocate (particles(n), a(n))
+      read (11) a; particles(:)%x(1) = a
+      read (11) a; particles(:)%x(2) = a
+      read (11) a; particles(:)%x(3) = a
+      read (11) a; particles(:)%data%q = a
+      close (11)
+      do i = 1, n
+         particles(i)%label = i - 1
+         particles(i)%work = 1.0_8
+      end do
+      call pepc_particleresults_clear(particles)
+      call pepc_grow_tree(particles)
+      call pepc_traverse_tree(particles)
+      call write_particles_mpiio(0, 3, int(n, 8), particles, f1)
+      call read_particles_mpiio(3, 0, itime, ntot, back, f3, noparams=.true.)
+      call vtk_write_branches(3, 0.25_8, VTK_STEP_FIRST, global_tree)
+      call vtk_write_leaves(3, 0.25_8, VTK_STEP_FIRST, global_tree)
+      call pepc_timber_tree()
+      call vtk_write_leaves(4, 0.5_8, VTK_STEP_LAST, global_tree)
+      open (12, file=trim(fout), access='stream', form='unformatted', status='replace')
+      write (12) ntot, int(itime, 8)
+      write (12) back
+      close (12)
+      call pepc_finalize()
+   end subroutine solve_output
+
+   subroutine traverse_twice()
+      logical :: avail
+      character(len=255) :: pf
+      real(8), allocatable :: w1(:)
+      call pepc_initialize("pepc-mini-gpu", my_rank, n_ranks, .true.)
+      call pepc_get_para_file(avail, pf, my_rank)
+      if (.not. avail .or. trim(pf) /= '--twice') stop 2
+      call get_command_argument(2, pf)
+      call pepc_read_parameters_from_file_name(trim(pf))
+      call pepc_prepare(3_kind_dim)
+      call get_command_argument(3, fin)
+      call get_command_argument(4, fout)
+      open (11, file=trim(fin), access='stream', form='unformatted', status='old')
+      read (11) n
+      allocate (particles(n), a(n), w1(n))
+      read (11) a; particles(:)%x(1) = a
+      read (11) a; particles(:)%x(2) = a
+      read (11) a; particles(:)%x(3) = a
+      read (11) a; particles(:)%data%q = a
+      close (11)
+      do i = 1, n
+         particles(i)%label = i - 1
+         particles(i)%work = 7.0_8          ! grow resets it to 1
+      end do
+      call pepc_particleresults_clear(particles)
+      call pepc_grow_tree(particles)
+      call pepc_traverse_tree(particles)
+      w1 = particles(:)%work
+      call pepc_traverse_tree(particles)
+      open (12, file=trim(fout), access='stream', form='unformatted', status='replace')
+      write (12) n
+      write (12) particles(:)%label
+      write (12) particles(:)%x(1)
+      write (12) particles(:)%x(2)
+      write (12) particles(:)%x(3)
+      write (12) particles(:)%results%e(1)
+      write (12) particles(:)%results%e(2)
+      write (12) particles(:)%results%e(3)
+      write (12) particles(:)%results%pot
+      write (12) w1
+      write (12) particles(:)%work
+      close (12)
+      call pepc_timber_tree()
+      call pepc_finalize()
+   end subroutine traverse_twice
 
    subroutine direct_all()
       type(t_particle_results), allocatable :: res(:)

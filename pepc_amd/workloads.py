@@ -153,3 +153,29 @@ def sample_targets(n, m, seed=7):
     r = _rng(seed)
     m = min(m, n)
     return np.sort(r.choice(n, size=m, replace=False)).astype(np.int64)
+
+
+def coincident(n=20000, seed=8, pairs=0.05, triples=0.01, cluster=0, cluster_side=2e-4,
+               cluster_at=(0.3, 0.3, 0.3)):
+    """Particles that share positions, so that keys tie: uniform in the unit
+    cube, optionally `cluster` of them in a small cube of side cluster_side
+    (its particles share many leading key digits), then a fraction `pairs`
+    of the particles copied onto other particles' positions and a fraction
+    `triples` placed on a third copy; random +-1 charges, eps=0.01,
+    theta=0.6.  Exercises the order of equal keys in the sort and the
+    duplicate-key fix-up that separates them."""
+    r = _rng(seed)
+    p = r.random((n, 3))
+    if cluster:
+        p[:cluster] = np.asarray(cluster_at) + cluster_side * r.random((cluster, 3))
+    idx = r.permutation(n)
+    npair = int(pairs * n)
+    ntri = int(triples * n)
+    src, dst = idx[:npair], idx[npair:2 * npair]
+    p[dst] = p[src]
+    third = idx[2 * npair:2 * npair + ntri]
+    p[third] = p[src[:ntri]]
+    q = r.choice(np.array([-1.0, 1.0]), n)
+    return System("coincident", p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), q,
+                  np.ones(n), np.zeros((n, 3)), eps2=1e-4, theta2=0.36, seed=seed,
+                  extra={"pairs": npair, "triples": ntri, "cluster": cluster})

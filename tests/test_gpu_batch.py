@@ -1,8 +1,8 @@
 """Batched systems: one grow/traverse over many independent systems equals a
 separate solve of each system, bit for bit (keys, permutation, positions after
 the duplicate fix-up, node counts, walk groups, interaction and MAC counts,
-fields and potentials).  The single-system solve is itself pinned against the
-oracle by test_gpu_parity.py."""
+fields and potentials), and every system equals the oracle's solve of that
+system alone (integer work bit-exact, FP64 fields to 1e-12)."""
 import numpy as np
 import pytest
 
@@ -88,6 +88,39 @@ def test_batch_equals_separate_solves(gpu, precision, G, pack):
         assert bst[key] == val, key
     sol.close()
     one.close()
+
+
+@pytest.mark.parametrize("G,pack", [(16, 0), (32, 32), (64, 64)])
+def test_batch_matches_oracle_per_system(gpu, orc, G, pack):
+    systems = _batch_systems()
+    x, y, z, q, m, v, off = workloads.concat(systems)
+    sol = gpu(theta2=0.36, eps2=1e-4, group_size=G, precision="fp64", group_pack=pack)
+    sol.grow_batch(x, y, z, q, off)
+    res = sol.traverse()
+    keys, perm, xs, ys, zs, qs = sol.sorted_host()
+    wk = sol.read_walk()
+    gf = wk["gfirst"].astype(np.int64)
+    e, pot, work = res.e.cpu().numpy().T, res.pot.cpu().numpy(), res.work.cpu().numpy()
+    for k, s in enumerate(systems):
+        a, b = off[k], off[k + 1]
+        if b - a < 2:
+            continue
+        d = orc.domain_decompose(s.x, s.y, s.z, s.q)
+        t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
+        w = orc.walk_group(d.x, d.y, d.z, t, 0.36, 1e-4, tile_size=G, keys=d.keys, pack=pack)
+        assert np.array_equal(keys[a:b], d.keys), s.name
+        assert np.array_equal(perm[a:b] - a, d.perm), s.name
+        assert np.array_equal(xs[a:b], d.x) and np.array_equal(zs[a:b], d.z), s.name
+        sel = (gf >= a) & (gf < b)
+        assert np.array_equal(gf[sel] - a, w.tile_first), s.name
+        assert np.array_equal(wk["gcount"][sel].astype(np.int64), w.tile_count), s.name
+        assert np.array_equal(wk["gmac"][sel].astype(np.int64), w.tile_mac), s.name
+        assert np.array_equal(wk["p2p"][a:b].astype(np.int32), w.p2p), s.name
+        assert np.array_equal(wk["m2p"][a:b].astype(np.int32), w.m2p), s.name
+        assert np.array_equal(work[a:b], w.work), s.name
+        assert np.max(np.abs(e[a:b] - w.e)) <= 1e-12 * np.abs(w.e).max(), s.name
+        assert np.max(np.abs(pot[a:b] - w.pot)) <= 1e-12 * np.abs(w.pot).max(), s.name
+    sol.close()
 
 
 def test_batch_rejects_bad_offsets_and_degenerate_system(gpu):

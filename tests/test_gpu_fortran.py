@@ -140,3 +140,81 @@ def test_fortran_directforce_matches_oracle(orc, tmp_path):
     e_ref, p_ref = orc.direct(x, y, z, q, np.arange(n), 1e-4)
     assert np.max(np.abs(f[:3].T - e_ref)) <= 1e-12 * np.abs(e_ref).max()
     assert np.max(np.abs(f[3] - p_ref)) <= 1e-12 * np.abs(p_ref).max()
+
+
+# The &libpepc and &calc_force_coulomb values of the reference frontends'
+# shipped parameter files (src/frontends/pepc-essential/params:33-52,
+# pepc-benchmark/params-check:39-61, pepc-mini/params:26-57), plus the GPU
+# walk's own group for exact FP64 comparison.
+REFERENCE_PARAMS = {
+    "pepc-essential": dict(theta2=0.09, np_mult="1.0", factor=4),
+    "pepc-benchmark": dict(theta2=0.25, np_mult="1.0", factor=4),
+    "pepc-mini": dict(theta2=0.36, np_mult="-50", factor=8),
+}
+
+
+def _reference_params(path, theta2, np_mult, factor):
+    with open(path, "w") as f:
+        f.write("&calc_force_coulomb\n! 3D coulomb\nforce_law  = 3\n! BH-mac\nmac_select = 0\n"
+                f"theta2     = {theta2!r}\neps2 = 1e-4\n/\n\n")
+        f.write(f"&libpepc\n num_threads = 8\n debug_level = 0\n np_mult = {np_mult}\n\n"
+                "! 0: no load balancing, 1: load balancing\n weighted = 1\n\n curve_type = 1\n"
+                f" \n interaction_list_length_factor = {factor}\n/\n\n")
+        f.write("&walk_para_pthreads\n max_particles_per_thread = 100\n/\n\n")
+        f.write("&walk_para_gpu\n group_size = 16\n precision = 'fp64'\n/\n")
+
+
+@pytest.mark.parametrize("frontend", list(REFERENCE_PARAMS))
+def test_fortran_reference_parameter_files(orc, tmp_path, frontend):
+    """The frontends' own &libpepc settings run (np_mult sizes the node pool
+    with the reference's rule, module_tree.f90:177-191), pepc_get_para_file
+    finds the file argument, and a second traversal of the grown array adds
+    to %work and %results as the reference's walk does
+    (module_tree_walk_simple.f90:335, :380): work = 1 + k after the first,
+    1 + 2k after the second."""
+    if not os.path.exists(EXE):
+        pytest.fail("pepc_mini_gpu not built (python -m pepc_amd.build)")
+    p = REFERENCE_PARAMS[frontend]
+    s = workloads.mini(n=8192, seed=44)
+    fin, fout, par = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "params"
+    with open(fin, "wb") as f:
+        np.array([s.n], np.int64).tofile(f)
+        for a in (s.x, s.y, s.z, s.q):
+            a.astype(np.float64).tofile(f)
+    _reference_params(par, p["theta2"], p["np_mult"], p["factor"])
+    r = subprocess.run([EXE, "--twice", str(par), str(fin), str(fout)], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    assert "found parameter file" in r.stdout
+    raw = open(fout, "rb").read()
+    n = int(np.frombuffer(raw[:8], np.int64)[0])
+    label = np.frombuffer(raw[8:8 + 8 * n], np.int64)
+    f = np.frombuffer(raw[8 + 8 * n:], np.float64).reshape(9, n)
+    d, t, w = orc.solve(s.x, s.y, s.z, s.q, p["theta2"], 1e-4, tile_size=16)
+    assert n == s.n and np.array_equal(label, d.perm)
+    assert np.array_equal(f[0], d.x) and np.array_equal(f[1], d.y) and np.array_equal(f[2], d.z)
+    e = f[3:6].T
+    assert np.max(np.abs(e - 2.0 * w.e)) <= 2e-12 * np.abs(w.e).max()
+    assert np.max(np.abs(f[6] - 2.0 * w.pot)) <= 2e-12 * np.abs(w.pot).max()
+    assert np.array_equal(f[7], w.work)
+    assert np.array_equal(f[8], 2.0 * w.work - 1.0)
+
+
+def test_fortran_np_mult_too_small_stops(tmp_path):
+    """A negative np_mult whose table cannot hold one node per particle stops
+    the run with the reference's message (module_tree.f90:187-189)."""
+    if not os.path.exists(EXE):
+        pytest.fail("pepc_mini_gpu not built (python -m pepc_amd.build)")
+    s = workloads.mini(n=8192, seed=45)
+    fin, fout, par = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "params"
+    with open(fin, "wb") as f:
+        np.array([s.n], np.int64).tofile(f)
+        for a in (s.x, s.y, s.z, s.q):
+            a.astype(np.float64).tofile(f)
+    _reference_params(par, 0.36, "-0.5", 8)    # 5000 entries for 8192 particles
+    r = subprocess.run([EXE, "--twice", str(par), str(fin), str(fout)], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode != 0
+    assert "You should increase np_mult" in r.stdout

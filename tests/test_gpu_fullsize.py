@@ -1,5 +1,7 @@
-"""Full-size workloads through size-independent properties (the oracle is too
-slow at these sizes; the small-size parity tests pin the same code paths):
+"""Full-size workloads through size-independent properties (every-target
+oracle parity at the benchmark sizes and rules is in
+test_gpu_fullsize_oracle.py; these checks cover what the oracle does not
+compute: the FP32 path against FP64, the direct-sum error, timesteps):
 
 * C2 Coulomb explosion, N = 4,194,304, theta in {0.3, 0.6, 0.9}: keys sorted
   and in range, a permutation, tree and group invariants, work = 1 + counts,

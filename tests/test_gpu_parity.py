@@ -74,7 +74,8 @@ def test_keys_permutation_tree_bitexact(gpu, orc, name):
     sol.close()
 
 
-@pytest.mark.parametrize("G,pack", [(1, 0), (16, 0), (64, 0), (16, 32), (16, 64), (64, 64)])
+@pytest.mark.parametrize("G,pack", [(1, 0), (16, 0), (32, 0), (64, 0), (16, 32), (16, 64),
+                                    (32, 32), (32, 64), (64, 64)])
 @pytest.mark.parametrize("name", ["mini", "plummer64k"])
 def test_groups_and_counts_bitexact(gpu, orc, name, G, pack):
     s, d, t, w = _oracle(orc, name, G, pack)
@@ -136,24 +137,28 @@ def test_direct_sum_matches_oracle(gpu, orc):
 
 def test_coincident_particles_dupfix(gpu, orc):
     # pairs of particles at identical positions exercise the duplicate-key
-    # fix-up (module_domains.f90:248-307); equal charges, so the result does
-    # not depend on which particle of a tied pair is moved
+    # fix-up (module_domains.f90:248-307); opposite charges on the pairs, so
+    # which particle of a tied pair moves shows in the charges and fields
     r = np.random.default_rng(5)
     p = r.random((3000, 3))
     p[1000:1010] = p[0:10]
     q = np.ones(3000)
+    q[1000:1010] = -1.0
     d = orc.domain_decompose(p[:, 0], p[:, 1], p[:, 2], q)
     assert d.nfixed > 0
-    sol = gpu(theta2=0.36, eps2=1e-4)
+    sol = gpu(theta2=0.36, eps2=1e-4, precision="fp64")
     st = sol.grow(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), q)
-    keys, perm, x, y, z, _ = sol.sorted_host()
+    keys, perm, x, y, z, qs = sol.sorted_host()
     assert st["dup_fixed"] == 10
     assert np.array_equal(keys, d.keys)
-    got = np.stack([x, y, z], 1)
-    want = np.stack([d.x, d.y, d.z], 1)
-    assert np.array_equal(got[np.lexsort(got.T)], want[np.lexsort(want.T)])
+    assert np.array_equal(perm, d.perm)
+    assert np.array_equal(x, d.x) and np.array_equal(y, d.y) and np.array_equal(z, d.z)
+    assert np.array_equal(qs, d.q)
     t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
     assert st["nnodes"] == t.nnodes
+    w = orc.walk_group(d.x, d.y, d.z, t, 0.36, 1e-4, tile_size=16, keys=d.keys)
+    res = sol.traverse()
+    assert np.max(np.abs(res.e.cpu().numpy().T - w.e)) <= 1e-12 * np.abs(w.e).max()
     sol.close()
 
 
@@ -215,4 +220,38 @@ def test_direct_host_form(gpu, orc):
             e = np.stack(out[:3], 1)
             assert np.max(np.abs(e - e_ref)) <= 1e-12 * np.abs(e_ref).max()
             assert np.max(np.abs(out[3] - p_ref)) <= 1e-12 * np.abs(p_ref).max()
+    sol.close()
+
+
+def test_host_grow_enlarges_the_node_pool(gpu, orc):
+    """The host form pepc_gpu_grow (the Fortran shim's path) never fails for
+    lack of nodes: with a pool of one node per particle, too small for any
+    real tree, it grows the pool to the count the build found and returns
+    the oracle's keys and permutation; pepc_gpu_reserve_nodes raises the pool
+    ahead of time."""
+    import ctypes as C
+    from pepc_amd import _lib
+    s = workloads.mini(n=20000, seed=9)
+    d = orc.domain_decompose(s.x, s.y, s.z, s.q)
+    t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    src = [np.ascontiguousarray(a, dtype=np.float64) for a in (s.x, s.y, s.z, s.q)]
+    for reserve in (0, 3 * s.n):
+        sol = gpu(theta2=s.theta2, eps2=s.eps2, node_factor=1.0)
+        if reserve:
+            assert sol.lib.pepc_gpu_reserve_nodes(sol.h, reserve) == _lib.OK
+        keys = np.zeros(s.n, np.uint64)
+        perm = np.zeros(s.n, np.int64)
+        xs, ys, zs = (np.zeros(s.n) for _ in range(3))
+        rc = sol.lib.pepc_gpu_grow(sol.h, s.n, *[vp(a) for a in src], vp(keys), vp(perm),
+                                   vp(xs), vp(ys), vp(zs))
+        assert rc == _lib.OK
+        st = sol.sync()
+        assert st["nnodes"] == t.nnodes
+        assert np.array_equal(keys, d.keys) and np.array_equal(perm, d.perm)
+        assert np.array_equal(xs, d.x)
+        assert sol.check_tree() == 0
+        sol.close()
+    sol = gpu()
+    assert sol.lib.pepc_gpu_reserve_nodes(sol.h, -1) == _lib.EINVAL
     sol.close()
