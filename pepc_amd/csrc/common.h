@@ -39,7 +39,7 @@ struct DevScalars {
   uint32_t nnodes;
   uint32_t ntwig;
   uint32_t ngroups;
-  uint32_t reserved;
+  uint32_t dup_scan_n;          // particles the duplicate fix-up scans (0 without duplicates)
   uint32_t maxlevel_used;
   uint32_t level_offset[kMaxLevel + 2];
   unsigned long long interactions;

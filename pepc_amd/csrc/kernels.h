@@ -29,8 +29,11 @@ void launch_sink_groups(hipStream_t s, int64_t m, int g, uint32_t* gfirst, uint3
 // sort.hip
 int64_t scan_blocks(int64_t n);
 int64_t sort_blocks(int64_t n);
+// Exclusive scan of n values (in place allowed); *total gets the sum when
+// total != nullptr.  With ndev != nullptr only the first min(n, *ndev) values
+// are scanned (the count is read on the device; n sizes the grid).
 void launch_scan(hipStream_t s, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* bsum,
-                 uint32_t* total);
+                 uint32_t* total, const uint32_t* ndev = nullptr);
 void launch_radix_pass(hipStream_t s, const unsigned long long* kin, const uint32_t* vin,
                        unsigned long long* kout, uint32_t* vout, int64_t n, int shift, int nbits,
                        uint32_t* hist, uint32_t* bsum);

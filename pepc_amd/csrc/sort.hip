@@ -40,9 +40,17 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
   return off + inc - v;
 }
 
+// With ndev != nullptr the scanned length is min(n, *ndev), read on the
+// device (n only sizes the grid); entries past it are neither read nor written.
+__device__ __forceinline__ int64_t scan_len(int64_t n, const uint32_t* ndev) {
+  return (ndev && (int64_t)*ndev < n) ? (int64_t)*ndev : n;
+}
+
 __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* __restrict__ in,
                                                                     int64_t n,
-                                                                    uint32_t* __restrict__ bsum) {
+                                                                    uint32_t* __restrict__ bsum,
+                                                                    const uint32_t* __restrict__ ndev) {
+  n = scan_len(n, ndev);
   const int64_t base = (int64_t)blockIdx.x * kScanTile;
   uint32_t s = 0;
 #pragma unroll
@@ -79,9 +87,12 @@ __global__ __launch_bounds__(kScanThreads) void scan_small_kernel(uint32_t* __re
 __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t* __restrict__ in,
                                                                    uint32_t* __restrict__ out,
                                                                    int64_t n,
-                                                                   const uint32_t* __restrict__ bsum) {
+                                                                   const uint32_t* __restrict__ bsum,
+                                                                   const uint32_t* __restrict__ ndev) {
   __shared__ uint32_t tile[kScanTile];
+  n = scan_len(n, ndev);
   const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  if (base >= n) return;   // uniform over the block
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
     int64_t e = base + j * kScanThreads + threadIdx.x;
@@ -548,12 +559,12 @@ int64_t scan_blocks(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
 int64_t sort_blocks(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
 
 void launch_scan(hipStream_t s, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* bsum,
-                 uint32_t* total) {
+                 uint32_t* total, const uint32_t* ndev) {
   if (n <= 0) return;
   const int64_t nb = scan_blocks(n);
-  scan_reduce_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(in, n, bsum);
+  scan_reduce_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(in, n, bsum, ndev);
   scan_small_kernel<<<1, kScanThreads, 0, s>>>(bsum, (int)nb, total);
-  scan_apply_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(in, out, n, bsum);
+  scan_apply_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(in, out, n, bsum, ndev);
 }
 
 // One stable pass over bits [shift, shift + nbits), output in (kout, vout).

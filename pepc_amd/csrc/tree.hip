@@ -53,7 +53,8 @@ __global__ void dup_detect_kernel(int64_t n, const unsigned long long* __restric
 __global__ void dup_prepare_kernel(int64_t n, const unsigned long long* __restrict__ keys,
                                    const uint32_t* __restrict__ sys,
                                    long long* __restrict__ kd, unsigned long long* __restrict__ old,
-                                   uint32_t* __restrict__ flag, const DevScalars* __restrict__ sc) {
+                                   uint32_t* __restrict__ flag, DevScalars* __restrict__ sc) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->dup_scan_n = sc->dup_count ? (uint32_t)n : 0u;
   if (sc->dup_count == 0) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -261,9 +262,8 @@ __global__ void node_fill_kernel(int64_t cap, const unsigned long long* __restri
                                  NodeLink* __restrict__ link, uint32_t* __restrict__ nfirst,
                                  uint32_t* __restrict__ flag, const DevScalars* __restrict__ sc) {
   const int64_t m = tree_broken(sc) ? 0 : (int64_t)sc->nnodes;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cap;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
        i += (int64_t)gridDim.x * blockDim.x) {
-    if (i >= m) { flag[i] = 0; continue; }
     const uint32_t d = dval[i];
     const uint32_t l = (uint32_t)dkey[i];
     const uint32_t a = dpart[d];
@@ -669,7 +669,7 @@ void launch_dupfix(hipStream_t s, int64_t n, unsigned long long* keys, double* x
                    long long* kd, unsigned long long* oldkeys, uint32_t* dflag, uint32_t* dlist,
                    uint32_t* bsum, DevScalars* sc) {
   dup_prepare_kernel<<<grid_for(n), kT, 0, s>>>(n, keys, sys, kd, oldkeys, dflag, sc);
-  launch_scan(s, dflag, dflag, n, bsum, nullptr);
+  launch_scan(s, dflag, dflag, n, bsum, nullptr, &sc->dup_scan_n);
   dup_compact_kernel<<<grid_for(n), kT, 0, s>>>(n, kd, dflag, dlist, sc);
   dup_serial_kernel<<<1, 1, 0, s>>>(n, keys, kd, dlist, sys, soff, sc);
   dup_recoord_kernel<<<grid_for(n), kT, 0, s>>>(n, keys, oldkeys, x, y, z, boxes, sys, sc);
@@ -689,7 +689,8 @@ void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size, int 
   node_fill_kernel<<<grid_for(cap), kT, 0, s>>>(cap, t.dkey[1], t.dval[1], t.dpart, t.inv, t.hi,
                                                 t.x, t.y, t.z, t.q, t.geom, t.link, t.nfirst,
                                                 t.flag, t.sc);
-  launch_scan(s, t.flag, t.flag, cap, t.bsum, &t.sc->ntwig);
+  // twig flags of the nnodes nodes only (entries past nnodes are never read)
+  launch_scan(s, t.flag, t.flag, cap, t.bsum, &t.sc->ntwig, &t.sc->nnodes);
   twig_rank_kernel<<<grid_for(cap), kT, 0, s>>>(t.link, t.flag, t.fc_rank, t.node_rank, t.sc);
   root_parent_kernel<<<grid_for(n), kT, 0, s>>>(t.parent, t.sc);
   nchild_kernel<<<grid_for(cap), kT, 0, s>>>(t.link, t.fc_rank, t.node_rank, t.parent, t.sc);
