@@ -151,6 +151,18 @@ int pepc_gpu_traverse_sinks_device(void *ctx, int64_t m, const double *x, const 
 int pepc_gpu_traverse_sinks(void *ctx, int64_t m, const double *x, const double *y,
                             const double *z, int32_t mode, double *ex, double *ey, double *ez,
                             double *pot, double *work);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): pepc_gpu_traverse_sinks_device with the same arguments and
+ * results, but its sort buffers, staging and counters come from a static
+ * device array of 26.2 MB per device instead of the context's growing sink
+ * buffers, so the call allocates nothing.  At most 262144 sinks (more return
+ * PEPC_GPU_EINVAL).  All calls on one device share that array: calls that
+ * overlap on one device (any context) race.  pepc_gpu_read_sink_counts reads
+ * the counts of the last call, as after the _device form. */
+int pepc_gpu_traverse_sinks_noalloc(void *ctx, int64_t m, const double *x, const double *y,
+                                    const double *z, int32_t mode, double *ex, double *ey,
+                                    double *ez, double *pot, double *work);
 /* Per-sink P2P and M2P counts of the last sink traversal (sinks' order). */
 int pepc_gpu_read_sink_counts(void *ctx, int64_t cap, uint32_t *p2p, uint32_t *m2p);
 

@@ -31,7 +31,11 @@ EXPORTS = [
     "pepc_gpu_set_periodic", "pepc_gpu_lattice_get", "pepc_gpu_constrain_periodic_device",
     "pepc_gpu_traverse_sinks_device", "pepc_gpu_traverse_sinks", "pepc_gpu_read_sink_counts",
     "pepc_gpu_check_tree", "pepc_gpu_direct", "pepc_gpu_reserve_nodes", "pepc_gpu_read_box",
+    "pepc_gpu_traverse_sinks_noalloc",
 ]
+
+# Sink capacity of pepc_gpu_traverse_sinks_noalloc (its static device scratch).
+NOALLOC_SINKS = 262144
 
 SINKS_PER_PARTICLE = 0
 SINKS_GROUPED = 1
@@ -138,6 +142,7 @@ def load(path: str = LIB_PATH):
     L.pepc_gpu_traverse_sinks_device.argtypes = [_vp, _i64, _vp, _vp, _vp, C.c_int32] + [_vp] * 5
     L.pepc_gpu_traverse_sinks.argtypes = [_vp, _i64, _vp, _vp, _vp, C.c_int32] + [_vp] * 5
     L.pepc_gpu_read_sink_counts.argtypes = [_vp, _i64, _vp, _vp]
+    L.pepc_gpu_traverse_sinks_noalloc.argtypes = [_vp, _i64, _vp, _vp, _vp, C.c_int32] + [_vp] * 5
     L.pepc_gpu_check_tree.argtypes = [_vp, C.POINTER(_i64)]
     L.pepc_gpu_reserve_nodes.argtypes = [_vp, _i64]
     L.pepc_gpu_read_box.argtypes = [_vp, _i64, _vp, _vp, _vp]

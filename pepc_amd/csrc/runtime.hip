@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -113,7 +114,91 @@ struct Ctx {
   unsigned long long *sk_k0 = nullptr, *sk_k1 = nullptr;
   uint32_t *sk_v0 = nullptr, *sk_v1 = nullptr, *sk_gfirst = nullptr, *sk_gcount = nullptr;
   uint32_t *sk_p2p = nullptr, *sk_m2p = nullptr, *sk_hist = nullptr, *sk_bsum = nullptr;
+  // where the last sink traversal left its counts and order (the context's
+  // sink buffers, or the no-allocation scratch of the device)
+  const uint32_t *sk_last_p2p = nullptr, *sk_last_m2p = nullptr, *sk_last_perm = nullptr;
 };
+
+// Device buffers of one sink traversal.
+struct SinkBufs {
+  double *x, *y, *z;              // sorted sink coordinates (grouped mode)
+  double* res[5];                 // results in sorted order (grouped mode)
+  unsigned long long *k0, *k1;    // sort keys (ping-pong)
+  uint32_t *v0, *v1;              // sort values (ping-pong); v0 = sorted -> input index
+  uint32_t *gfirst, *gcount;      // walk groups (grouped mode)
+  uint32_t *p2p, *m2p;            // per-sink interaction counts
+  uint32_t *hist, *bsum;          // radix histogram and scan blocks
+};
+
+// ---------------------------------------------------------------- scratch
+// Device memory of pepc_gpu_traverse_sinks_noalloc: one static array per
+// device, carved into the SinkBufs of up to kNoAllocSinks sinks.  Host code
+// never uses the array's name as an address: each device's copy is looked
+// up once (hipGetSymbolAddress at context creation, with the device current)
+// and that pointer is what the kernels receive.  All calls on one device
+// share its copy, so calls that overlap on one device race.  Every entry a
+// traversal reads is first written by the same traversal (keys, values,
+// histograms, groups, sorted coordinates, results, counts), so nothing needs
+// clearing between calls.
+constexpr int64_t kNoAllocSinks = 262144;
+constexpr size_t scratch_pad(size_t b) { return (b + 255) / 256 * 256; }
+constexpr int64_t kNoAllocSortBlocks = (kNoAllocSinks + 2047) / 2048;   // sort_blocks(kNoAllocSinks)
+constexpr int64_t kNoAllocHist = 256 * kNoAllocSortBlocks;
+constexpr int64_t kNoAllocBsum = 512;   // >= scan_blocks(kNoAllocHist) + 64 and the 256 digit totals
+constexpr size_t kScratchD = scratch_pad(sizeof(double) * kNoAllocSinks);
+constexpr size_t kScratchK = scratch_pad(sizeof(unsigned long long) * kNoAllocSinks);
+constexpr size_t kScratchU = scratch_pad(sizeof(uint32_t) * kNoAllocSinks);
+constexpr size_t kScratchBytes = 8 * kScratchD + 2 * kScratchK + 6 * kScratchU +
+                                 scratch_pad(sizeof(uint32_t) * kNoAllocHist) +
+                                 scratch_pad(sizeof(uint32_t) * kNoAllocBsum);   // 26.2 MB
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};           // device address of each device's scratch
+hipError_t g_scratch_err[kMaxDevices] = {};      // lookup failure, if any
+
+// Once per device, with that device current.
+void scratch_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    g_scratch_err[dev] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+    (void)hipGetLastError();
+    return;
+  }
+  g_scratch_ptr[dev] = ptr;
+  g_scratch_err[dev] = hipSuccess;
+}
+
+SinkBufs scratch_sink_bufs(void* base) {
+  unsigned char* p = static_cast<unsigned char*>(base);
+  SinkBufs b;
+  auto take = [&p](size_t bytes) {
+    unsigned char* q = p;
+    p += bytes;
+    return q;
+  };
+  b.x = reinterpret_cast<double*>(take(kScratchD));
+  b.y = reinterpret_cast<double*>(take(kScratchD));
+  b.z = reinterpret_cast<double*>(take(kScratchD));
+  for (int k = 0; k < 5; ++k) b.res[k] = reinterpret_cast<double*>(take(kScratchD));
+  b.k0 = reinterpret_cast<unsigned long long*>(take(kScratchK));
+  b.k1 = reinterpret_cast<unsigned long long*>(take(kScratchK));
+  b.v0 = reinterpret_cast<uint32_t*>(take(kScratchU));
+  b.v1 = reinterpret_cast<uint32_t*>(take(kScratchU));
+  b.gfirst = reinterpret_cast<uint32_t*>(take(kScratchU));
+  b.gcount = reinterpret_cast<uint32_t*>(take(kScratchU));
+  b.p2p = reinterpret_cast<uint32_t*>(take(kScratchU));
+  b.m2p = reinterpret_cast<uint32_t*>(take(kScratchU));
+  b.hist = reinterpret_cast<uint32_t*>(take(scratch_pad(sizeof(uint32_t) * kNoAllocHist)));
+  b.bsum = reinterpret_cast<uint32_t*>(take(scratch_pad(sizeof(uint32_t) * kNoAllocBsum)));
+  return b;
+}
 
 int fail(Ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -467,18 +552,46 @@ int ensure_sinks(Ctx* c, int64_t m) {
 // the sinks' order.  mode 0: one sink per walk group, in the given order;
 // mode 1: sinks sorted along the Hilbert curve of the source box, runs of
 // group_size per group, results scattered back.
-int traverse_sinks_impl(Ctx* c, int64_t m, const double* x, const double* y, const double* z,
-                        int mode, double* ex, double* ey, double* ez, double* pot, double* work) {
+SinkBufs ctx_sink_bufs(Ctx* c) {
+  SinkBufs b;
+  b.x = c->sk_x; b.y = c->sk_y; b.z = c->sk_z;
+  for (int k = 0; k < 5; ++k) b.res[k] = c->sk_res[k];
+  b.k0 = c->sk_k0; b.k1 = c->sk_k1; b.v0 = c->sk_v0; b.v1 = c->sk_v1;
+  b.gfirst = c->sk_gfirst; b.gcount = c->sk_gcount; b.p2p = c->sk_p2p; b.m2p = c->sk_m2p;
+  b.hist = c->sk_hist; b.bsum = c->sk_bsum;
+  return b;
+}
+
+int traverse_sinks_checks(Ctx* c, int64_t m, const double* x, const double* y, const double* z,
+                          int mode, double* ex, double* ey, double* ez, double* pot, double* work) {
   if (!c->grown) return fail(c, PEPC_GPU_EINVAL, "traverse_sinks: no tree (call grow first)");
   if (c->nsys > 1) return fail(c, PEPC_GPU_EINVAL, "traverse_sinks: needs a single grown system");
   if (m < 0 || m > (int64_t)0x7fffffff) return fail(c, PEPC_GPU_EINVAL, "traverse_sinks: bad sink count");
   if (mode != PEPC_GPU_SINKS_PER_PARTICLE && mode != PEPC_GPU_SINKS_GROUPED)
     return fail(c, PEPC_GPU_EINVAL, "traverse_sinks: unknown mode");
-  if (m == 0) return 0;
-  if (!x || !y || !z || !ex || !ey || !ez || !pot || !work)
+  if (m > 0 && (!x || !y || !z || !ex || !ey || !ez || !pot || !work))
     return fail(c, PEPC_GPU_EINVAL, "traverse_sinks: null argument");
-  int r = ensure_sinks(c, m);
+  return 0;
+}
+
+// The traversal proper on given sink buffers (enough for m sinks).
+int traverse_sinks_run(Ctx* c, const SinkBufs& b, int64_t m, const double* x, const double* y,
+                       const double* z, int mode, double* ex, double* ey, double* ez, double* pot,
+                       double* work);
+
+int traverse_sinks_impl(Ctx* c, int64_t m, const double* x, const double* y, const double* z,
+                        int mode, double* ex, double* ey, double* ez, double* pot, double* work) {
+  int r = traverse_sinks_checks(c, m, x, y, z, mode, ex, ey, ez, pot, work);
   if (r) return r;
+  if (m == 0) return 0;
+  r = ensure_sinks(c, m);
+  if (r) return r;
+  return traverse_sinks_run(c, ctx_sink_bufs(c), m, x, y, z, mode, ex, ey, ez, pot, work);
+}
+
+int traverse_sinks_run(Ctx* c, const SinkBufs& b, int64_t m, const double* x, const double* y,
+                       const double* z, int mode, double* ex, double* ey, double* ez, double* pot,
+                       double* work) {
   hipStream_t s = c->stream;
   WalkArgs a;
   a.n = c->n;
@@ -486,23 +599,23 @@ int traverse_sinks_impl(Ctx* c, int64_t m, const double* x, const double* y, con
   a.geom = c->geom; a.link = c->link; a.wm32 = c->wm32; a.wm64 = c->wm64;
   a.sc = c->sc;
   a.theta2 = c->cfg.theta2; a.eps2 = c->cfg.eps2; a.sign = c->cfg.sign;
-  a.np2p = c->sk_p2p; a.nm2p = c->sk_m2p; a.gstat = nullptr; a.queue = c->queue; a.nqueue = c->nxcd;
+  a.np2p = b.p2p; a.nm2p = b.m2p; a.gstat = nullptr; a.queue = c->queue; a.nqueue = c->nxcd;
   a.sys = nullptr; a.sysroot = c->sysroot;
   a.nbox = c->periodic ? c->nbox : 1;
   a.vbox = c->periodic ? c->vbox : nullptr;
   const double *tx = x, *ty = y, *tz = z;
   double* out[5] = {ex, ey, ez, pot, work};
   if (mode == PEPC_GPU_SINKS_GROUPED) {
-    launch_sink_keys(s, m, x, y, z, c->box, c->sk_k0, c->sk_v0);
-    launch_sort_keys(s, c->sk_k0, c->sk_v0, c->sk_k1, c->sk_v1, m, c->sk_hist, c->sk_bsum, nullptr, 1);
+    launch_sink_keys(s, m, x, y, z, c->box, b.k0, b.v0);
+    launch_sort_keys(s, b.k0, b.v0, b.k1, b.v1, m, b.hist, b.bsum, nullptr, 1);
     // the fourth gathered array is a scratch copy of x (gather4 moves four)
-    launch_gather4(s, m, c->sk_v0, x, y, z, x, c->sk_x, c->sk_y, c->sk_z, c->sk_res[4]);
+    launch_gather4(s, m, b.v0, x, y, z, x, b.x, b.y, b.z, b.res[4]);
     const int G = c->cfg.group_size;
-    launch_sink_groups(s, m, G, c->sk_gfirst, c->sk_gcount);
-    a.gfirst = c->sk_gfirst; a.gcount = c->sk_gcount;
+    launch_sink_groups(s, m, G, b.gfirst, b.gcount);
+    a.gfirst = b.gfirst; a.gcount = b.gcount;
     a.nsink_groups = (uint32_t)((m + G - 1) / G);
-    tx = c->sk_x; ty = c->sk_y; tz = c->sk_z;
-    for (int k = 0; k < 5; ++k) out[k] = c->sk_res[k];
+    tx = b.x; ty = b.y; tz = b.z;
+    for (int k = 0; k < 5; ++k) out[k] = b.res[k];
   } else {
     a.gfirst = nullptr; a.gcount = nullptr;
     a.nsink_groups = (uint32_t)m;
@@ -519,11 +632,14 @@ int traverse_sinks_impl(Ctx* c, int64_t m, const double* x, const double* y, con
   }
   if (mode == PEPC_GPU_SINKS_GROUPED) {
     double* dst[5] = {ex, ey, ez, pot, work};
-    for (int k = 0; k < 5; ++k) launch_scatter_back(s, m, c->sk_v0, c->sk_res[k], dst[k]);
+    for (int k = 0; k < 5; ++k) launch_scatter_back(s, m, b.v0, b.res[k], dst[k]);
   }
   HIPCHECK(c, hipGetLastError());
   c->sk_m = m;
   c->sk_mode = mode;
+  c->sk_last_p2p = b.p2p;
+  c->sk_last_m2p = b.m2p;
+  c->sk_last_perm = b.v0;
   return 0;
 }
 
@@ -591,6 +707,7 @@ int pepc_gpu_create(void** out, const pepc_gpu_config* cfg) {
                        ? cfg->walk_blocks
                        : walk_default_blocks(cfg->precision == PEPC_GPU_FP64 ? kWalkFP64 : kWalkFP32Kahan,
                                              cfg->list_tile);
+  scratch_setup(cfg->device);   // the no-allocation entry point's memory on this device
   *out = c;
   return 0;
 }
@@ -722,6 +839,33 @@ int pepc_gpu_traverse_sinks_device(void* p, int64_t m, const double* x, const do
   return traverse_sinks_impl(c, m, x, y, z, mode, ex, ey, ez, pot, work);
 }
 
+int pepc_gpu_traverse_sinks_noalloc(void* p, int64_t m, const double* x, const double* y,
+                                    const double* z, int32_t mode, double* ex, double* ey,
+                                    double* ez, double* pot, double* work) {
+  Ctx* c = as_ctx(p);
+  if (!c) return PEPC_GPU_EINVAL;
+  int r = traverse_sinks_checks(c, m, x, y, z, mode, ex, ey, ez, pot, work);
+  if (r) return r;
+  if (m > kNoAllocSinks)
+    return fail(c, PEPC_GPU_EINVAL,
+                "traverse_sinks_noalloc: " + std::to_string(m) + " sinks exceed the " +
+                    std::to_string(kNoAllocSinks) + "-sink scratch (use pepc_gpu_traverse_sinks_device)");
+  const int dev = c->cfg.device;
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev >= 0 && dev < kMaxDevices) base = g_scratch_ptr[dev];
+    if (!base)
+      return fail(c, PEPC_GPU_EHIP,
+                  std::string("traverse_sinks_noalloc: no scratch on this device (hipGetSymbolAddress: ") +
+                      hipGetErrorString(dev >= 0 && dev < kMaxDevices ? g_scratch_err[dev]
+                                                                       : hipErrorInvalidDevice) + ")");
+  }
+  HIPCHECK(c, hipSetDevice(dev));
+  if (m == 0) return 0;
+  return traverse_sinks_run(c, scratch_sink_bufs(base), m, x, y, z, mode, ex, ey, ez, pot, work);
+}
+
 int pepc_gpu_traverse_sinks(void* p, int64_t m, const double* x, const double* y, const double* z,
                             int32_t mode, double* ex, double* ey, double* ez, double* pot,
                             double* work) {
@@ -756,11 +900,11 @@ int pepc_gpu_read_sink_counts(void* p, int64_t cap, uint32_t* p2p, uint32_t* m2p
   if (cap < m) return fail(c, PEPC_GPU_EINVAL, "read_sink_counts: capacity below the sink count");
   if (m == 0) return 0;
   std::vector<uint32_t> a(m), b(m), perm;
-  HIPCHECK(c, hipMemcpy(a.data(), c->sk_p2p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
-  HIPCHECK(c, hipMemcpy(b.data(), c->sk_m2p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(a.data(), c->sk_last_p2p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+  HIPCHECK(c, hipMemcpy(b.data(), c->sk_last_m2p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
   if (c->sk_mode == PEPC_GPU_SINKS_GROUPED) {
     perm.resize(m);
-    HIPCHECK(c, hipMemcpy(perm.data(), c->sk_v0, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(perm.data(), c->sk_last_perm, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
   }
   for (int64_t i = 0; i < m; ++i) {
     const int64_t d = perm.empty() ? i : (int64_t)perm[i];
