@@ -358,27 +358,36 @@ int pepc_gpu_read_energies(void *ctx, int64_t cap, double *out, int64_t *nrec);
  * far_field, adds the lattice contribution of everything beyond
  * (fmm_framework_timestep + fmm_sum_lattice_force, module_fmm_framework.f90:
  * 132-149, 766-833: lattice coefficients to order 40, central-box multipoles
- * to order 20, fictitious-charge extrinsic correction).  A unit-cube lattice
- * uses the reference's pretabulated coefficients (:114-121, :260-508); other
- * lattices compute them (calc_lattice_coefficients, :154-213) on the device
- * in set_periodic.  The on-device timestep wraps positions back into the box
- * after each drift (constrain_periodic, module_mirror_boxes.f90:174-210).
+ * to order 20, well separation ws = mirror_box_layers, the extrinsic-to-
+ * intrinsic correction fmm_extrinsic_correction, :42-49, 807-829).  A
+ * unit-cube lattice with one mirror layer uses the reference's pretabulated
+ * coefficients (:114-121, :260-508); other lattices and layer counts compute
+ * them (calc_lattice_coefficients, :154-213) on the device in set_periodic.
+ * The on-device timestep wraps positions back into the box after each drift
+ * (constrain_periodic, module_mirror_boxes.f90:174-210).
  *
- * Limits: far_field needs all three axes periodic and mirror_box_layers = 1;
- * near field alone takes mirror_box_layers 1..3.  Not with batched systems,
- * the adjoint or frozen lists (EINVAL).  Particles must lie in the central
- * box (the reference warns, check_lattice_boundaries :261-281). */
+ * Limits: far_field needs all three axes periodic; mirror_box_layers 1..3.
+ * Not with batched systems, the adjoint or frozen lists (EINVAL).  Particles
+ * must lie in the central box (the reference warns, check_lattice_boundaries
+ * :261-281). */
+/* fmm_extrinsic_correction values (module_fmm_framework.f90:42-46; the
+ * reference's MEASUREMENT (3) is unimplemented there and EINVAL here). */
+enum {
+    PEPC_GPU_EXTRINSIC_NONE = 0,        /* none */
+    PEPC_GPU_EXTRINSIC_REDLACK = 1,     /* Redlack-Grindlay dipole and trace terms */
+    PEPC_GPU_EXTRINSIC_FICTCHARGE = 2   /* fictitious charges (Kudin), the default */
+};
 typedef struct {
     int32_t periodicity[3];       /* nonzero = periodic along t_lattice_k */
     int32_t mirror_box_layers;    /* default 1 */
     double t_lattice[9];          /* rows: t_lattice_1, t_lattice_2, t_lattice_3 */
     double origin[3];             /* LatticeOrigin */
     int32_t far_field;            /* include_far_field_if_periodic (default 1) */
-    int32_t pad;
+    int32_t extrinsic_correction; /* fmm_extrinsic_correction (default FICTCHARGE) */
 } pepc_gpu_periodic;
 
 /* Fills *p with the reference defaults: unit cube at the origin, no axis
- * periodic, one layer, far field on. */
+ * periodic, one layer, far field on, fictitious-charge correction. */
 void pepc_gpu_periodic_default(pepc_gpu_periodic *p);
 
 /* Sets (p != NULL) or clears (p == NULL, or no periodic axis) periodic

@@ -169,14 +169,20 @@ int  or_neighbour_boxes(const int32_t per[3], int32_t layers, int32_t *boxes);
 void or_lattice_vect(const double lat[9], const int32_t ijk[3], double v[3]);
 int  or_lattice_coefficients(const double lat[9], const int32_t per[3], int32_t layers,
                              int32_t maxiter, double *ml);
+/* extrinsic-to-intrinsic corrections, module_fmm_framework.f90:42-46 */
+#define OR_EXTRINSIC_NONE 0
+#define OR_EXTRINSIC_REDLACK 1
+#define OR_EXTRINSIC_FICTCHARGE 2
 void or_lattice_timestep(const double lat[9], const double origin[3], const int32_t per[3],
                          const double *ml, int64_t n, const double *x, const double *y,
-                         const double *z, const double *q, double *omega, double *mu,
-                         double *fict, int32_t *nfict, double dipole[3]);
+                         const double *z, const double *q, int32_t correction, double *omega,
+                         double *mu, double *fict, int32_t *nfict, double dipole[3],
+                         double *quad_trace);
 void or_lattice_field(const double lat[9], const double origin[3], const int32_t per[3],
-                      int32_t layers, const double *mu, const double *fict, int32_t nfict,
-                      int64_t n, const double *x, const double *y, const double *z,
-                      double *ex, double *ey, double *ez, double *pot);
+                      int32_t layers, const double *mu, int32_t correction, const double *fict,
+                      int32_t nfict, const double dipole[3], double quad_trace, int64_t n,
+                      const double *x, const double *y, const double *z, double *ex,
+                      double *ey, double *ez, double *pot);
 
 /* ---- kernels and direct sum ------------------------------------------------ */
 void or_p2p(const double *t_charge, const double d[3], double dist2,

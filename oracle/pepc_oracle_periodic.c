@@ -245,23 +245,27 @@ static void lattice_center(const double lat[9], const double origin[3], double c
         c[d] = 0.5 * (lat[d] + lat[3 + d] + lat[6 + d]) + origin[d];
 }
 
-/* fmm_framework_timestep (:132-149) with the default fictitious-charge
- * correction: box dipole (calc_box_dipole :605-631), omega_tilde of the
- * particles and the fictitious charges about the box centre with the
- * monopole removed (calc_omega_tilde :514-600), mu_cent = M2L(ML, omega)
- * (calc_mu_cent :636-654).  Outputs: omega (2*OR_NMULT), mu (2*OR_NTAYLOR),
- * fict (4 x {x, y, z, q}), nfict, dipole[3]. */
+/* fmm_framework_timestep (:132-149): box dipole and quadrupole trace
+ * (calc_box_dipole :605-631), omega_tilde of the particles (and, with the
+ * fictitious-charge correction, of the fictitious charges) about the box
+ * centre with the monopole removed (calc_omega_tilde :514-600), mu_cent =
+ * M2L(ML, omega) (calc_mu_cent :636-654).  correction: OR_EXTRINSIC_NONE,
+ * _REDLACK or _FICTCHARGE (module_fmm_framework.f90:42-46).  Outputs: omega
+ * (2*OR_NMULT), mu (2*OR_NTAYLOR), fict (4 x {x, y, z, q}; none unless
+ * FICTCHARGE), nfict, dipole[3], quad_trace[1]. */
 void or_lattice_timestep(const double lat[9], const double origin[3], const int32_t per[3],
                          const double *ml_in, int64_t n, const double *x, const double *y,
-                         const double *z, const double *q, double *omega_out, double *mu_out,
-                         double *fict, int32_t *nfict, double dipole[3])
+                         const double *z, const double *q, int32_t correction, double *omega_out,
+                         double *mu_out, double *fict, int32_t *nfict, double dipole[3],
+                         double *quad_trace)
 {
     double c[3];
     lattice_center(lat, origin, c);
-    double dp[3] = {0.0, 0.0, 0.0};
+    double dp[3] = {0.0, 0.0, 0.0}, qt = 0.0;
     for (int64_t p = 0; p < n; ++p) {
         double r[3] = {x[p] - c[0], y[p] - c[1], z[p] - c[2]};
         for (int d = 0; d < 3; ++d) dp[d] += q[p] * r[d];
+        qt += q[p] * (r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
     }
     cplx *om = (cplx *)calloc(NM, sizeof(cplx));
     for (int64_t p = 0; p < n; ++p) {
@@ -271,18 +275,20 @@ void or_lattice_timestep(const double lat[9], const double origin[3], const int3
             for (int m = 0; m <= l; ++m) om[tblinv(l, m)] += q[p] * o_multipole(l, m, s);
     }
     int nf = 0;
-    double fq_sum = 0.0;
-    for (int p = 0; p < 3; ++p) {
-        if (!per[p]) continue;
-        const double *t = lat + 3 * p;
-        for (int d = 0; d < 3; ++d) fict[4 * nf + d] = origin[d] + t[d];
-        fict[4 * nf + 3] = -dp[p] / sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-        fq_sum += fict[4 * nf + 3];
+    if (correction == OR_EXTRINSIC_FICTCHARGE) {
+        double fq_sum = 0.0;
+        for (int p = 0; p < 3; ++p) {
+            if (!per[p]) continue;
+            const double *t = lat + 3 * p;
+            for (int d = 0; d < 3; ++d) fict[4 * nf + d] = origin[d] + t[d];
+            fict[4 * nf + 3] = -dp[p] / sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+            fq_sum += fict[4 * nf + 3];
+            nf++;
+        }
+        for (int d = 0; d < 3; ++d) fict[4 * nf + d] = origin[d];
+        fict[4 * nf + 3] = -fq_sum;
         nf++;
     }
-    for (int d = 0; d < 3; ++d) fict[4 * nf + d] = origin[d];
-    fict[4 * nf + 3] = -fq_sum;
-    nf++;
     for (int f = 0; f < nf; ++f) {
         double r[3] = {fict[4 * f] - c[0], fict[4 * f + 1] - c[1], fict[4 * f + 2] - c[2]}, s[3];
         to_spherical(r, s);
@@ -295,20 +301,31 @@ void or_lattice_timestep(const double lat[9], const double origin[3], const int3
     memcpy(omega_out, om, sizeof(cplx) * NM);
     memcpy(mu_out, mu, sizeof(cplx) * NT);
     for (int d = 0; d < 3; ++d) dipole[d] = dp[d];
+    *quad_trace = qt;
     *nfict = nf;
     free(om); free(mu);
 }
 
 /* fmm_sum_lattice_force (:766-833) per position: mu shifted to the particle
- * (L2L to order 1), E = -grad, then the fictitious charges of every
- * neighbour box (plain Coulomb pair, no softening).  Adds nothing else. */
+ * (L2L to order 1), E = -grad, then the extrinsic correction: REDLACK adds
+ * E += 2 pf D, phi += -2 pf (r - c).D + pf Q with pf = 2 pi / (3 V), D the
+ * box dipole and Q the quadrupole trace (:810-812); FICTCHARGE adds the
+ * fictitious charges of every neighbour box (plain Coulomb pair, no
+ * softening, :813-823); NONE adds nothing.  V = |(t1 x t2).t3|
+ * (module_mirror_boxes.f90:153). */
 void or_lattice_field(const double lat[9], const double origin[3], const int32_t per[3],
-                      int32_t layers, const double *mu_in, const double *fict, int32_t nfict,
+                      int32_t layers, const double *mu_in, int32_t correction,
+                      const double *fict, int32_t nfict, const double dipole[3], double quad_trace,
                       int64_t n, const double *x, const double *y, const double *z,
                       double *ex, double *ey, double *ez, double *pot)
 {
     double c[3];
     lattice_center(lat, origin, c);
+    const double vol = fabs((lat[1] * lat[5] - lat[2] * lat[4]) * lat[6] +
+                            (lat[2] * lat[3] - lat[0] * lat[5]) * lat[7] +
+                            (lat[0] * lat[4] - lat[1] * lat[3]) * lat[8]);
+    const double pf = 2.0 * acos(-1.0) / (3.0 * vol);
+    if (correction != OR_EXTRINSIC_FICTCHARGE) nfict = 0;
     int nb = or_neighbour_boxes(per, layers, NULL);
     int32_t *boxes = (int32_t *)malloc(sizeof(int32_t) * 3 * (size_t)nb);
     or_neighbour_boxes(per, layers, boxes);
@@ -323,6 +340,11 @@ void or_lattice_field(const double lat[9], const double origin[3], const int32_t
         l2l(o, mu, 1, ms);
         double e[3] = {-creal(ms[2]), -cimag(ms[2]), -creal(ms[1])};
         double phi = creal(ms[0]);
+        if (correction == OR_EXTRINSIC_REDLACK) {
+            for (int d = 0; d < 3; ++d) e[d] += 2.0 * pf * dipole[d];
+            phi += -2.0 * pf * (r[0] * dipole[0] + r[1] * dipole[1] + r[2] * dipole[2]) +
+                   pf * quad_trace;
+        }
         for (int f = 0; f < nfict; ++f)
             for (int b = 0; b < nb; ++b) {
                 double v[3], d[3];

@@ -41,9 +41,10 @@ def _lib():
         L.or_lattice_coefficients.argtypes = [_f64, _i32, C.c_int32, C.c_int32, _f64]
         L.or_lattice_coefficients.restype = C.c_int
         L.or_lattice_timestep.argtypes = [_f64, _f64, _i32, _f64, C.c_int64, _f64, _f64, _f64,
-                                          _f64, _f64, _f64, _f64, _i32, _f64]
-        L.or_lattice_field.argtypes = [_f64, _f64, _i32, C.c_int32, _f64, _f64, C.c_int32,
-                                       C.c_int64, _f64, _f64, _f64, _f64, _f64, _f64, _f64]
+                                          _f64, C.c_int32, _f64, _f64, _f64, _i32, _f64, _f64]
+        L.or_lattice_field.argtypes = [_f64, _f64, _i32, C.c_int32, _f64, C.c_int32, _f64,
+                                       C.c_int32, _f64, C.c_double, C.c_int64, _f64, _f64, _f64,
+                                       _f64, _f64, _f64, _f64]
         L.or_walk_group_boxes.argtypes = [C.c_int64, _f64, _f64, _f64, C.c_int64, _i32, _o._i64,
                                           _o._i64, _o._i64, _f64, _f64, C.c_double, C.c_double,
                                           C.c_double, C.c_int64, _o._i64, _o._i64, C.c_int64,
@@ -104,24 +105,31 @@ def unit_cube_table(fixture) -> np.ndarray:
     return ml
 
 
+# extrinsic-to-intrinsic corrections (module_fmm_framework.f90:42-46)
+EXTRINSIC_NONE, EXTRINSIC_REDLACK, EXTRINSIC_FICTCHARGE = 0, 1, 2
+
+
 @dataclass
 class Step:
     omega: np.ndarray      # complex (NM,)
     mu: np.ndarray         # complex (NT,)
     fict: np.ndarray       # (nfict, 4) x, y, z, q
     dipole: np.ndarray     # (3,)
+    quad_trace: float = 0.0
 
 
 class Lattice:
-    """3-D periodic far field with the fictitious-charge correction."""
+    """3-D periodic far field with an extrinsic-to-intrinsic correction
+    (default: fictitious charges, the reference's default)."""
 
     def __init__(self, ml, lattice=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0),
-                 periodicity=(1, 1, 1), layers=1):
+                 periodicity=(1, 1, 1), layers=1, correction=EXTRINSIC_FICTCHARGE):
         self.ml = np.ascontiguousarray(np.asarray(ml, np.complex128))
         self.lat = _lat(lattice)
         self.origin = np.ascontiguousarray(np.asarray(origin, np.float64))
         self.per = _per(periodicity)
         self.layers = int(layers)
+        self.correction = int(correction)
 
     def timestep(self, x, y, z, q) -> Step:
         om = np.zeros(2 * NM)
@@ -129,12 +137,13 @@ class Lattice:
         fict = np.zeros(16)
         nf = np.zeros(1, np.int32)
         dip = np.zeros(3)
+        qt = np.zeros(1)
         c = _o._c
         _lib().or_lattice_timestep(self.lat, self.origin, self.per, self.ml.view(np.float64),
                                    len(q), c(x, np.float64), c(y, np.float64), c(z, np.float64),
-                                   c(q, np.float64), om, mu, fict, nf, dip)
+                                   c(q, np.float64), self.correction, om, mu, fict, nf, dip, qt)
         self.step = Step(om.view(np.complex128), mu.view(np.complex128),
-                         fict[:4 * nf[0]].reshape(-1, 4).copy(), dip)
+                         fict[:4 * nf[0]].reshape(-1, 4).copy(), dip, float(qt[0]))
         return self.step
 
     def field(self, x, y, z):
@@ -146,8 +155,9 @@ class Lattice:
         fict[:st.fict.size] = st.fict.ravel()
         c = _o._c
         _lib().or_lattice_field(self.lat, self.origin, self.per, self.layers,
-                                np.ascontiguousarray(st.mu).view(np.float64), fict,
-                                len(st.fict), n, c(x, np.float64), c(y, np.float64),
+                                np.ascontiguousarray(st.mu).view(np.float64), self.correction,
+                                fict, len(st.fict), np.ascontiguousarray(st.dipole, np.float64),
+                                float(st.quad_trace), n, c(x, np.float64), c(y, np.float64),
                                 c(z, np.float64), ex, ey, ez, pot)
         return np.stack([ex, ey, ez], 1), pot
 
@@ -209,7 +219,7 @@ def walk_sinks(sx, sy, sz, tree, nsources, theta2, eps2, vbox=((0.0, 0.0, 0.0),)
 
 def solve_periodic(x, y, z, q, theta2, eps2, ml, lattice=(1.0, 1.0, 1.0),
                    origin=(0.0, 0.0, 0.0), layers=1, far_field=True, tile_size=16, sign=1.0,
-                   nthreads=0):
+                   nthreads=0, correction=EXTRINSIC_FICTCHARGE):
     """Grow + traverse with 3-D periodicity (libpepc_traverse_tree,
     module_libpepc_main.f90:161-200): mirror-box walks, then the lattice
     contribution of every particle (calc_force_per_particle,
@@ -221,7 +231,7 @@ def solve_periodic(x, y, z, q, theta2, eps2, ml, lattice=(1.0, 1.0, 1.0),
     w = walk_group_boxes(d.x, d.y, d.z, t, theta2, eps2, vb, tile_size=tile_size, sign=sign,
                          nthreads=nthreads, keys=d.keys)
     if far_field:
-        lat = Lattice(ml, lattice, origin, (1, 1, 1), layers)
+        lat = Lattice(ml, lattice, origin, (1, 1, 1), layers, correction)
         lat.timestep(d.x, d.y, d.z, d.q)
         e, p = lat.field(d.x, d.y, d.z)
         w.e = w.e + sign * e

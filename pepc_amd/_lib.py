@@ -76,7 +76,11 @@ class TuneEntry(C.Structure):
 class Periodic(C.Structure):
     _fields_ = [("periodicity", C.c_int32 * 3), ("mirror_box_layers", C.c_int32),
                 ("t_lattice", C.c_double * 9), ("origin", C.c_double * 3),
-                ("far_field", C.c_int32), ("pad", C.c_int32)]
+                ("far_field", C.c_int32), ("extrinsic_correction", C.c_int32)]
+
+
+# fmm_extrinsic_correction values (pepc_gpu_periodic.extrinsic_correction)
+EXTRINSIC_NONE, EXTRINSIC_REDLACK, EXTRINSIC_FICTCHARGE = 0, 1, 2
 
 
 class PepcGpuError(RuntimeError):

@@ -179,7 +179,14 @@ struct LatticeGeom {
   double latinv[9];                             // inverse of lat
   int per[3];                                   // periodic axes
   int simple;                                   // principal axes
+  int correction;                               // extrinsic correction (kExtrinsic*)
+  double volume;                                // |(t1 x t2) . t3|
 };
+// Extrinsic-to-intrinsic corrections, module_fmm_framework.f90:42-46.
+enum { kExtrinsicNone = 0, kExtrinsicRedlack = 1, kExtrinsicFictCharge = 2 };
+// fict block of the far field: 4 x {x, y, z, q} fictitious charges, their
+// count, then the box dipole (3) and quadrupole trace of the last timestep.
+constexpr int kLatticeFict = 21;
 // Lattice coefficients for neighbour vectors vec[3*nb] (central box last),
 // UL scale 2*layers+1.  Writes mstar, lstar (scratch) and ml.
 void launch_lattice_coefficients(hipStream_t s, const double* vec, int nb, double scale,
@@ -190,8 +197,8 @@ int lattice_moment_blocks(int64_t n);
 // inv[kLatticeNM + m] = 1 / (2m); kLatticeNM + 21 doubles.
 void launch_lattice_inv(hipStream_t s, double* inv);
 // Per-step moments, mu and the field added to ex..pot (sign applied).
-// part: lattice_moment_blocks(n) * kLatticeNM, dpart: 3 per block, om and
-// mu: kLatticeNM, dip: 3, fict: 17 doubles, coef: 8 * kLatticeNM doubles.
+// part: lattice_moment_blocks(n) * kLatticeNM, dpart: 4 per block, om and
+// mu: kLatticeNM, dip: 4, fict: kLatticeFict doubles, coef: 8 * kLatticeNM.
 void launch_lattice_far(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
                         const double* q, const LatticeGeom& g, const double2* ml, const double* fact,
                         const double* inv, const double* vbox, int nb, double sign, double2* part,

@@ -174,6 +174,79 @@ __global__ __launch_bounds__(kTermCap) void lattice_terms_kernel(const double* _
   }
 }
 
+// More terms than one block sorts in LDS (mirror_box_layers >= 2: 124 x 125
+// or 342 x 343 L* terms per entry): every term is added in double-double
+// arithmetic (TwoSum, Knuth; contraction is off in this build), per thread
+// over a strided run, then in a fixed tree over the block, and rounded once.
+// The result is the exactly rounded sum up to about 2^-100 relative, which
+// is at least as accurate as the reference's smallest-first order.
+struct DD {
+  double hi, lo;
+};
+
+__device__ __forceinline__ DD dd_add(DD a, double b) {
+  const double s = a.hi + b;
+  const double bb = s - a.hi;
+  double e = (a.hi - (s - bb)) + (b - bb);
+  e += a.lo;
+  const double h = s + e;
+  return DD{h, e - (h - s)};
+}
+
+__device__ __forceinline__ DD dd_add(DD a, DD b) {
+  const double s = a.hi + b.hi;
+  const double bb = s - a.hi;
+  double e = (a.hi - (s - bb)) + (b.hi - bb);
+  e += a.lo + b.lo;
+  const double h = s + e;
+  return DD{h, e - (h - s)};
+}
+
+constexpr int kTermDD = 1024;
+
+__global__ __launch_bounds__(kTermDD) void lattice_terms_dd_kernel(const double* __restrict__ vec,
+                                                                   int nb, double scale,
+                                                                   const double* __restrict__ fact,
+                                                                   double2* __restrict__ mstar,
+                                                                   double2* __restrict__ lstar) {
+  __shared__ DD sre[kTermDD], sim[kTermDD];
+  const bool is_m = blockIdx.x < (unsigned)kNM;
+  const int t = is_m ? (int)blockIdx.x : (int)blockIdx.x - kNM;
+  int l, m;
+  tdecode(t, l, m);
+  const int64_t nterm = is_m ? (int64_t)nb : (int64_t)(nb - 1) * nb;
+  DD re{0.0, 0.0}, im{0.0, 0.0};
+  for (int64_t i = threadIdx.x; i < nterm; i += kTermDD) {
+    double2 v;
+    if (is_m) {
+      v = o_multipole(l, m, to_sph(-vec[3 * i], -vec[3 * i + 1], -vec[3 * i + 2]), fact);
+    } else {
+      const int64_t a = i / nb, b = i - a * nb;   // a: outer box (central box excluded)
+      const double cx = scale * vec[3 * a] + vec[3 * b];
+      const double cy = scale * vec[3 * a + 1] + vec[3 * b + 1];
+      const double cz = scale * vec[3 * a + 2] + vec[3 * b + 2];
+      v = m_taylor(l, m, to_sph(cx, cy, cz), fact);
+    }
+    re = dd_add(re, v.x);
+    im = dd_add(im, v.y);
+  }
+  sre[threadIdx.x] = re;
+  sim[threadIdx.x] = im;
+  __syncthreads();
+  for (int h = kTermDD / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      sre[threadIdx.x] = dd_add(sre[threadIdx.x], sre[threadIdx.x + h]);
+      sim[threadIdx.x] = dd_add(sim[threadIdx.x], sim[threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double2 r = make_double2(sre[0].hi + sre[0].lo, sim[0].hi + sim[0].lo);
+    if (is_m) mstar[t] = r;
+    else lstar[t] = r;
+  }
+}
+
 // ML = L*; then maxiter times ML = M2L(UL(ML), M*) + L* (UL divides order l by
 // scale^(l+1), :982-1000; M2L :912-938).  One block, one entry per thread.
 __global__ __launch_bounds__(1024) void lattice_iterate_kernel(const double2* __restrict__ mstar,
@@ -215,12 +288,12 @@ __global__ __launch_bounds__(1024) void lattice_iterate_kernel(const double2* __
 //   O_MM = (x - i y)^M / (2^M M!),
 //   O_lM = ((2l-1) z O_{l-1,M} - r^2 O_{l-2,M}) / ((l-M)(l+M)),
 // which equal the reference's r^l P_l^M / (l+M)! e^{-i M p}.  Row 0 also
-// sums the dipole q r.  Every thread handles a strided run of particles (the
+// sums the dipole q r and the quadrupole trace q r.r (calc_box_dipole).  Every thread handles a strided run of particles (the
 // next one's coordinates loaded while the current one is evaluated); each
 // block writes its partial sums, which lattice_omega_kernel adds in block
 // order (deterministic).
 constexpr int kMomT = 256;
-constexpr int kMomRed = 2 * (kLM + 1) + 3;
+constexpr int kMomRed = 2 * (kLM + 1) + 4;
 
 template <int M>
 __device__ __forceinline__ void moments_row(int64_t n, const double* __restrict__ x,
@@ -231,7 +304,7 @@ __device__ __forceinline__ void moments_row(int64_t n, const double* __restrict_
                                             double (*red)[kMomRed]) {
 #pragma clang fp contract(fast)   // tolerance-checked sums: fused multiply-adds allowed
   constexpr int K = kLM + 1 - M;
-  double ar[K], ai[K], dp[3] = {0.0, 0.0, 0.0};
+  double ar[K], ai[K], dp[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int j = 0; j < K; ++j) { ar[j] = 0.0; ai[j] = 0.0; }
   const int64_t stride = (int64_t)gridDim.x * kMomT;
@@ -245,7 +318,9 @@ __device__ __forceinline__ void moments_row(int64_t n, const double* __restrict_
     if (i2 < n) { nx = x[i2]; ny = y[i2]; nz = z[i2]; nq = q[i2]; }
     i = i2;
     const double r2 = rx * rx + ry * ry + rz * rz;
-    if constexpr (M == 0) { dp[0] += qi * rx; dp[1] += qi * ry; dp[2] += qi * rz; }
+    if constexpr (M == 0) {
+      dp[0] += qi * rx; dp[1] += qi * ry; dp[2] += qi * rz; dp[3] += qi * r2;
+    }
     double wr = 1.0, wi = 0.0;
 #pragma unroll
     for (int k = 1; k <= M; ++k) {
@@ -275,21 +350,21 @@ __device__ __forceinline__ void moments_row(int64_t n, const double* __restrict_
   }
   if constexpr (M == 0) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
+    for (int d = 0; d < 4; ++d) {
       const double sd = wave_sum(dp[d]);
       if (lane_id() == 0) red[w][2 * K + d] = sd;
     }
   }
   __syncthreads();
   const int k = threadIdx.x;
-  if (k < 2 * K + (M == 0 ? 3 : 0)) {
+  if (k < 2 * K + (M == 0 ? 4 : 0)) {
     double sum = 0.0;
     for (int v = 0; v < kMomT / kWave; ++v) sum += red[v][k];
     if (k < 2 * K) {
       double* dst = reinterpret_cast<double*>(&part[(size_t)blockIdx.x * kNM + tix(M + (k >> 1), M)]);
       dst[k & 1] = sum;
     } else {
-      dpart[(size_t)blockIdx.x * 3 + (k - 2 * K)] = sum;
+      dpart[(size_t)blockIdx.x * 4 + (k - 2 * K)] = sum;
     }
   }
 }
@@ -314,8 +389,9 @@ __global__ __launch_bounds__(kMomT) void lattice_moments_kernel(int64_t n, const
   moments_dispatch<0>((int)blockIdx.y, n, x, y, z, q, g, part, dpart, red);
 }
 
-// omega (and the dipole) from the per-block partials: one wave per entry,
-// lane j adds blocks j, j+64, ... in order, then a fixed butterfly.
+// omega (and the dipole and quadrupole trace) from the per-block partials:
+// one wave per entry, lane j adds blocks j, j+64, ... in order, then a fixed
+// butterfly.
 __global__ __launch_bounds__(kWave) void lattice_omega_kernel(const double2* __restrict__ part,
                                                              const double* __restrict__ dpart,
                                                              int nblk, double2* __restrict__ om,
@@ -329,7 +405,7 @@ __global__ __launch_bounds__(kWave) void lattice_omega_kernel(const double2* __r
       sr += v.x; si += v.y;
     }
   } else {
-    for (int b = lane; b < nblk; b += kWave) sr += dpart[(size_t)b * 3 + (e - kNM)];
+    for (int b = lane; b < nblk; b += kWave) sr += dpart[(size_t)b * 4 + (e - kNM)];
   }
   sr = wave_sum(sr);
   si = wave_sum(si);
@@ -339,9 +415,9 @@ __global__ __launch_bounds__(kWave) void lattice_omega_kernel(const double2* __r
   }
 }
 
-// Adds the fictitious charges (Kudin's extrinsic correction, :535-554): for
-// each periodic axis p a charge -D_p / |t_p| at origin + t_p, and their
-// negated sum at the origin; removes the monopole (:570-573) and forms
+// With the fictitious-charge correction (Kudin, :535-554) adds, for each
+// periodic axis p, a charge -D_p / |t_p| at origin + t_p, and their negated
+// sum at the origin, to omega; removes the monopole (:570-573) and forms
 // mu = M2L(ML, omega) for l <= kLM (the orders the field needs), four
 // threads per entry (j = part mod 4), their sums added in order.  One block.
 constexpr int kMuT = 1024;
@@ -363,18 +439,21 @@ __global__ __launch_bounds__(kMuT) void lattice_mu_kernel(const double2* __restr
   for (int k = t; k < kNT; k += kMuT) ml[k] = ml_g[k];
   if (t == 0) {
     int k = 0;
-    double sum = 0.0;
-    for (int p = 0; p < 3; ++p) {
-      if (!g.per[p]) continue;
-      const double* tv = g.lat + 3 * p;
-      for (int d = 0; d < 3; ++d) fpos[k][d] = g.origin[d] + tv[d];
-      fq[k] = -dip_g[p] / sqrt(tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2]);
-      sum += fq[k];
+    if (g.correction == kExtrinsicFictCharge) {
+      double sum = 0.0;
+      for (int p = 0; p < 3; ++p) {
+        if (!g.per[p]) continue;
+        const double* tv = g.lat + 3 * p;
+        for (int d = 0; d < 3; ++d) fpos[k][d] = g.origin[d] + tv[d];
+        fq[k] = -dip_g[p] / sqrt(tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2]);
+        sum += fq[k];
+        ++k;
+      }
+      for (int d = 0; d < 3; ++d) fpos[k][d] = g.origin[d];
+      fq[k] = -sum;
       ++k;
     }
-    for (int d = 0; d < 3; ++d) fpos[k][d] = g.origin[d];
-    fq[k] = -sum;
-    nf = k + 1;
+    nf = k;
   }
   __syncthreads();
   if (t < kNM) {
@@ -409,6 +488,7 @@ __global__ __launch_bounds__(kMuT) void lattice_mu_kernel(const double2* __restr
   }
   if (t < 4 * 4) fict[t] = (t / 4) < nf ? (t % 4 < 3 ? fpos[t / 4][t % 4] : fq[t / 4]) : 0.0;
   if (t == 0) fict[16] = (double)nf;
+  if (t >= 17 && t < kLatticeFict) fict[t] = dip_g[t - 17];   // box dipole, quadrupole trace
   __syncthreads();
   // Field coefficients per entry (l, m): the L00, L10 and L11 sums of the
   // field kernel are linear in O_lm = (c, s):
@@ -446,9 +526,11 @@ __global__ __launch_bounds__(kMuT) void lattice_mu_kernel(const double2* __restr
 //         - sum_{m>=2} conj(O_lm mu_{l+1,m-1}),
 // all linear in (Re, Im) O_lm, so each entry is eight multiply-adds against
 // the coefficients lattice_mu_kernel tabulates; O_lm column by column (order
-// m, l = m..kLM by the recurrence).  Then the fictitious charges of every
-// neighbour box (plain Coulomb pair, no softening; 1/r by reciprocal square
-// root).  Coefficients, recurrence constants, box vectors and charges are
+// m, l = m..kLM by the recurrence).  Then the extrinsic correction
+// (:807-829): Redlack-Grindlay adds E += 2 pf D and phi += -2 pf r.D + pf Q
+// (pf = 2 pi / 3V, D the box dipole, Q the quadrupole trace, r relative to
+// the centre); the fictitious charges of every neighbour box (plain Coulomb
+// pair, no softening; 1/r by reciprocal square root) otherwise, when set.  Coefficients, recurrence constants, box vectors and charges are
 // wave-uniform and come through the scalar cache.  Adds sign * (E, phi) to
 // the walk's results.
 constexpr int kFieldT = 256;
@@ -498,6 +580,14 @@ __global__ __launch_bounds__(kFieldT) void lattice_field_kernel(int64_t n, const
     }
   }
   double e0 = -l11r, e1 = -l11i, e2 = -l10;
+  if (g.correction == kExtrinsicRedlack) {
+    const double pf = 2.0 * 3.141592653589793 / (3.0 * g.volume);
+    const double dx = fict[17], dy = fict[18], dz = fict[19], qt = fict[20];
+    e0 += 2.0 * pf * dx;
+    e1 += 2.0 * pf * dy;
+    e2 += 2.0 * pf * dz;
+    phi += -2.0 * pf * (rx * dx + ry * dy + rz * dz) + pf * qt;
+  }
   const int nf = (int)fict[16];
   for (int fi = 0; fi < nf; ++fi) {
     const double fx = fict[4 * fi], fy = fict[4 * fi + 1], fz = fict[4 * fi + 2], fq = fict[4 * fi + 3];
@@ -585,7 +675,10 @@ void launch_constrain_periodic(hipStream_t s, int64_t n, double* x, double* y, d
 void launch_lattice_coefficients(hipStream_t s, const double* vec, int nb, double scale,
                                  const double* fact, double2* mstar, double2* lstar,
                                  int maxiter, double2* ml) {
-  lattice_terms_kernel<<<kNM + kNT, kTermCap, 0, s>>>(vec, nb, scale, fact, mstar, lstar);
+  if ((int64_t)(nb - 1) * nb <= kTermCap)   // one mirror layer: sorted sums as the reference
+    lattice_terms_kernel<<<kNM + kNT, kTermCap, 0, s>>>(vec, nb, scale, fact, mstar, lstar);
+  else
+    lattice_terms_dd_kernel<<<kNM + kNT, kTermDD, 0, s>>>(vec, nb, scale, fact, mstar, lstar);
   lattice_iterate_kernel<<<1, 1024, 0, s>>>(mstar, lstar, scale, maxiter, ml);
 }
 
@@ -605,7 +698,7 @@ void launch_lattice_sources(hipStream_t s, int64_t n, const double* x, const dou
                             double2* om, double* dip, double2* mu, double* fict, double* coef) {
   const int nblk = lattice_moment_blocks(n);
   lattice_moments_kernel<<<dim3(nblk, kLM + 1), kMomT, 0, s>>>(n, x, y, z, q, g, part, dpart);
-  lattice_omega_kernel<<<kNM + 3, kWave, 0, s>>>(part, dpart, nblk, om, dip);
+  lattice_omega_kernel<<<kNM + 4, kWave, 0, s>>>(part, dpart, nblk, om, dip);
   lattice_mu_kernel<<<1, kMuT, 0, s>>>(om, dip, g, ml, fact, mu, fict, coef);
 }
 

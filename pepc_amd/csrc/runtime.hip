@@ -1648,6 +1648,7 @@ void pepc_gpu_periodic_default(pepc_gpu_periodic* p) {
   p->mirror_box_layers = 1;
   p->t_lattice[0] = p->t_lattice[4] = p->t_lattice[8] = 1.0;
   p->far_field = 1;
+  p->extrinsic_correction = PEPC_GPU_EXTRINSIC_FICTCHARGE;
 }
 
 int pepc_gpu_set_periodic(void* ctx, const pepc_gpu_periodic* p) {
@@ -1676,8 +1677,15 @@ int pepc_gpu_set_periodic(void* ctx, const pepc_gpu_periodic* p) {
   const bool far = p->far_field != 0;
   if (far && !all3)
     return fail(c, PEPC_GPU_EINVAL, "set_periodic: the far field needs all three axes periodic");
-  if (far && layers != 1)
-    return fail(c, PEPC_GPU_EINVAL, "set_periodic: the far field supports mirror_box_layers = 1");
+  const int corr = p->extrinsic_correction;
+  if (far && corr != PEPC_GPU_EXTRINSIC_NONE && corr != PEPC_GPU_EXTRINSIC_REDLACK &&
+      corr != PEPC_GPU_EXTRINSIC_FICTCHARGE)
+    return fail(c, PEPC_GPU_EINVAL,
+                "set_periodic: extrinsic_correction must be NONE (0), REDLACK (1) or FICTCHARGE (2)");
+  g.correction = corr;
+  g.volume = std::fabs((g.lat[1] * g.lat[5] - g.lat[2] * g.lat[4]) * g.lat[6] +
+                       (g.lat[2] * g.lat[3] - g.lat[0] * g.lat[5]) * g.lat[7] +
+                       (g.lat[0] * g.lat[4] - g.lat[1] * g.lat[3]) * g.lat[8]);
   for (int d = 0; d < 3; ++d) {
     g.origin[d] = p->origin[d];
     g.center[d] = 0.5 * (g.lat[d] + g.lat[3 + d] + g.lat[6 + d]) + g.origin[d];
@@ -1704,8 +1712,8 @@ int pepc_gpu_set_periodic(void* ctx, const pepc_gpu_periodic* p) {
     ALLOC(c, c->fact, kLatticeFact);
     ALLOC(c, c->ml, kLatticeNT); ALLOC(c, c->mstar, kLatticeNM); ALLOC(c, c->lstar, kLatticeNT);
     ALLOC(c, c->mu, kLatticeNM); ALLOC(c, c->lpart, (int64_t)kLatticeMaxBlocks * kLatticeNM);
-    ALLOC(c, c->ldpart, (int64_t)kLatticeMaxBlocks * 3); ALLOC(c, c->fict, 17);
-    ALLOC(c, c->linv, kLatticeNM + kLatticeLM + 1); ALLOC(c, c->ldip, 3);
+    ALLOC(c, c->ldpart, (int64_t)kLatticeMaxBlocks * 4); ALLOC(c, c->fict, kLatticeFict);
+    ALLOC(c, c->linv, kLatticeNM + kLatticeLM + 1); ALLOC(c, c->ldip, 4);
     ALLOC(c, c->lom, kLatticeNM); ALLOC(c, c->lcoef, 8 * kLatticeNM);
     launch_lattice_inv(s, c->linv);
     f.resize(kLatticeFact);
@@ -1713,8 +1721,12 @@ int pepc_gpu_set_periodic(void* ctx, const pepc_gpu_periodic* p) {
     HIPCHECK(c, hipMemcpyAsync(c->fact, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice, s));
     auto nsq = [](const double* v) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; };
     // system_is_unit_cube (module_mirror_boxes.f90:248-255): the reference
-    // then loads its pretabulated coefficients (module_fmm_framework.f90:114-121)
-    const bool unit = g.simple && nsq(t) == 1.0 && nsq(t + 3) == 1.0 && nsq(t + 6) == 1.0;
+    // then loads its pretabulated coefficients (module_fmm_framework.f90:114-121).
+    // Those are the lattice beyond one mirror layer, so with more layers the
+    // coefficients are computed for the layers set (the reference loads the
+    // one-layer table for any layer count).
+    const bool unit = layers == 1 && g.simple && nsq(t) == 1.0 && nsq(t + 3) == 1.0 &&
+                      nsq(t + 6) == 1.0;
     if (unit) {
       mlh.assign(kLatticeNT, make_double2(0.0, 0.0));
       for (int k = 0; k < kUnitCubeEntries; ++k) {
@@ -1745,7 +1757,7 @@ int pepc_gpu_lattice_get(void* ctx, double* ml, double* mu, double* fict, int32_
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   if (ml) HIPCHECK(c, hipMemcpy(ml, c->ml, sizeof(double2) * kLatticeNT, hipMemcpyDeviceToHost));
   if (mu) HIPCHECK(c, hipMemcpy(mu, c->mu, sizeof(double2) * kLatticeNM, hipMemcpyDeviceToHost));
-  double f[17];
+  double f[kLatticeFict];
   if (fict || nfict) HIPCHECK(c, hipMemcpy(f, c->fict, sizeof(f), hipMemcpyDeviceToHost));
   if (fict) memcpy(fict, f, sizeof(double) * 16);
   if (nfict) *nfict = (int32_t)f[16];

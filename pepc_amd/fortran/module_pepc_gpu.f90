@@ -179,7 +179,7 @@ module pepc_gpu_c
    type, bind(C) :: pepc_gpu_periodic
       integer(c_int32_t) :: periodicity(3), mirror_box_layers
       real(c_double) :: t_lattice(9), origin(3)
-      integer(c_int32_t) :: far_field, pad
+      integer(c_int32_t) :: far_field, extrinsic_correction
    end type pepc_gpu_periodic
 
    type, bind(C) :: pepc_gpu_stats
@@ -529,6 +529,20 @@ contains
    end subroutine constrain_periodic
 end module module_mirror_boxes
 
+!> The far field's extrinsic-to-intrinsic correction, as the frontends set it
+!> (coulomb/module_fmm_framework.f90:42-49; pepc-b's setup.f90:141 selects
+!> Redlack-Grindlay).  MEASUREMENT is unimplemented in the reference and
+!> refused here.
+module module_fmm_framework
+   implicit none
+   private
+   integer, public, parameter :: FMM_EXTRINSIC_CORRECTION_NONE = 0
+   integer, public, parameter :: FMM_EXTRINSIC_CORRECTION_REDLACK = 1
+   integer, public, parameter :: FMM_EXTRINSIC_CORRECTION_FICTCHARGE = 2
+   integer, public, parameter :: FMM_EXTRINSIC_CORRECTION_MEASUREMENT = 3
+   integer, public :: fmm_extrinsic_correction = FMM_EXTRINSIC_CORRECTION_FICTCHARGE
+end module module_fmm_framework
+
 !> The frontend API of module_pepc.f90:32-56 on top of the GPU library.
 module module_pepc
    use iso_c_binding
@@ -539,6 +553,7 @@ module module_pepc
    use module_debug
    use module_interaction_specific
    use module_mirror_boxes
+   use module_fmm_framework, only: fmm_extrinsic_correction
    implicit none
    private
 
@@ -769,6 +784,7 @@ contains
          per%t_lattice(7:9) = t_lattice_3
          per%origin = LatticeOrigin
          per%far_field = merge(1, 0, include_far_field_if_periodic)
+         per%extrinsic_correction = fmm_extrinsic_correction
          call check(pepc_gpu_set_periodic(ctx, per), 'pepc_prepare (periodic)')
       end if
    end subroutine pepc_prepare

@@ -349,10 +349,11 @@ class Solver:
 
     # ------------------------------------------------------ periodic boundaries
     def set_periodic(self, periodicity=(1, 1, 1), lattice=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0),
-                     layers=1, far_field=True):
+                     layers=1, far_field=True, correction=_lib.EXTRINSIC_FICTCHARGE):
         """Periodic boundaries for the following traverses (periodicity=None
         clears them).  lattice: three box lengths (principal axes) or a 3x3
-        array whose rows are t_lattice_1..3."""
+        array whose rows are t_lattice_1..3; correction: the far field's
+        extrinsic-to-intrinsic correction (fmm_extrinsic_correction)."""
         if periodicity is None:
             self._check(self.lib.pepc_gpu_set_periodic(self.h, None))
             return
@@ -370,6 +371,7 @@ class Solver:
             p.t_lattice[k] = float(v)
         p.mirror_box_layers = int(layers)
         p.far_field = int(bool(far_field))
+        p.extrinsic_correction = int(correction)
         self._check(self.lib.pepc_gpu_set_periodic(self.h, C.byref(p)))
 
     def lattice(self):

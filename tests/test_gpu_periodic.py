@@ -1,6 +1,6 @@
 """Periodic boundaries on the GPU (through the C-ABI) against the oracle.
 
-Near field: the group walk over the 27 mirror boxes must reproduce the
+Near field: the group walk over the mirror boxes must reproduce the
 oracle's interaction counts per target exactly (MAC decisions on shifted
 targets) and its fields to 1e-12 in FP64.  Far field: the lattice
 coefficients are the reference's table bit for bit for the unit cube and
@@ -86,7 +86,8 @@ def _solve(gpu, s, precision, G=16, **per):
     return sol, st, res
 
 
-def _oracle(P, s, ml, box, origin, layers=1, far=True, periodicity=(1, 1, 1), G=16):
+def _oracle(P, s, ml, box, origin, layers=1, far=True, periodicity=(1, 1, 1), G=16,
+            correction=2):
     from oracle import oracle as orc
     d = orc.domain_decompose(s.x, s.y, s.z, s.q)
     t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
@@ -94,7 +95,7 @@ def _oracle(P, s, ml, box, origin, layers=1, far=True, periodicity=(1, 1, 1), G=
     w = P.walk_group_boxes(d.x, d.y, d.z, t, s.theta2, s.eps2, vb, tile_size=G, keys=d.keys)
     lat = None
     if far:
-        lat = P.Lattice(ml, box, origin)
+        lat = P.Lattice(ml, box, origin, layers=layers, correction=correction)
         lat.timestep(d.x, d.y, d.z, d.q)
         e, p = lat.field(d.x, d.y, d.z)
         w.e = w.e + e
@@ -132,6 +133,103 @@ def test_periodic_solve_fp64_parity(gpu, P, table, name):
           f"max|dE|/max|E|={np.abs(e - w.e).max() / scale:.2e}", flush=True)
     assert np.abs(e - w.e).max() <= 1e-11 * scale
     assert np.abs(pot - w.pot).max() <= 1e-11 * np.abs(w.pot).max()
+    sol.close()
+
+
+@pytest.mark.parametrize("name", ["unit", "box2"])
+@pytest.mark.parametrize("correction", [0, 1])
+def test_extrinsic_corrections_fp64_parity(gpu, P, table, name, correction):
+    """fmm_extrinsic_correction NONE (0) and REDLACK (1) (module_fmm_framework.
+    f90:807-829) against the oracle under the same setting: counts exact, the
+    field to 1e-11, the potential to 1e-11 in the unit cube and up to a
+    constant in a computed lattice (the constant carries the l = 1 lattice
+    row's rounding noise times the box dipole, which the two computations do
+    not share)."""
+    c = CASES[name]
+    s = workloads.periodic(n=c["n"], seed=38, box=c["box"], origin=c["origin"])
+    sol, st, res = _solve(gpu, s, "fp64", lattice=c["box"], origin=c["origin"],
+                          correction=correction)
+    ml = sol.lattice()[0] if name != "unit" else table
+    d, w, lat = _oracle(P, s, ml, c["box"], c["origin"], correction=correction)
+    wk = sol.read_walk()
+    assert np.array_equal(wk["p2p"].astype(np.int32), w.p2p)
+    assert np.array_equal(wk["m2p"].astype(np.int32), w.m2p)
+    _, _, fict = sol.lattice()
+    assert len(fict) == 0 and len(lat.step.fict) == 0
+    e = res.e.cpu().numpy().T
+    pot = res.pot.cpu().numpy()
+    scale = np.abs(w.e).max()
+    print(f"[correction {correction} {name}] max|dE|/max|E|={np.abs(e - w.e).max() / scale:.2e}",
+          flush=True)
+    assert np.abs(e - w.e).max() <= 1e-11 * scale
+    dp = pot - w.pot
+    if name == "unit":
+        assert np.abs(dp).max() <= 1e-11 * np.abs(w.pot).max()
+    else:
+        assert np.abs(dp - dp.mean()).max() <= 1e-11 * np.abs(w.pot).max()
+    sol.close()
+
+
+def test_redlack_equals_ewald_on_gpu(gpu, P):
+    """theta = 0, unit cube, Redlack-Grindlay: the GPU sum is the tin-foil
+    Ewald sum itself, field and potential (no constant), to the scheme's
+    truncation at multipole order 20."""
+    s = workloads.periodic(n=64, seed=34, eps2=0.0, theta2=0.0)
+    sol, st, res = _solve(gpu, s, "fp64", correction=1)
+    keys, perm, xs, ys, zs, qs = sol.sorted_host()
+    e = res.e.cpu().numpy().T
+    pot = res.pot.cpu().numpy()
+    e_ew, p_ew = P.ewald(xs, ys, zs, qs)
+    rms = math.sqrt((e_ew ** 2).mean())
+    prms = math.sqrt((p_ew ** 2).mean())
+    print(f"[ewald redlack] max|dE|/rms {np.abs(e - e_ew).max() / rms:.2e} "
+          f"max|dphi|/rms {np.abs(pot - p_ew).max() / prms:.2e}", flush=True)
+    assert np.abs(e - e_ew).max() <= 1e-5 * rms
+    assert np.abs(pot - p_ew).max() <= 1e-5 * prms
+    sol.close()
+
+
+def test_two_mirror_layers_far_field(gpu, P):
+    """mirror_box_layers = 2 with the far field (ws = 2): the coefficients
+    computed on the GPU (double-double sums of the 124 x 125 L* terms per
+    entry) against the oracle's sorted sums to 1e-11 of the row scale (row 1
+    excluded, rounding noise); the solve over 125 mirror boxes against the
+    oracle (counts exact, field 1e-11, potential 1e-11 up to a constant) with
+    the GPU's coefficients; and, at
+    theta = 0, the GPU output against the tin-foil Ewald sum (field 1e-7 of
+    its rms, potential up to -(2 pi / 3V) sum q r^2)."""
+    ref = P.lattice_coefficients(layers=2)
+    s = workloads.periodic(n=2000, seed=39)
+    sol, st, res = _solve(gpu, s, "fp64", layers=2)
+    ml, _, _ = sol.lattice()
+    _rows_close(ml, ref, [0] + list(range(2, 41)), 1e-11, 40)
+    d, w, lat = _oracle(P, s, ml, (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), layers=2)
+    wk = sol.read_walk()
+    assert np.array_equal(wk["p2p"].astype(np.int32), w.p2p)
+    assert np.array_equal(wk["m2p"].astype(np.int32), w.m2p)
+    e = res.e.cpu().numpy().T
+    pot = res.pot.cpu().numpy()
+    scale = np.abs(w.e).max()
+    dp = pot - w.pot
+    print(f"[two layers] p2p={w.p2p.sum()} m2p={w.m2p.sum()} "
+          f"max|dE|/max|E|={np.abs(e - w.e).max() / scale:.2e} dphi const {dp.mean():.2e}", flush=True)
+    assert np.abs(e - w.e).max() <= 1e-11 * scale
+    # up to a constant: the l = 1 lattice row's rounding noise times the
+    # (rounding-level) remaining dipole of the box
+    assert np.abs(dp - dp.mean()).max() <= 1e-11 * np.abs(w.pot).max()
+    sol.close()
+    s = workloads.periodic(n=64, seed=34, eps2=0.0, theta2=0.0)
+    sol, st, res = _solve(gpu, s, "fp64", layers=2)
+    keys, perm, xs, ys, zs, qs = sol.sorted_host()
+    e = res.e.cpu().numpy().T
+    pot = res.pot.cpu().numpy()
+    e_ew, p_ew = P.ewald(xs, ys, zs, qs)
+    rms = math.sqrt((e_ew ** 2).mean())
+    print(f"[two layers ewald] max|dE|/rms {np.abs(e - e_ew).max() / rms:.2e}", flush=True)
+    assert np.abs(e - e_ew).max() <= 1e-7 * rms
+    r2 = (xs - 0.5) ** 2 + (ys - 0.5) ** 2 + (zs - 0.5) ** 2
+    shift = -2 * math.pi / 3 * (qs * r2).sum()
+    assert np.abs(pot - p_ew - shift).max() <= 1e-7 * math.sqrt((p_ew ** 2).mean())
     sol.close()
 
 
@@ -262,7 +360,7 @@ def test_periodic_errors(gpu):
     with pytest.raises(PepcGpuError):
         sol.set_periodic(periodicity=(1, 1, 0))                  # far field needs 3-D
     with pytest.raises(PepcGpuError):
-        sol.set_periodic(layers=2)                               # far field: one layer
+        sol.set_periodic(correction=3)                           # MEASUREMENT: not in the reference
     with pytest.raises(PepcGpuError):
         sol.set_periodic(layers=4, far_field=False)
     with pytest.raises(PepcGpuError):

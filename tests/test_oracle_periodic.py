@@ -129,6 +129,77 @@ def test_lattice_sum_equals_ewald(table, box, origin):
     assert abs(u - u_ew) <= 1e-7 * abs(u_ew)
 
 
+def _dipole_terms(x, y, z, q, box, origin):
+    c = origin + 0.5 * box
+    r = np.stack([x - c, y - c, z - c], 1)
+    return r, (q[:, None] * r).sum(0), (q * (r * r).sum(1)).sum(), box ** 3
+
+
+@pytest.mark.parametrize("box,origin", [(1.0, 0.0), (2.0, -1.0)])
+def test_extrinsic_corrections_against_ewald(table, box, origin):
+    """The three extrinsic-to-intrinsic corrections of the reference
+    (module_fmm_framework.f90:42-46, 807-829) against the tin-foil Ewald sum,
+    theta = 0, one mirror layer.  Redlack-Grindlay (E += 2 pf D, phi += -2 pf
+    r.D + pf Q, pf = 2 pi / 3V) is the tin-foil sum itself: the field to 1e-7
+    of its rms; the potential too in the unit cube, whose tabulated l = 1
+    lattice row is zero, and up to a constant in a computed lattice (the
+    reference's l = 1 row holds rounding noise, which multiplies the box
+    dipole in the constant term).  No correction is the extrinsic sum, which
+    differs from the tin-foil one by closed forms: E - (4 pi / 3V) D and
+    phi + (4 pi / 3V) r.D - pf Q.  A dropped factor 2 or a sign in either
+    correction fails these by O(1)."""
+    x, y, z, q = _neutral(48, 3, box, origin)
+    ml = table if box == 1.0 else P.lattice_coefficients(lattice=(box, box, box))
+    e_ew, p_ew = P.ewald(x - origin, y - origin, z - origin, q, box=box)
+    rms = math.sqrt((e_ew ** 2).mean())
+    prms = math.sqrt((p_ew ** 2).mean())
+    r, D, Q, V = _dipole_terms(x, y, z, q, box, origin)
+    pf = 2 * math.pi / (3 * V)
+    kw = dict(lattice=(box,) * 3, origin=(origin,) * 3)
+    d, t, w = P.solve_periodic(x, y, z, q, 0.0, 0.0, ml, correction=P.EXTRINSIC_REDLACK, **kw)
+    e, p = _input_order(d, w)
+    assert np.abs(e - e_ew).max() <= 1e-7 * rms
+    dev = p - p_ew
+    if box == 1.0:
+        assert np.abs(dev).max() <= 1e-7 * prms
+    else:
+        assert np.abs(dev - dev.mean()).max() <= 1e-7 * prms
+    d, t, w = P.solve_periodic(x, y, z, q, 0.0, 0.0, ml, correction=P.EXTRINSIC_NONE, **kw)
+    e, p = _input_order(d, w)
+    assert np.abs(e - (e_ew - 2 * pf * D)).max() <= 1e-7 * rms
+    dev = p - (p_ew + 2 * pf * (r @ D) - pf * Q)
+    if box == 1.0:
+        assert np.abs(dev).max() <= 1e-7 * prms
+    else:
+        assert np.abs(dev - dev.mean()).max() <= 1e-7 * prms
+
+
+@pytest.mark.slow
+def test_two_mirror_layers_against_ewald():
+    """mirror_box_layers = 2 (ws = 2 in the far field, module_fmm_framework.
+    f90:111, 994): 125 mirror walks at theta = 0 plus the lattice beyond
+    (coefficients with the 5x-scaled L* terms, 124 x 125 per entry) equal the
+    tin-foil Ewald sum, with the fictitious charges (field 1e-9 of its rms,
+    potential up to -pf Q to 1e-9) and with Redlack-Grindlay (field 1e-9,
+    potential up to a constant)."""
+    ml2 = P.lattice_coefficients(layers=2)
+    x, y, z, q = _neutral(48, 3)
+    e_ew, p_ew = P.ewald(x, y, z, q)
+    rms = math.sqrt((e_ew ** 2).mean())
+    prms = math.sqrt((p_ew ** 2).mean())
+    r, D, Q, V = _dipole_terms(x, y, z, q, 1.0, 0.0)
+    pf = 2 * math.pi / (3 * V)
+    d, t, w = P.solve_periodic(x, y, z, q, 0.0, 0.0, ml2, layers=2)
+    e, p = _input_order(d, w)
+    assert np.abs(e - e_ew).max() <= 1e-9 * rms
+    assert np.abs(p - (p_ew - pf * Q)).max() <= 1e-9 * prms
+    d, t, w = P.solve_periodic(x, y, z, q, 0.0, 0.0, ml2, layers=2, correction=P.EXTRINSIC_REDLACK)
+    e, p = _input_order(d, w)
+    assert np.abs(e - e_ew).max() <= 1e-9 * rms
+    dev = p - p_ew
+    assert np.abs(dev - dev.mean()).max() <= 1e-7 * prms
+
+
 def test_rock_salt_madelung(table):
     x, y, z, q = P.rock_salt(1)
     x, y, z = x + 0.125, y + 0.125, z + 0.125      # no ion on a box corner
