@@ -41,7 +41,8 @@ struct DevScalars {
   uint32_t ngroups;
   uint32_t dup_scan_n;          // particles the duplicate fix-up scans (0 without duplicates)
   uint32_t maxlevel_used;
-  uint32_t level_offset[kMaxLevel + 2];
+  uint32_t level_offset[kMaxLevel + 2];   // first node of every level; [kMaxLevel + 1] = nnodes
+  uint32_t twig_offset[kMaxLevel + 2];    // first twig rank of every level; [kMaxLevel + 1] = ntwig
   unsigned long long interactions;
   unsigned long long mac_evals;
   unsigned long long p2p;
@@ -101,6 +102,24 @@ __device__ __forceinline__ double wave_min(double v) {
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t t = (uint32_t)__shfl_xor((int)v, o, kWave);
+    v = t < v ? t : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int wave_max_i32(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int t = __shfl_xor(v, o, kWave);
+    v = t > v ? t : v;
+  }
   return v;
 }
 

@@ -74,12 +74,8 @@ struct TreeBuffers {
   const double *x, *y, *z, *q;       // sorted particles
   uint8_t* lo;                       // L(a): level of the first node of particle a
   uint8_t* hi;                       // leaf level of particle a
-  uint32_t* cnt;                     // nodes per particle -> DFS offsets
-  unsigned long long* dkey[2];       // level keys (sort ping-pong)
-  uint32_t* dval[2];                 // DFS indices (sort ping-pong)
-  uint32_t* dpart;                   // DFS node -> particle
-  uint32_t* inv;                     // DFS index -> BFS index
-  uint32_t* flag;                    // twig flags -> twig ranks
+  uint32_t* cnt;                     // per-particle scratch (group marks -> group ranks)
+  uint32_t* tab;                     // (level, tile) node and twig counts, bfs_table_words(n)
   uint32_t* fc_rank;                 // first child by twig rank (+1 sentinel)
   uint32_t* node_rank;               // node index by twig rank
   uint32_t* nfirst;                  // first particle of each node
@@ -112,6 +108,7 @@ void launch_dupfix(hipStream_t s, int64_t n, unsigned long long* keys, double* x
                    long long* kd, unsigned long long* oldkeys, uint32_t* dflag, uint32_t* dlist,
                    uint32_t* bsum, DevScalars* sc);
 void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size, int group_pack);
+int64_t bfs_table_words(int64_t n);
 void launch_leaf_moments(hipStream_t s, const TreeBuffers& t);
 // Tree consistency counters res[7] (zeroed by the caller), see tree.hip.
 void launch_tree_check(hipStream_t s, const NodeLink* link, const uint32_t* parent, int64_t n,

@@ -46,9 +46,8 @@ struct Ctx {
   uint32_t *np2p = nullptr, *nm2p = nullptr;
   double *res[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // host-path outputs
   // nodes
-  unsigned long long *dkey[2] = {nullptr, nullptr};
-  uint32_t *dval[2] = {nullptr, nullptr};
-  uint32_t *dpart = nullptr, *inv = nullptr, *flag = nullptr, *fc_rank = nullptr,
+  uint32_t* tab = nullptr;          // BFS numbering table, bfs_table_words(ncap) words
+  uint32_t *fc_rank = nullptr,
            *node_rank = nullptr, *nfirst = nullptr, *nparent = nullptr;
   NodeLink* link = nullptr;
   NodeGeom* geom = nullptr;
@@ -242,8 +241,8 @@ void free_all(Ctx* c) {
   void* ptrs[] = {c->in[0], c->in[1], c->in[2], c->in[3], c->k0, c->k1, c->oldkeys, c->v0, c->v1,
                   c->xs, c->ys, c->zs, c->qs, c->lo, c->hi, c->cnt, c->gmark, c->dflag, c->dlist,
                   c->kd, c->gfirst, c->gcount, c->gstat, c->queue, c->np2p, c->nm2p, c->res[0], c->res[1],
-                  c->res[2], c->res[3], c->res[4], c->dkey[0], c->dkey[1], c->dval[0], c->dval[1],
-                  c->dpart, c->inv, c->flag, c->fc_rank, c->node_rank, c->nfirst, c->link, c->geom,
+                  c->res[2], c->res[3], c->res[4], c->tab,
+                  c->fc_rank, c->node_rank, c->nfirst, c->link, c->geom,
                   c->mom64, c->wm32, c->wm64, c->hist, c->bsum, c->partial, c->box, c->sc,
                   c->sys, c->soff, c->sysroot, c->label, c->tlabel, c->ehist, c->ecount,
                   c->nadj, c->parent, c->leafof, c->aqueue, c->lcount, c->loff, c->lent,
@@ -281,17 +280,15 @@ int alloc_nodes(Ctx* c, int64_t mc) {
   if ((double)mc >= 4294967295.0)
     return fail(c, PEPC_GPU_EINVAL, "node pool exceeds 32-bit node indices");
   const int64_t nc = c->ncap;
-  ALLOC(c, c->dkey[0], mc); ALLOC(c, c->dkey[1], mc);
-  ALLOC(c, c->dval[0], mc); ALLOC(c, c->dval[1], mc);
-  ALLOC(c, c->dpart, mc); ALLOC(c, c->inv, mc); ALLOC(c, c->flag, mc);
   ALLOC(c, c->fc_rank, mc + 1); ALLOC(c, c->node_rank, mc); ALLOC(c, c->nfirst, mc);
   ALLOC(c, c->nparent, mc);
   ALLOC(c, c->link, mc); ALLOC(c, c->geom, mc); ALLOC(c, c->mom64, mc * 16);
   ALLOC(c, c->wm32, mc); ALLOC(c, c->wm64, mc);
-  // radix histogram: 256 digits x sort blocks of the largest array sorted
-  const int64_t hc = 256 * sort_blocks(mc > nc ? mc : nc);
+  // radix histogram: 256 digits x sort blocks of the particle arrays; scan
+  // blocks for the largest array scanned (particle-sized)
+  const int64_t hc = 256 * sort_blocks(nc);
   ALLOC(c, c->hist, hc);
-  int64_t bc = scan_blocks(hc > mc ? hc : mc) + 64;
+  int64_t bc = scan_blocks(hc > nc ? hc : nc) + 64;
   if (bc < 512) bc = 512;   // also holds the 256 digit totals of a radix pass
   ALLOC(c, c->bsum, bc);
   c->mcap = mc;
@@ -320,6 +317,7 @@ int ensure_particles(Ctx* c, int64_t n) {
   ALLOC(c, c->gfirst, nc); ALLOC(c, c->gcount, nc); ALLOC(c, c->gstat, 8 * nc);
   ALLOC(c, c->sys, nc);
   ALLOC(c, c->np2p, nc); ALLOC(c, c->nm2p, nc);
+  ALLOC(c, c->tab, bfs_table_words(nc));
   c->ncap = nc;
   int r = alloc_nodes(c, mc);
   if (r) return r;
@@ -429,9 +427,10 @@ int grow_device_impl(Ctx* c, int64_t n, const double* x, const double* y, const 
   launch_sort_keys(s, c->k0, c->v0, c->k1, c->v1, n, c->hist, c->bsum, sys, nsys);
   launch_dup_detect(s, n, c->k0, sys, c->sc);
   // equal keys into the reference presort's order (the sort's ping-pong
-  // buffers and the build's node-sized index buffers are free here)
-  launch_tie_order(s, n, nsys, c->soff, c->k0, c->v0, c->k1, c->v1, c->dval[0],
-                   reinterpret_cast<uint16_t*>(c->dval[1]), c->sc);
+  // buffers and the group scratch arrays, cleared before the groups use
+  // them, are free here)
+  launch_tie_order(s, n, nsys, c->soff, c->k0, c->v0, c->k1, c->v1, c->cnt,
+                   reinterpret_cast<uint16_t*>(c->gmark), c->sc);
   launch_gather_rec(s, n, c->v0, c->prec4, c->xs, c->ys, c->zs, c->qs);
   launch_dupfix(s, n, c->k0, c->xs, c->ys, c->zs, c->box, sys, nsys > 1 ? c->soff : nullptr, c->kd,
                 c->oldkeys, c->dflag, c->dlist, c->bsum, c->sc);
@@ -441,9 +440,7 @@ int grow_device_impl(Ctx* c, int64_t n, const double* x, const double* y, const 
   t.keys = c->k0;
   t.x = c->xs; t.y = c->ys; t.z = c->zs; t.q = c->qs;
   t.lo = c->lo; t.hi = c->hi; t.cnt = c->cnt;
-  t.dkey[0] = c->dkey[0]; t.dkey[1] = c->dkey[1];
-  t.dval[0] = c->dval[0]; t.dval[1] = c->dval[1];
-  t.dpart = c->dpart; t.inv = c->inv; t.flag = c->flag; t.fc_rank = c->fc_rank;
+  t.tab = c->tab; t.fc_rank = c->fc_rank;
   t.node_rank = c->node_rank; t.nfirst = c->nfirst; t.link = c->link; t.geom = c->geom;
   t.parent = c->nparent;
   t.mom64 = c->mom64; t.wm32 = c->wm32; t.wm64 = c->wm64;

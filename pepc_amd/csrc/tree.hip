@@ -6,11 +6,14 @@
 // first level at which particles a-1 and a fall into different cells (0 at
 // both ends).  Particle a then owns the nodes of levels L(a)..max(L(a),L(a+1)):
 // twigs down to the level where it separates from its right neighbour, and
-// its leaf at the deepest of those levels.  Numbering the nodes particle by
-// particle gives the reference's pre-order (insert_helper provisions a twig,
-// then its children in key order).  One stable counting pass keyed on the
-// level turns that into a level-by-level (BFS) order in which the children of
-// every twig are contiguous; the k-th run of children belongs to the k-th twig.
+// its leaf at the deepest of those levels (the reference's insert_helper
+// creates exactly these, a twig before its children in key order).  Nodes
+// are stored level by level (BFS), within a level in particle order, so the
+// children of every twig are contiguous and the k-th run of children belongs
+// to the k-th twig.  A node's BFS index is the number of nodes at lower
+// levels plus the number of particles before its owner that own a node at
+// its level: per-block counts per level, one scan over the (level, block)
+// table, and an emit pass that writes every node to its final place.
 //
 // Upward pass (tree_node_update_from_children, :762-821, with
 // shift_multipoles_up, coulomb/module_interaction_specific.f90:84-140): one
@@ -182,7 +185,7 @@ __device__ __forceinline__ bool tree_broken(const DevScalars* sc) {
 __global__ void levels_kernel(int64_t n, const unsigned long long* __restrict__ keys,
                               const uint32_t* __restrict__ sys,
                               uint8_t* __restrict__ lo, uint8_t* __restrict__ hi,
-                              uint32_t* __restrict__ cnt, DevScalars* __restrict__ sc) {
+                              DevScalars* __restrict__ sc) {
   for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < n;
        a += (int64_t)gridDim.x * blockDim.x) {
     int la = 0, lb = 0;
@@ -200,102 +203,179 @@ __global__ void levels_kernel(int64_t n, const unsigned long long* __restrict__ 
     const int lam = la > lb ? la : lb;
     lo[a] = (uint8_t)la;
     hi[a] = (uint8_t)lam;
-    cnt[a] = (uint32_t)(lam - la + 1);
   }
 }
 
-__global__ void dfs_emit_kernel(int64_t n, int64_t cap, const uint8_t* __restrict__ lo,
-                                const uint8_t* __restrict__ hi, const uint32_t* __restrict__ pre,
-                                unsigned long long* __restrict__ dkey, uint32_t* __restrict__ dval,
-                                uint32_t* __restrict__ dpart, DevScalars* __restrict__ sc) {
-  if ((int64_t)sc->nnodes > cap) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&sc->flags, kFlagNodeOverflow);
-    return;
-  }
-  if (tree_broken(sc)) return;
-  for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < n;
-       a += (int64_t)gridDim.x * blockDim.x) {
-    const int la = lo[a], lam = hi[a];
-    const uint32_t base = pre[a];
-    for (int l = la; l <= lam; ++l) {
-      const uint32_t d = base + (uint32_t)(l - la);
-      dkey[d] = (unsigned long long)l;
-      dval[d] = d;
-      dpart[d] = (uint32_t)a;
+// ---- BFS node numbering
+// Tiles of kBfsTile sorted particles; wave w of a tile owns the kBfsWave
+// consecutive particles from base + w kBfsWave and takes them 64 at a time,
+// one per lane, so the nodes one pass writes for a level are consecutive.
+// tab[l * nb + b] counts the nodes of level l owned by tile b's particles,
+// tab[kLevels * nb + l * nb + b] the twigs among them; after the exclusive
+// scan (level-major) the entries are the BFS index of the tile's first node
+// of that level and the twig rank of its first twig there.
+constexpr int kBfsTile = 4096;
+constexpr int kBfsWaves = kT / kWave;           // 4
+constexpr int kBfsWave = kBfsTile / kBfsWaves;  // 1024 particles per wave
+constexpr int kLevels = kMaxLevel + 1;
+
+// The wave's node and twig counts per level over its particles: lane l
+// accumulates level l (levels < 21 <= 64 lanes).
+__device__ __forceinline__ void bfs_wave_counts(int64_t n, int64_t base,
+                                                const uint8_t* __restrict__ lo,
+                                                const uint8_t* __restrict__ hi, uint32_t& cn,
+                                                uint32_t& ct) {
+  const int lane = lane_id();
+  cn = 0;
+  ct = 0;
+  for (int it = 0; it < kBfsWave / kWave; ++it) {
+    const int64_t a = base + (int64_t)it * kWave + lane;
+    const bool valid = a < n;
+    const int l0 = valid ? lo[a] : kLevels, l1 = valid ? hi[a] : -1;
+    const int lmin = (int)wave_min_u32((uint32_t)l0), lmax = (int)wave_max_i32(l1);
+    for (int l = lmin; l <= lmax; ++l) {
+      const uint32_t c = (uint32_t)popc64(__ballot(l0 <= l && l <= l1));
+      const uint32_t d = (uint32_t)popc64(__ballot(l0 <= l && l < l1));
+      if (lane == l) { cn += c; ct += d; }
     }
   }
 }
 
-// First node of every level from the level pass's digit totals.
-__global__ void level_offsets_kernel(const uint32_t* __restrict__ dtot,
-                                     DevScalars* __restrict__ sc) {
-  const int l = threadIdx.x;
-  const bool broken = tree_broken(sc);
+__global__ __launch_bounds__(kT) void bfs_count_kernel(int64_t n, const uint8_t* __restrict__ lo,
+                                                      const uint8_t* __restrict__ hi,
+                                                      uint32_t* __restrict__ tab, int64_t nb,
+                                                      const DevScalars* __restrict__ sc) {
+  if (tree_broken(sc)) return;
+  __shared__ uint32_t wn[kBfsWaves][kLevels], wt[kBfsWaves][kLevels];
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  uint32_t cn, ct;
+  bfs_wave_counts(n, (int64_t)blockIdx.x * kBfsTile + (int64_t)w * kBfsWave, lo, hi, cn, ct);
+  if (lane < kLevels) { wn[w][lane] = cn; wt[w][lane] = ct; }
   __syncthreads();
-  if (l <= kMaxLevel + 1) {
-    uint32_t off = 0;
-    for (int k = 0; k < l; ++k) off += dtot[k];
-    if (l == kMaxLevel + 1) off = sc->nnodes;
-    sc->level_offset[l] = broken ? 0u : off;
+  if (threadIdx.x < kLevels) {
+    const int l = threadIdx.x;
+    uint32_t a = 0, b = 0;
+    for (int k = 0; k < kBfsWaves; ++k) { a += wn[k][l]; b += wt[k][l]; }
+    tab[(int64_t)l * nb + blockIdx.x] = a;
+    tab[(int64_t)(kLevels + l) * nb + blockIdx.x] = b;
   }
 }
 
-__global__ void inv_kernel(const uint32_t* __restrict__ dval, uint32_t* __restrict__ inv,
-                           const DevScalars* __restrict__ sc) {
-  if (tree_broken(sc)) return;
-  const int64_t m = sc->nnodes;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
-       i += (int64_t)gridDim.x * blockDim.x)
-    inv[dval[i]] = (uint32_t)i;
-}
+// Exclusive scans of the node table (block 0) and of the twig table
+// (block 1), level-major, in place; the totals and per-level starts go to
+// the scalars (node count, first node of every level; twig count, first
+// twig rank of every level).  A node count above the pool raises the
+// overflow flag before any node is written.
+constexpr int kBfsScanT = 1024;
 
-// Leaves carry no moment record: a leaf's multipole is its particle
-// (multipole_from_particle, coulomb/module_interaction_specific.f90:67-79), so
-// parents and the walk read the sorted particle arrays directly.  Only the
-// leaf's geometry record (centre = particle position) is written here.
-__global__ void node_fill_kernel(int64_t cap, const unsigned long long* __restrict__ dkey,
-                                 const uint32_t* __restrict__ dval, const uint32_t* __restrict__ dpart,
-                                 const uint32_t* __restrict__ inv, const uint8_t* __restrict__ hi,
-                                 const double* __restrict__ x, const double* __restrict__ y,
-                                 const double* __restrict__ z, const double* __restrict__ q,
-                                 NodeGeom* __restrict__ geom,
-                                 NodeLink* __restrict__ link, uint32_t* __restrict__ nfirst,
-                                 uint32_t* __restrict__ flag, const DevScalars* __restrict__ sc) {
-  const int64_t m = tree_broken(sc) ? 0 : (int64_t)sc->nnodes;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t d = dval[i];
-    const uint32_t l = (uint32_t)dkey[i];
-    const uint32_t a = dpart[d];
-    NodeLink lk;
-    lk.level = l;
-    nfirst[i] = a;
-    if (l == hi[a]) {          // leaf
-      lk.first = a; lk.nchild = 0; lk.count = 1;
-      flag[i] = 0;
-      NodeGeom g;   // a leaf has no acceptance test: its b2 slot holds the charge
-      g.cx = x[a]; g.cy = y[a]; g.cz = z[a]; g.b2 = q[a];
-      geom[i] = g;
-    } else {                   // twig: first child is this particle's next node
-      lk.first = inv[d + 1]; lk.nchild = 0xFFFFFFFFu; lk.count = 0;
-      flag[i] = 1;
+__global__ __launch_bounds__(kBfsScanT) void bfs_scan_kernel(uint32_t* __restrict__ tab, int64_t nb,
+                                                            int64_t cap, DevScalars* __restrict__ sc) {
+  if (tree_broken(sc)) return;
+  const bool twigs = blockIdx.x == 1;
+  uint32_t* t = tab + (twigs ? (int64_t)kLevels * nb : 0);
+  const int64_t len = (int64_t)kLevels * nb;
+  const int64_t per = (len + kBfsScanT - 1) / kBfsScanT;
+  const int64_t b0 = (int64_t)threadIdx.x * per;
+  uint32_t sum = 0;
+  for (int64_t k = 0; k < per; ++k)
+    if (b0 + k < len) sum += t[b0 + k];
+  // block exclusive scan of the per-thread sums
+  __shared__ uint32_t ws[kBfsScanT / kWave];
+  const int w = threadIdx.x / kWave;
+  const uint32_t inc = wave_inclusive_scan(sum);
+  if (lane_id() == kWave - 1) ws[w] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (int k = 0; k < kBfsScanT / kWave; ++k) {
+    off += k < w ? ws[k] : 0u;
+    tot += ws[k];
+  }
+  uint32_t run = off + inc - sum;
+  for (int64_t k = 0; k < per; ++k) {
+    if (b0 + k < len) {
+      const uint32_t v = t[b0 + k];
+      t[b0 + k] = run;
+      run += v;
     }
-    link[i] = lk;
+  }
+  __syncthreads();
+  if (threadIdx.x <= kLevels) {
+    const int l = threadIdx.x;
+    const uint32_t start = l < kLevels ? t[(int64_t)l * nb] : tot;
+    if (twigs) sc->twig_offset[l] = start;
+    else sc->level_offset[l] = start;
+  }
+  if (threadIdx.x == 0) {
+    if (twigs) {
+      sc->ntwig = tot;
+    } else {
+      sc->nnodes = tot;
+      if ((int64_t)tot > cap) atomicOr(&sc->flags, kFlagNodeOverflow);
+    }
   }
 }
 
-__global__ void twig_rank_kernel(const NodeLink* __restrict__ link, const uint32_t* __restrict__ rank,
-                                 uint32_t* __restrict__ fc_rank, uint32_t* __restrict__ node_rank,
-                                 const DevScalars* __restrict__ sc) {
+// Every node to its BFS place.  The wave recounts its particles per level
+// for its start within the tile, then passes over them 64 at a time, levels
+// deepest first so that a twig's first child (its owner's node one level
+// down) was numbered just before it; the lanes owning a level take
+// consecutive indices (ballot ranks).  Leaves get their geometry record
+// (centre = particle, last slot = charge); twigs their rank-indexed first
+// child and node index.
+__global__ __launch_bounds__(kT) void bfs_emit_kernel(
+    int64_t n, const uint8_t* __restrict__ lo, const uint8_t* __restrict__ hi,
+    const uint32_t* __restrict__ tab, int64_t nb, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ q,
+    NodeGeom* __restrict__ geom, NodeLink* __restrict__ link, uint32_t* __restrict__ nfirst,
+    uint32_t* __restrict__ fc_rank, uint32_t* __restrict__ node_rank,
+    const DevScalars* __restrict__ sc) {
   if (tree_broken(sc)) return;
-  const int64_t m = sc->nnodes;
-  if (blockIdx.x == 0 && threadIdx.x == 0) fc_rank[sc->ntwig] = (uint32_t)m;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    if (link[i].nchild != 0u) {
-      const uint32_t r = rank[i];
-      fc_rank[r] = link[i].first;
-      node_rank[r] = (uint32_t)i;
+  __shared__ uint32_t wn[kBfsWaves][kLevels], wt[kBfsWaves][kLevels];
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  const int64_t base = (int64_t)blockIdx.x * kBfsTile + (int64_t)w * kBfsWave;
+  uint32_t rn, rt;   // lane l: next node index / twig rank of level l for this wave
+  bfs_wave_counts(n, base, lo, hi, rn, rt);
+  if (lane < kLevels) { wn[w][lane] = rn; wt[w][lane] = rt; }
+  __syncthreads();
+  if (lane < kLevels) {
+    rn = tab[(int64_t)lane * nb + blockIdx.x];
+    rt = tab[(int64_t)(kLevels + lane) * nb + blockIdx.x];
+    for (int k = 0; k < w; ++k) { rn += wn[k][lane]; rt += wt[k][lane]; }
+  }
+  for (int it = 0; it < kBfsWave / kWave; ++it) {
+    const int64_t a = base + (int64_t)it * kWave + lane;
+    const bool valid = a < n;
+    const int l0 = valid ? lo[a] : kLevels, l1 = valid ? hi[a] : -1;
+    NodeGeom g = {0.0, 0.0, 0.0, 0.0};
+    if (valid) { g.cx = x[a]; g.cy = y[a]; g.cz = z[a]; g.b2 = q[a]; }
+    const int lmin = (int)wave_min_u32((uint32_t)l0), lmax = (int)wave_max_i32(l1);
+    uint32_t child = 0;
+    for (int l = lmax; l >= lmin; --l) {
+      const bool own = l0 <= l && l <= l1;
+      const bool twig = own && l < l1;
+      const unsigned long long mn = __ballot(own), mt = __ballot(twig);
+      const uint32_t idx = (uint32_t)__builtin_amdgcn_readlane((int)rn, l) + mask_rank(mn);
+      const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)rt, l) + mask_rank(mt);
+      if (lane == l) { rn += (uint32_t)popc64(mn); rt += (uint32_t)popc64(mt); }
+      if (own) {
+        NodeLink lk;
+        lk.level = (uint32_t)l;
+        nfirst[idx] = (uint32_t)a;
+        if (!twig) {                           // leaf (a leaf has no acceptance test:
+          lk.first = (uint32_t)a;              // its b2 slot holds the charge)
+          lk.nchild = 0;
+          lk.count = 1;
+          geom[idx] = g;
+        } else {
+          lk.first = child;
+          lk.nchild = 0xFFFFFFFFu;
+          lk.count = 0;
+          fc_rank[r] = child;
+          node_rank[r] = idx;
+        }
+        link[idx] = lk;
+        child = idx;                           // first child of this particle's twig one level up
+      }
     }
   }
 }
@@ -305,12 +385,19 @@ __global__ void nchild_kernel(NodeLink* __restrict__ link, const uint32_t* __res
                               const DevScalars* __restrict__ sc) {
   if (tree_broken(sc)) return;
   const int64_t nt = sc->ntwig;
+  // children of the last twig end at the last node (fc_rank[nt] = nnodes, set
+  // by fc_sentinel_kernel)
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nt;
        r += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t node = node_rank[r], f = fc_rank[r], nc = fc_rank[r + 1] - f;
     link[node].nchild = nc;
     for (uint32_t j = 0; j < nc; ++j) parent[f + j] = node;
   }
+}
+
+__global__ void fc_sentinel_kernel(uint32_t* __restrict__ fc_rank, const DevScalars* __restrict__ sc) {
+  if (tree_broken(sc)) return;
+  fc_rank[sc->ntwig] = sc->nnodes;
 }
 
 // Roots have no parent; every other entry is written by nchild_kernel.
@@ -379,7 +466,6 @@ struct UpArgs {
   const DevBox* boxes;
   const uint32_t* sys;
   const uint32_t* nfirst;
-  const uint32_t* trank;      // twig rank of each node (exclusive count of twigs before it)
   const uint32_t* node_rank;  // node index of each twig rank
   const uint32_t* fc_rank;    // first child of each twig rank (+1 sentinel)
   const DevScalars* sc;
@@ -480,8 +566,8 @@ __device__ void twig_moments(int64_t i, uint32_t c0, uint32_t nc, int level, con
 }
 
 // One launch per level, deepest first, one thread per twig of the level:
-// nodes are ordered by level, so a level's twigs are one contiguous run of
-// twig ranks, [trank[first node], trank[first node of the next level]).
+// twig ranks follow the BFS order, so a level's twigs are one contiguous run
+// of ranks, [twig_offset[level], twig_offset[level + 1]).
 // A twig's children come from the rank-indexed first-child array (children
 // of consecutive twigs are consecutive), read together with its node index.
 // The launch boundary orders a level's writes before the next level's reads
@@ -489,10 +575,7 @@ __device__ void twig_moments(int64_t i, uint32_t c0, uint32_t nc, int level, con
 // node, which writes back the L2 of its XCD.
 __global__ __launch_bounds__(kT) void upward_level_kernel(int level, UpArgs u) {
   if (tree_broken(u.sc)) return;
-  const int64_t nn = (int64_t)u.sc->nnodes, nt = (int64_t)u.sc->ntwig;
-  const int64_t b = u.sc->level_offset[level], e = u.sc->level_offset[level + 1];
-  const int64_t tb = b < nn ? (int64_t)u.trank[b] : nt;
-  const int64_t te = e < nn ? (int64_t)u.trank[e] : nt;
+  const int64_t tb = u.sc->twig_offset[level], te = u.sc->twig_offset[level + 1];
   for (int64_t r = tb + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < te;
        r += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = u.node_rank[r];
@@ -586,18 +669,24 @@ __global__ void gmark_clear_kernel(int64_t n, uint32_t* __restrict__ gmark) {
 // G particles marks each child with at most G as a group start (the child's first particle gets the group size).  With
 // pack > 0, runs of consecutive such children are merged while the merged
 // size stays <= pack; a child that is split further ends a run.
+// Items: the roots (0 .. nroot-1), then the twigs by rank.
 __global__ void group_mark_kernel(int group_size, int pack, const NodeLink* __restrict__ link,
-                                  const uint32_t* __restrict__ nfirst, uint32_t* __restrict__ gmark,
+                                  const uint32_t* __restrict__ nfirst,
+                                  const uint32_t* __restrict__ node_rank, uint32_t* __restrict__ gmark,
                                   const DevScalars* __restrict__ sc) {
   if (tree_broken(sc)) return;
-  const int64_t m = sc->nnodes;
   const int64_t nroot = sc->level_offset[1];
+  const int64_t items = nroot + (int64_t)sc->ntwig;
   const uint32_t G = (uint32_t)group_size, P = (uint32_t)pack;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const NodeLink lk = link[i];
-    if (i < nroot && lk.count <= G) gmark[nfirst[i]] = lk.count;   // a root that is one group
-    if (lk.nchild == 0 || lk.count <= G) continue;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < items;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    if (t < nroot) {
+      const uint32_t cnt = link[t].count;
+      if (cnt <= G) gmark[nfirst[t]] = cnt;   // a root that is one group
+      continue;
+    }
+    const NodeLink lk = link[node_rank[t - nroot]];
+    if (lk.count <= G) continue;
     uint32_t run_first = 0xFFFFFFFFu, run_cnt = 0;
     for (uint32_t j = 0; j < lk.nchild; ++j) {
       const uint32_t c = lk.first + j;
@@ -675,29 +764,27 @@ void launch_dupfix(hipStream_t s, int64_t n, unsigned long long* keys, double* x
   dup_recoord_kernel<<<grid_for(n), kT, 0, s>>>(n, keys, oldkeys, x, y, z, boxes, sys, sc);
 }
 
+int64_t bfs_tiles(int64_t n) {
+  const int64_t b = (n + kBfsTile - 1) / kBfsTile;
+  return b < 1 ? 1 : b;
+}
+
+int64_t bfs_table_words(int64_t n) { return 2 * (int64_t)kLevels * bfs_tiles(n); }
+
 void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size, int group_pack) {
-  const int64_t n = t.n, cap = t.cap;
-  levels_kernel<<<grid_for(n), kT, 0, s>>>(n, t.keys, t.sys, t.lo, t.hi, t.cnt, t.sc);
-  launch_scan(s, t.cnt, t.cnt, n, t.bsum, &t.sc->nnodes);
-  dfs_emit_kernel<<<grid_for(n), kT, 0, s>>>(n, cap, t.lo, t.hi, t.cnt, t.dkey[0], t.dval[0], t.dpart, t.sc);
-  // one stable pass keyed on the level; entries beyond nnodes are never read
-  launch_radix_pass_dev(s, t.dkey[0], t.dval[0], t.dkey[1], t.dval[1], cap, &t.sc->nnodes, 0, 5,
-                        t.hist, t.bsum, nullptr);
-  // the level pass leaves the per-level node counts in bsum[0..31]
-  level_offsets_kernel<<<1, 64, 0, s>>>(t.bsum, t.sc);
-  inv_kernel<<<grid_for(cap), kT, 0, s>>>(t.dval[1], t.inv, t.sc);
-  node_fill_kernel<<<grid_for(cap), kT, 0, s>>>(cap, t.dkey[1], t.dval[1], t.dpart, t.inv, t.hi,
-                                                t.x, t.y, t.z, t.q, t.geom, t.link, t.nfirst,
-                                                t.flag, t.sc);
-  // twig flags of the nnodes nodes only (entries past nnodes are never read)
-  launch_scan(s, t.flag, t.flag, cap, t.bsum, &t.sc->ntwig, &t.sc->nnodes);
-  twig_rank_kernel<<<grid_for(cap), kT, 0, s>>>(t.link, t.flag, t.fc_rank, t.node_rank, t.sc);
+  const int64_t n = t.n, cap = t.cap, nb = bfs_tiles(n);
+  levels_kernel<<<grid_for(n), kT, 0, s>>>(n, t.keys, t.sys, t.lo, t.hi, t.sc);
+  bfs_count_kernel<<<(unsigned)nb, kT, 0, s>>>(n, t.lo, t.hi, t.tab, nb, t.sc);
+  bfs_scan_kernel<<<2, kBfsScanT, 0, s>>>(t.tab, nb, cap, t.sc);
+  bfs_emit_kernel<<<(unsigned)nb, kT, 0, s>>>(n, t.lo, t.hi, t.tab, nb, t.x, t.y, t.z, t.q, t.geom,
+                                              t.link, t.nfirst, t.fc_rank, t.node_rank, t.sc);
+  fc_sentinel_kernel<<<1, 1, 0, s>>>(t.fc_rank, t.sc);
   root_parent_kernel<<<grid_for(n), kT, 0, s>>>(t.parent, t.sc);
   nchild_kernel<<<grid_for(cap), kT, 0, s>>>(t.link, t.fc_rank, t.node_rank, t.parent, t.sc);
   launch_upward(s, t);
   gmark_clear_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark);
-  group_mark_kernel<<<grid_for(cap), kT, 0, s>>>(group_size, group_pack, t.link, t.nfirst, t.gmark,
-                                                 t.sc);
+  group_mark_kernel<<<grid_for(n), kT, 0, s>>>(group_size, group_pack, t.link, t.nfirst, t.node_rank,
+                                               t.gmark, t.sc);
   group_flag_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark, t.cnt);
   launch_scan(s, t.cnt, t.cnt, n, t.bsum, &t.sc->ngroups);
   group_emit_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark, t.cnt, t.gfirst, t.gcount);
@@ -710,7 +797,7 @@ void launch_upward(hipStream_t s, const TreeBuffers& t) {
   u.x = t.x; u.y = t.y; u.z = t.z; u.q = t.q;
   u.mom = t.mom64; u.geom = t.geom; u.wm32 = t.wm32; u.wm64 = t.wm64;
   u.boxes = t.box; u.sys = t.sys; u.nfirst = t.nfirst;
-  u.trank = t.flag; u.node_rank = t.node_rank; u.fc_rank = t.fc_rank;
+  u.node_rank = t.node_rank; u.fc_rank = t.fc_rank;
   u.sc = t.sc;
   // A fixed grid near the resident capacity (six 256-thread blocks per CU of
   // MI355X) with a twig-stride loop: the large levels keep every CU busy and
