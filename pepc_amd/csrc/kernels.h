@@ -79,6 +79,7 @@ struct TreeBuffers {
   uint32_t* fc_rank;                 // first child by twig rank (+1 sentinel)
   uint32_t* node_rank;               // node index by twig rank
   uint32_t* nfirst;                  // first particle of each node
+  uint8_t* ntype;                    // node kind: 0 leaf, 1 twig
   uint32_t* parent;                  // parent node (roots: 0xFFFFFFFF)
   NodeLink* link;
   NodeGeom* geom;

@@ -47,6 +47,7 @@ struct Ctx {
   double *res[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // host-path outputs
   // nodes
   uint32_t* tab = nullptr;          // BFS numbering table, bfs_table_words(ncap) words
+  uint8_t* ntype = nullptr;         // node kinds (0 leaf, 1 twig)
   uint32_t *fc_rank = nullptr,
            *node_rank = nullptr, *nfirst = nullptr, *nparent = nullptr;
   NodeLink* link = nullptr;
@@ -141,7 +142,7 @@ struct SinkBufs {
 // clearing between calls.
 constexpr int64_t kNoAllocSinks = 262144;
 constexpr size_t scratch_pad(size_t b) { return (b + 255) / 256 * 256; }
-constexpr int64_t kNoAllocSortBlocks = (kNoAllocSinks + 2047) / 2048;   // sort_blocks(kNoAllocSinks)
+constexpr int64_t kNoAllocSortBlocks = (kNoAllocSinks + 2047) / 2048;   // >= sort_blocks(kNoAllocSinks)
 constexpr int64_t kNoAllocHist = 256 * kNoAllocSortBlocks;
 constexpr int64_t kNoAllocBsum = 512;   // >= scan_blocks(kNoAllocHist) + 64 and the 256 digit totals
 constexpr size_t kScratchD = scratch_pad(sizeof(double) * kNoAllocSinks);
@@ -241,7 +242,7 @@ void free_all(Ctx* c) {
   void* ptrs[] = {c->in[0], c->in[1], c->in[2], c->in[3], c->k0, c->k1, c->oldkeys, c->v0, c->v1,
                   c->xs, c->ys, c->zs, c->qs, c->lo, c->hi, c->cnt, c->gmark, c->dflag, c->dlist,
                   c->kd, c->gfirst, c->gcount, c->gstat, c->queue, c->np2p, c->nm2p, c->res[0], c->res[1],
-                  c->res[2], c->res[3], c->res[4], c->tab,
+                  c->res[2], c->res[3], c->res[4], c->tab, c->ntype,
                   c->fc_rank, c->node_rank, c->nfirst, c->link, c->geom,
                   c->mom64, c->wm32, c->wm64, c->hist, c->bsum, c->partial, c->box, c->sc,
                   c->sys, c->soff, c->sysroot, c->label, c->tlabel, c->ehist, c->ecount,
@@ -280,7 +281,7 @@ int alloc_nodes(Ctx* c, int64_t mc) {
   if ((double)mc >= 4294967295.0)
     return fail(c, PEPC_GPU_EINVAL, "node pool exceeds 32-bit node indices");
   const int64_t nc = c->ncap;
-  ALLOC(c, c->fc_rank, mc + 1); ALLOC(c, c->node_rank, mc); ALLOC(c, c->nfirst, mc);
+  ALLOC(c, c->fc_rank, mc + 1); ALLOC(c, c->ntype, mc); ALLOC(c, c->node_rank, mc); ALLOC(c, c->nfirst, mc);
   ALLOC(c, c->nparent, mc);
   ALLOC(c, c->link, mc); ALLOC(c, c->geom, mc); ALLOC(c, c->mom64, mc * 16);
   ALLOC(c, c->wm32, mc); ALLOC(c, c->wm64, mc);
@@ -440,7 +441,7 @@ int grow_device_impl(Ctx* c, int64_t n, const double* x, const double* y, const 
   t.keys = c->k0;
   t.x = c->xs; t.y = c->ys; t.z = c->zs; t.q = c->qs;
   t.lo = c->lo; t.hi = c->hi; t.cnt = c->cnt;
-  t.tab = c->tab; t.fc_rank = c->fc_rank;
+  t.tab = c->tab; t.fc_rank = c->fc_rank; t.ntype = c->ntype;
   t.node_rank = c->node_rank; t.nfirst = c->nfirst; t.link = c->link; t.geom = c->geom;
   t.parent = c->nparent;
   t.mom64 = c->mom64; t.wm32 = c->wm32; t.wm64 = c->wm64;

@@ -327,7 +327,7 @@ __global__ __launch_bounds__(kT) void bfs_emit_kernel(
     const uint32_t* __restrict__ tab, int64_t nb, const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ q,
     NodeGeom* __restrict__ geom, NodeLink* __restrict__ link, uint32_t* __restrict__ nfirst,
-    uint32_t* __restrict__ fc_rank, uint32_t* __restrict__ node_rank,
+    uint8_t* __restrict__ ntype, uint32_t* __restrict__ fc_rank, uint32_t* __restrict__ node_rank,
     const DevScalars* __restrict__ sc) {
   if (tree_broken(sc)) return;
   __shared__ uint32_t wn[kBfsWaves][kLevels], wt[kBfsWaves][kLevels];
@@ -374,6 +374,7 @@ __global__ __launch_bounds__(kT) void bfs_emit_kernel(
           node_rank[r] = idx;
         }
         link[idx] = lk;
+        ntype[idx] = twig ? 1 : 0;
         child = idx;                           // first child of this particle's twig one level up
       }
     }
@@ -466,6 +467,7 @@ struct UpArgs {
   const DevBox* boxes;
   const uint32_t* sys;
   const uint32_t* nfirst;
+  const uint8_t* ntype;       // 0 leaf, 1 twig
   const uint32_t* node_rank;  // node index of each twig rank
   const uint32_t* fc_rank;    // first child of each twig rank (+1 sentinel)
   const DevScalars* sc;
@@ -473,24 +475,27 @@ struct UpArgs {
 
 __device__ void twig_moments(int64_t i, uint32_t c0, uint32_t nc, int level, const UpArgs& u) {
   const double b2 = u.boxes[u.sys ? u.sys[u.nfirst[i]] : 0u].b2[level];
-  // Every child's charge, |charge| and centre are requested together (a leaf
-  // child's from its geometry record, which holds its particle's position
-  // and charge and is loaded with its link; a twig child's from the first 48
-  // bytes of its moment record), so the loads of all children overlap.
-  double cq[8], ca[8], cx[8], cy[8], cz[8];
+  // The children's kinds come first (one byte each, one line); then every
+  // child's charge, |charge| and centre are requested together, a leaf
+  // child's from its geometry record (its particle's position and charge), a
+  // twig child's from the first 48 bytes of its moment record with its
+  // particle count from its link, so the loads of all children overlap.
   uint32_t leafmask = 0, count = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 8; ++j)
+    if (j < nc && u.ntype[c0 + j] == 0) leafmask |= 1u << j;
+  double cq[8], ca[8], cx[8], cy[8], cz[8];
 #pragma unroll
   for (uint32_t j = 0; j < 8; ++j) {
     cq[j] = ca[j] = cx[j] = cy[j] = cz[j] = 0.0;
     if (j < nc) {
-      const NodeLink cl = u.link[c0 + j];
-      const NodeGeom g = u.geom[c0 + j];
-      count += cl.count;
-      if (cl.nchild == 0) {
-        leafmask |= 1u << j;
+      if (leafmask & (1u << j)) {
+        const NodeGeom g = u.geom[c0 + j];
+        count += 1u;
         cx[j] = g.cx; cy[j] = g.cy; cz[j] = g.cz; cq[j] = g.b2;
         ca[j] = fabs(cq[j]);
       } else {
+        count += u.link[c0 + j].count;
         const double2* r = reinterpret_cast<const double2*>(u.mom + (int64_t)(c0 + j) * 16);
         const double2 r0 = r[0], r1 = r[1], r2 = r[2];
         cx[j] = r0.x; cy[j] = r0.y; cz[j] = r1.x; cq[j] = r1.y; ca[j] = r2.x;
@@ -777,7 +782,7 @@ void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size, int 
   bfs_count_kernel<<<(unsigned)nb, kT, 0, s>>>(n, t.lo, t.hi, t.tab, nb, t.sc);
   bfs_scan_kernel<<<2, kBfsScanT, 0, s>>>(t.tab, nb, cap, t.sc);
   bfs_emit_kernel<<<(unsigned)nb, kT, 0, s>>>(n, t.lo, t.hi, t.tab, nb, t.x, t.y, t.z, t.q, t.geom,
-                                              t.link, t.nfirst, t.fc_rank, t.node_rank, t.sc);
+                                              t.link, t.nfirst, t.ntype, t.fc_rank, t.node_rank, t.sc);
   fc_sentinel_kernel<<<1, 1, 0, s>>>(t.fc_rank, t.sc);
   root_parent_kernel<<<grid_for(n), kT, 0, s>>>(t.parent, t.sc);
   nchild_kernel<<<grid_for(cap), kT, 0, s>>>(t.link, t.fc_rank, t.node_rank, t.parent, t.sc);
@@ -797,7 +802,7 @@ void launch_upward(hipStream_t s, const TreeBuffers& t) {
   u.x = t.x; u.y = t.y; u.z = t.z; u.q = t.q;
   u.mom = t.mom64; u.geom = t.geom; u.wm32 = t.wm32; u.wm64 = t.wm64;
   u.boxes = t.box; u.sys = t.sys; u.nfirst = t.nfirst;
-  u.node_rank = t.node_rank; u.fc_rank = t.fc_rank;
+  u.node_rank = t.node_rank; u.fc_rank = t.fc_rank; u.ntype = t.ntype;
   u.sc = t.sc;
   // A fixed grid near the resident capacity (six 256-thread blocks per CU of
   // MI355X) with a twig-stride loop: the large levels keep every CU busy and
