@@ -31,9 +31,10 @@ int64_t scan_blocks(int64_t n);
 int64_t sort_blocks(int64_t n);
 // Exclusive scan of n values (in place allowed); *total gets the sum when
 // total != nullptr.  With ndev != nullptr only the first min(n, *ndev) values
-// are scanned (the count is read on the device; n sizes the grid).
+// are scanned (the count is read on the device; n sizes the grid).  With
+// flags, each value counts as 1 when nonzero and 0 otherwise.
 void launch_scan(hipStream_t s, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* bsum,
-                 uint32_t* total, const uint32_t* ndev = nullptr);
+                 uint32_t* total, const uint32_t* ndev = nullptr, bool flags = false);
 void launch_radix_pass(hipStream_t s, const unsigned long long* kin, const uint32_t* vin,
                        unsigned long long* kout, uint32_t* vout, int64_t n, int shift, int nbits,
                        uint32_t* hist, uint32_t* bsum);
@@ -75,7 +76,7 @@ struct TreeBuffers {
   uint8_t* lo;                       // L(a): level of the first node of particle a
   uint8_t* hi;                       // leaf level of particle a
   uint32_t* cnt;                     // per-particle scratch (group marks -> group ranks)
-  uint32_t* tab;                     // (level, tile) node and twig counts, bfs_table_words(n)
+  uint32_t* tab;                     // (level, tile) node and twig counts and their scans, bfs_table_words(n)
   uint32_t* fc_rank;                 // first child by twig rank (+1 sentinel)
   uint32_t* node_rank;               // node index by twig rank
   uint32_t* nfirst;                  // first particle of each node

@@ -49,14 +49,15 @@ __device__ __forceinline__ int64_t scan_len(int64_t n, const uint32_t* ndev) {
 __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* __restrict__ in,
                                                                     int64_t n,
                                                                     uint32_t* __restrict__ bsum,
-                                                                    const uint32_t* __restrict__ ndev) {
+                                                                    const uint32_t* __restrict__ ndev,
+                                                                    bool flags) {
   n = scan_len(n, ndev);
   const int64_t base = (int64_t)blockIdx.x * kScanTile;
   uint32_t s = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
     int64_t e = base + j * kScanThreads + threadIdx.x;
-    if (e < n) s += in[e];
+    if (e < n) s += flags ? (in[e] != 0u ? 1u : 0u) : in[e];
   }
   uint32_t tot;
   (void)block_exclusive_scan(s, &tot);
@@ -88,7 +89,8 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t
                                                                    uint32_t* __restrict__ out,
                                                                    int64_t n,
                                                                    const uint32_t* __restrict__ bsum,
-                                                                   const uint32_t* __restrict__ ndev) {
+                                                                   const uint32_t* __restrict__ ndev,
+                                                                   bool flags) {
   __shared__ uint32_t tile[kScanTile];
   n = scan_len(n, ndev);
   const int64_t base = (int64_t)blockIdx.x * kScanTile;
@@ -96,7 +98,8 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
     int64_t e = base + j * kScanThreads + threadIdx.x;
-    tile[j * kScanThreads + threadIdx.x] = e < n ? in[e] : 0u;
+    const uint32_t v = e < n ? in[e] : 0u;
+    tile[j * kScanThreads + threadIdx.x] = flags ? (v != 0u ? 1u : 0u) : v;
   }
   __syncthreads();
   uint32_t loc[kScanItems];
@@ -559,12 +562,12 @@ int64_t scan_blocks(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
 int64_t sort_blocks(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
 
 void launch_scan(hipStream_t s, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* bsum,
-                 uint32_t* total, const uint32_t* ndev) {
+                 uint32_t* total, const uint32_t* ndev, bool flags) {
   if (n <= 0) return;
   const int64_t nb = scan_blocks(n);
-  scan_reduce_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(in, n, bsum, ndev);
+  scan_reduce_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(in, n, bsum, ndev, flags);
   scan_small_kernel<<<1, kScanThreads, 0, s>>>(bsum, (int)nb, total);
-  scan_apply_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(in, out, n, bsum, ndev);
+  scan_apply_kernel<<<(unsigned)nb, kScanThreads, 0, s>>>(in, out, n, bsum, ndev, flags);
 }
 
 // One stable pass over bits [shift, shift + nbits), output in (kout, vout).

@@ -185,7 +185,7 @@ __device__ __forceinline__ bool tree_broken(const DevScalars* sc) {
 __global__ void levels_kernel(int64_t n, const unsigned long long* __restrict__ keys,
                               const uint32_t* __restrict__ sys,
                               uint8_t* __restrict__ lo, uint8_t* __restrict__ hi,
-                              DevScalars* __restrict__ sc) {
+                              uint32_t* __restrict__ gmark, DevScalars* __restrict__ sc) {
   for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < n;
        a += (int64_t)gridDim.x * blockDim.x) {
     int la = 0, lb = 0;
@@ -203,6 +203,7 @@ __global__ void levels_kernel(int64_t n, const unsigned long long* __restrict__ 
     const int lam = la > lb ? la : lb;
     lo[a] = (uint8_t)la;
     hi[a] = (uint8_t)lam;
+    gmark[a] = 0;   // group marks start cleared (group_mark_kernel sets them)
   }
 }
 
@@ -261,56 +262,94 @@ __global__ __launch_bounds__(kT) void bfs_count_kernel(int64_t n, const uint8_t*
   }
 }
 
-// Exclusive scans of the node table (block 0) and of the twig table
-// (block 1), level-major, in place; the totals and per-level starts go to
-// the scalars (node count, first node of every level; twig count, first
-// twig rank of every level).  A node count above the pool raises the
-// overflow flag before any node is written.
+// Exclusive scans of the node table and of the twig table, level-major:
+// block (k, l) scans row l of table k (one count per tile) into the output
+// half of tab and starts it at the sum of rows 0..l-1, which it reads
+// itself with coalesced loads, so the 2 x 21 rows are scanned in one launch
+// with no wait between blocks.  The totals and per-level starts go to the
+// scalars (node count, first node of every level; twig count, first twig
+// rank of every level).  A node count above the pool raises the overflow
+// flag before any node is written.
 constexpr int kBfsScanT = 1024;
 
 __global__ __launch_bounds__(kBfsScanT) void bfs_scan_kernel(uint32_t* __restrict__ tab, int64_t nb,
                                                             int64_t cap, DevScalars* __restrict__ sc) {
   if (tree_broken(sc)) return;
-  const bool twigs = blockIdx.x == 1;
-  uint32_t* t = tab + (twigs ? (int64_t)kLevels * nb : 0);
-  const int64_t len = (int64_t)kLevels * nb;
-  const int64_t per = (len + kBfsScanT - 1) / kBfsScanT;
+  const int kind = blockIdx.x / kLevels, l = blockIdx.x % kLevels;
+  const uint32_t* t = tab + (int64_t)kind * kLevels * nb;
+  uint32_t* out = tab + (int64_t)(2 + kind) * kLevels * nb + (int64_t)l * nb;
+  constexpr int kBatch = 8;   // loads in flight together per thread
+  // sum of the earlier rows
+  const int64_t plen = (int64_t)l * nb;
+  uint32_t prev = 0;
+  for (int64_t e0 = threadIdx.x; e0 < plen; e0 += (int64_t)kBfsScanT * kBatch) {
+    uint32_t v[kBatch];
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) {
+      const int64_t e = e0 + (int64_t)j * kBfsScanT;
+      v[j] = e < plen ? t[e] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) prev += v[j];
+  }
+  // this row: thread i holds entries [i * per, (i + 1) * per)
+  const uint32_t* row = t + plen;
+  const int64_t per = (nb + kBfsScanT - 1) / kBfsScanT;
   const int64_t b0 = (int64_t)threadIdx.x * per;
   uint32_t sum = 0;
-  for (int64_t k = 0; k < per; ++k)
-    if (b0 + k < len) sum += t[b0 + k];
-  // block exclusive scan of the per-thread sums
-  __shared__ uint32_t ws[kBfsScanT / kWave];
+  for (int64_t k0 = 0; k0 < per; k0 += kBatch) {
+    uint32_t v[kBatch];
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) {
+      const int64_t e = b0 + k0 + j;
+      v[j] = (k0 + j < per && e < nb) ? row[e] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) sum += v[j];
+  }
+  __shared__ uint32_t ws[kBfsScanT / kWave], wp[kBfsScanT / kWave];
   const int w = threadIdx.x / kWave;
   const uint32_t inc = wave_inclusive_scan(sum);
+  const uint32_t pw = wave_sum(prev);
   if (lane_id() == kWave - 1) ws[w] = inc;
+  if (lane_id() == 0) wp[w] = pw;
   __syncthreads();
-  uint32_t off = 0, tot = 0;
+  uint32_t off = 0, tot = 0, start = 0;
   for (int k = 0; k < kBfsScanT / kWave; ++k) {
     off += k < w ? ws[k] : 0u;
     tot += ws[k];
+    start += wp[k];
   }
-  uint32_t run = off + inc - sum;
-  for (int64_t k = 0; k < per; ++k) {
-    if (b0 + k < len) {
-      const uint32_t v = t[b0 + k];
-      t[b0 + k] = run;
-      run += v;
+  uint32_t run = start + off + inc - sum;
+  for (int64_t k0 = 0; k0 < per; k0 += kBatch) {
+    uint32_t v[kBatch];
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) {
+      const int64_t e = b0 + k0 + j;
+      v[j] = (k0 + j < per && e < nb) ? row[e] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) {
+      const int64_t e = b0 + k0 + j;
+      if (k0 + j < per && e < nb) {
+        out[e] = run;
+        run += v[j];
+      }
     }
   }
-  __syncthreads();
-  if (threadIdx.x <= kLevels) {
-    const int l = threadIdx.x;
-    const uint32_t start = l < kLevels ? t[(int64_t)l * nb] : tot;
-    if (twigs) sc->twig_offset[l] = start;
-    else sc->level_offset[l] = start;
-  }
   if (threadIdx.x == 0) {
-    if (twigs) {
-      sc->ntwig = tot;
-    } else {
-      sc->nnodes = tot;
-      if ((int64_t)tot > cap) atomicOr(&sc->flags, kFlagNodeOverflow);
+    if (kind) sc->twig_offset[l] = start;
+    else sc->level_offset[l] = start;
+    if (l == kLevels - 1) {
+      const uint32_t total = start + tot;
+      if (kind) {
+        sc->twig_offset[kLevels] = total;
+        sc->ntwig = total;
+      } else {
+        sc->level_offset[kLevels] = total;
+        sc->nnodes = total;
+        if ((int64_t)total > cap) atomicOr(&sc->flags, kFlagNodeOverflow);
+      }
     }
   }
 }
@@ -328,8 +367,10 @@ __global__ __launch_bounds__(kT) void bfs_emit_kernel(
     const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ q,
     NodeGeom* __restrict__ geom, NodeLink* __restrict__ link, uint32_t* __restrict__ nfirst,
     uint8_t* __restrict__ ntype, uint32_t* __restrict__ fc_rank, uint32_t* __restrict__ node_rank,
-    const DevScalars* __restrict__ sc) {
+    uint32_t* __restrict__ parent, const DevScalars* __restrict__ sc) {
   if (tree_broken(sc)) return;
+  // children of the last twig end at the last node
+  if (blockIdx.x == 0 && threadIdx.x == 0) fc_rank[sc->ntwig] = sc->nnodes;
   __shared__ uint32_t wn[kBfsWaves][kLevels], wt[kBfsWaves][kLevels];
   const int w = threadIdx.x / kWave, lane = lane_id();
   const int64_t base = (int64_t)blockIdx.x * kBfsTile + (int64_t)w * kBfsWave;
@@ -338,8 +379,8 @@ __global__ __launch_bounds__(kT) void bfs_emit_kernel(
   if (lane < kLevels) { wn[w][lane] = rn; wt[w][lane] = rt; }
   __syncthreads();
   if (lane < kLevels) {
-    rn = tab[(int64_t)lane * nb + blockIdx.x];
-    rt = tab[(int64_t)(kLevels + lane) * nb + blockIdx.x];
+    rn = tab[(int64_t)(2 * kLevels + lane) * nb + blockIdx.x];
+    rt = tab[(int64_t)(3 * kLevels + lane) * nb + blockIdx.x];
     for (int k = 0; k < w; ++k) { rn += wn[k][lane]; rt += wt[k][lane]; }
   }
   for (int it = 0; it < kBfsWave / kWave; ++it) {
@@ -375,40 +416,14 @@ __global__ __launch_bounds__(kT) void bfs_emit_kernel(
         }
         link[idx] = lk;
         ntype[idx] = twig ? 1 : 0;
+        if (l == 0) parent[idx] = 0xFFFFFFFFu;   // a root (every other node's parent is
+                                                // written by the upward pass)
         child = idx;                           // first child of this particle's twig one level up
       }
     }
   }
 }
 
-__global__ void nchild_kernel(NodeLink* __restrict__ link, const uint32_t* __restrict__ fc_rank,
-                              const uint32_t* __restrict__ node_rank, uint32_t* __restrict__ parent,
-                              const DevScalars* __restrict__ sc) {
-  if (tree_broken(sc)) return;
-  const int64_t nt = sc->ntwig;
-  // children of the last twig end at the last node (fc_rank[nt] = nnodes, set
-  // by fc_sentinel_kernel)
-  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nt;
-       r += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t node = node_rank[r], f = fc_rank[r], nc = fc_rank[r + 1] - f;
-    link[node].nchild = nc;
-    for (uint32_t j = 0; j < nc; ++j) parent[f + j] = node;
-  }
-}
-
-__global__ void fc_sentinel_kernel(uint32_t* __restrict__ fc_rank, const DevScalars* __restrict__ sc) {
-  if (tree_broken(sc)) return;
-  fc_rank[sc->ntwig] = sc->nnodes;
-}
-
-// Roots have no parent; every other entry is written by nchild_kernel.
-__global__ void root_parent_kernel(uint32_t* __restrict__ parent, const DevScalars* __restrict__ sc) {
-  if (tree_broken(sc)) return;
-  const int64_t nroot = sc->level_offset[1];
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nroot;
-       i += (int64_t)gridDim.x * blockDim.x)
-    parent[i] = 0xFFFFFFFFu;
-}
 
 // Moment record slots, reference layout (t_tree_node_interaction_data order).
 enum { M_CX = 0, M_CY, M_CZ, M_Q, M_AQ, M_PX, M_PY, M_PZ, M_QX, M_QY, M_QZ, M_XY, M_YZ, M_ZX, M_BMAX };
@@ -470,6 +485,7 @@ struct UpArgs {
   const uint8_t* ntype;       // 0 leaf, 1 twig
   const uint32_t* node_rank;  // node index of each twig rank
   const uint32_t* fc_rank;    // first child of each twig rank (+1 sentinel)
+  uint32_t* parent;           // parent node of each node
   const DevScalars* sc;
 };
 
@@ -563,7 +579,9 @@ __device__ void twig_moments(int64_t i, uint32_t c0, uint32_t nc, int level, con
   m[M_Q] = charge; m[M_AQ] = absq;
   for (int d = 0; d < 3; ++d) { m[M_PX + d] = dip[d]; m[M_QX + d] = quad[d]; }
   m[M_XY] = xy; m[M_YZ] = yz; m[M_ZX] = zx; m[M_BMAX] = bmax; m[15] = 0.0;
+  u.link[i].nchild = nc;
   u.link[i].count = count;
+  for (uint32_t j = 0; j < nc; ++j) u.parent[c0 + j] = (uint32_t)i;
   double2* out = reinterpret_cast<double2*>(u.mom + i * 16);
 #pragma unroll
   for (int k = 0; k < 8; ++k) out[k] = make_double2(m[2 * k], m[2 * k + 1]);
@@ -664,12 +682,6 @@ __global__ void leaf_mom_kernel(const NodeLink* __restrict__ link, const double*
 }
 
 // ------------------------------------------------------------ groups
-__global__ void gmark_clear_kernel(int64_t n, uint32_t* __restrict__ gmark) {
-  for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < n;
-       a += (int64_t)gridDim.x * blockDim.x)
-    gmark[a] = 0;
-}
-
 // A root holding at most G particles is one group.  A twig holding more than
 // G particles marks each child with at most G as a group start (the child's first particle gets the group size).  With
 // pack > 0, runs of consecutive such children are merged while the merged
@@ -712,12 +724,6 @@ __global__ void group_mark_kernel(int group_size, int pack, const NodeLink* __re
     }
     if (run_first != 0xFFFFFFFFu) gmark[run_first] = run_cnt;
   }
-}
-
-__global__ void group_flag_kernel(int64_t n, const uint32_t* __restrict__ gmark, uint32_t* __restrict__ f) {
-  for (int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; a < n;
-       a += (int64_t)gridDim.x * blockDim.x)
-    f[a] = gmark[a] ? 1u : 0u;
 }
 
 __global__ void group_emit_kernel(int64_t n, const uint32_t* __restrict__ gmark,
@@ -774,24 +780,20 @@ int64_t bfs_tiles(int64_t n) {
   return b < 1 ? 1 : b;
 }
 
-int64_t bfs_table_words(int64_t n) { return 2 * (int64_t)kLevels * bfs_tiles(n); }
+int64_t bfs_table_words(int64_t n) { return 4 * (int64_t)kLevels * bfs_tiles(n); }
 
 void launch_tree_build(hipStream_t s, const TreeBuffers& t, int group_size, int group_pack) {
   const int64_t n = t.n, cap = t.cap, nb = bfs_tiles(n);
-  levels_kernel<<<grid_for(n), kT, 0, s>>>(n, t.keys, t.sys, t.lo, t.hi, t.sc);
+  levels_kernel<<<grid_for(n), kT, 0, s>>>(n, t.keys, t.sys, t.lo, t.hi, t.gmark, t.sc);
   bfs_count_kernel<<<(unsigned)nb, kT, 0, s>>>(n, t.lo, t.hi, t.tab, nb, t.sc);
-  bfs_scan_kernel<<<2, kBfsScanT, 0, s>>>(t.tab, nb, cap, t.sc);
+  bfs_scan_kernel<<<2 * kLevels, kBfsScanT, 0, s>>>(t.tab, nb, cap, t.sc);
   bfs_emit_kernel<<<(unsigned)nb, kT, 0, s>>>(n, t.lo, t.hi, t.tab, nb, t.x, t.y, t.z, t.q, t.geom,
-                                              t.link, t.nfirst, t.ntype, t.fc_rank, t.node_rank, t.sc);
-  fc_sentinel_kernel<<<1, 1, 0, s>>>(t.fc_rank, t.sc);
-  root_parent_kernel<<<grid_for(n), kT, 0, s>>>(t.parent, t.sc);
-  nchild_kernel<<<grid_for(cap), kT, 0, s>>>(t.link, t.fc_rank, t.node_rank, t.parent, t.sc);
+                                              t.link, t.nfirst, t.ntype, t.fc_rank, t.node_rank,
+                                              t.parent, t.sc);
   launch_upward(s, t);
-  gmark_clear_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark);
   group_mark_kernel<<<grid_for(n), kT, 0, s>>>(group_size, group_pack, t.link, t.nfirst, t.node_rank,
                                                t.gmark, t.sc);
-  group_flag_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark, t.cnt);
-  launch_scan(s, t.cnt, t.cnt, n, t.bsum, &t.sc->ngroups);
+  launch_scan(s, t.gmark, t.cnt, n, t.bsum, &t.sc->ngroups, nullptr, true);
   group_emit_kernel<<<grid_for(n), kT, 0, s>>>(n, t.gmark, t.cnt, t.gfirst, t.gcount);
   if (t.sys) sysroot_kernel<<<grid_for(t.nsys), kT, 0, s>>>(t.sys, t.nfirst, t.sysroot, t.sc);
 }
@@ -802,7 +804,7 @@ void launch_upward(hipStream_t s, const TreeBuffers& t) {
   u.x = t.x; u.y = t.y; u.z = t.z; u.q = t.q;
   u.mom = t.mom64; u.geom = t.geom; u.wm32 = t.wm32; u.wm64 = t.wm64;
   u.boxes = t.box; u.sys = t.sys; u.nfirst = t.nfirst;
-  u.node_rank = t.node_rank; u.fc_rank = t.fc_rank; u.ntype = t.ntype;
+  u.node_rank = t.node_rank; u.fc_rank = t.fc_rank; u.ntype = t.ntype; u.parent = t.parent;
   u.sc = t.sc;
   // A fixed grid near the resident capacity (six 256-thread blocks per CU of
   // MI355X) with a twig-stride loop: the large levels keep every CU busy and
