@@ -50,11 +50,13 @@ void launch_sort_keys(hipStream_t s, unsigned long long* k0, uint32_t* v0, unsig
 // Reorders runs of equal keys in perm (after launch_sort_keys and
 // launch_dup_detect) into the order the reference's in-place MSD presort
 // gives them; nothing to do (one uniform load per block) when dup_count is 0.
-// Scratch: kin[n] (keys by input index), ord/ord2[n], cls[n].  One block per
-// system (soff: nsys + 1 offsets when nsys > 1).
+// Scratch: kin[n] (keys by input index), ord/ord2[n], cls[n], and the
+// per-pass slot lists fp[n] (positions) and fec[n] (element | class << 32).
+// One block per system (soff: nsys + 1 offsets when nsys > 1).
 void launch_tie_order(hipStream_t s, int64_t n, int64_t nsys, const int64_t* soff,
                       const unsigned long long* ks, uint32_t* perm, unsigned long long* kin,
-                      uint32_t* ord, uint32_t* ord2, uint16_t* cls, DevScalars* sc);
+                      uint32_t* ord, uint32_t* ord2, uint16_t* cls, uint32_t* fp,
+                      unsigned long long* fec, DevScalars* sc);
 // out[perm[i]] = in[i] (back to input order)
 void launch_scatter_back(hipStream_t s, int64_t n, const uint32_t* perm, const double* in,
                          double* out);

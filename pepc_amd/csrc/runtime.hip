@@ -428,10 +428,11 @@ int grow_device_impl(Ctx* c, int64_t n, const double* x, const double* y, const 
   launch_sort_keys(s, c->k0, c->v0, c->k1, c->v1, n, c->hist, c->bsum, sys, nsys);
   launch_dup_detect(s, n, c->k0, sys, c->sc);
   // equal keys into the reference presort's order (the sort's ping-pong
-  // buffers and the group scratch arrays, cleared before the groups use
-  // them, are free here)
+  // buffers, the group scratch arrays and the duplicate fix-up's arrays,
+  // all written afresh before their own use, are free here)
   launch_tie_order(s, n, nsys, c->soff, c->k0, c->v0, c->k1, c->v1, c->cnt,
-                   reinterpret_cast<uint16_t*>(c->gmark), c->sc);
+                   reinterpret_cast<uint16_t*>(c->gmark), c->dflag,
+                   reinterpret_cast<unsigned long long*>(c->kd), c->sc);
   launch_gather_rec(s, n, c->v0, c->prec4, c->xs, c->ys, c->zs, c->qs);
   launch_dupfix(s, n, c->k0, c->xs, c->ys, c->zs, c->box, sys, nsys > 1 ? c->soff : nullptr, c->kd,
                 c->oldkeys, c->dflag, c->dlist, c->bsum, c->sc);

@@ -341,8 +341,11 @@ __global__ __launch_bounds__(256) void scatter_back_kernel(int64_t n, const uint
 // r's next not-yet-taken foreign slot (positions in increasing order), and
 // the cycles started from region i (phase i, i = 0, 1, ...) end on their
 // first return to i.  The walk is replayed here literally.  It is serial by
-// nature (every step depends on the class read by the previous one), so
-// one thread walks while the block prepares classes and counts in parallel.
+// nature (every step depends on the class read by the previous one), so the
+// block prepares in parallel what the walk reads: the classes, the region
+// bounds, and each region's foreign slots in increasing position as a list
+// of (position, element, class) entries; one thread then walks, taking one
+// list entry per step.
 //
 // Buckets without equal keys need nothing: for distinct keys the stable LSD
 // order is the only sorted order.  So the replay descends only into buckets
@@ -355,25 +358,11 @@ constexpr int kTieThreshold = 256;      // sl_tune_intern.h:30 (sort_radix_thres
 constexpr int kTieClasses = 1 << kTieWidth;
 constexpr int kTieDepth = 8;            // 64 bits in 10-bit digits: at most 7 passes
 
-// Next foreign slot of region c (positions relative to the segment), or -1
-// when region c has none left.  rot[c] is the region's rotor.
-__device__ __forceinline__ int64_t tie_next(int c, uint32_t* rot, const uint32_t* reg,
-                                            const uint16_t* __restrict__ cls) {
-  uint32_t p = rot[c];
-  const uint32_t end = reg[c + 1];
-  while (p < end && cls[p] == (uint16_t)c) ++p;
-  if (p >= end) {
-    rot[c] = end;
-    return -1;
-  }
-  rot[c] = p + 1;
-  return (int64_t)p;
-}
-
 __global__ __launch_bounds__(kTieThreads) void tie_order_kernel(
     int64_t n, const int64_t* __restrict__ soff, const unsigned long long* __restrict__ ks,
     uint32_t* __restrict__ perm, unsigned long long* __restrict__ kin, uint32_t* __restrict__ ord,
-    uint32_t* __restrict__ ord2, uint16_t* __restrict__ cls, DevScalars* __restrict__ sc) {
+    uint32_t* __restrict__ ord2, uint16_t* __restrict__ cls, uint32_t* __restrict__ fp,
+    unsigned long long* __restrict__ fec, DevScalars* __restrict__ sc) {
   if (sc->dup_count == 0) return;
   const int tid = threadIdx.x;
   const int64_t b = soff ? soff[blockIdx.x] : 0;
@@ -381,7 +370,8 @@ __global__ __launch_bounds__(kTieThreads) void tie_order_kernel(
   if (e - b < 2) return;
 
   __shared__ uint32_t reg[kTieClasses + 1];   // region starts of the current pass
-  __shared__ uint32_t rot[kTieClasses];       // rotors
+  __shared__ uint32_t nxt[kTieClasses];       // next unused entry of each region's slot list
+  __shared__ uint32_t run_base;
   __shared__ unsigned long long skey[kTieThreshold];
   __shared__ int64_t fs[kTieDepth], fe[kTieDepth], fcur[kTieDepth];
   __shared__ int frh[kTieDepth], fnext[kTieDepth];
@@ -422,6 +412,7 @@ __global__ __launch_bounds__(kTieThreads) void tie_order_kernel(
       const int K = 1 << cw;
       const unsigned long long mask = (unsigned long long)(K - 1);
       for (int c = tid; c <= K; c += kTieThreads) reg[c] = 0;
+      for (int c = tid; c < K; c += kTieThreads) nxt[c] = 0;
       __syncthreads();
       for (int64_t t = tid; t < m; t += kTieThreads) {
         const uint32_t el = ord[s0 + t];
@@ -431,27 +422,88 @@ __global__ __launch_bounds__(kTieThreads) void tie_order_kernel(
         atomicAdd(&reg[c + 1], 1u);
       }
       __syncthreads();
-      if (tid == 0) {
+      if (tid == 0)
         for (int c = 0; c < K; ++c) reg[c + 1] += reg[c];
-        for (int c = 0; c < K; ++c) rot[c] = reg[c];
-        const uint16_t* cl = cls + s0;
-        const uint32_t* o1 = ord + s0;
+      __syncthreads();
+      // elements already inside their class's region stay; count them
+      for (int64_t t = tid; t < m; t += kTieThreads) {
+        const int c = cls[s0 + t];
+        if ((uint32_t)t >= reg[c] && (uint32_t)t < reg[c + 1]) atomicAdd(&nxt[c], 1u);
+      }
+      __syncthreads();
+      // region r's foreign slots, in increasing position, become entries
+      // [start_r, start_r+1) of the slot list (fp: position, fec: element and
+      // its class); nxt[r] = start_r, and reg[r] is reused for start_r+1
+      if (tid == 0) {
+        uint32_t acc = 0;
+        for (int c = 0; c < K; ++c) {
+          const uint32_t f = reg[c + 1] - reg[c] - nxt[c];
+          nxt[c] = acc;
+          acc += f;
+        }
+        run_base = 0;
+      }
+      __syncthreads();
+      {
+        constexpr int kPer = 4;   // consecutive positions per thread per chunk
+        for (int64_t base = 0; base < m; base += (int64_t)kTieThreads * kPer) {
+          const int64_t t0 = base + (int64_t)tid * kPer;
+          uint32_t flag[kPer], nf = 0;
+#pragma unroll
+          for (int j = 0; j < kPer; ++j) {
+            const int64_t t = t0 + j;
+            flag[j] = 0u;
+            if (t < m) {
+              const int c = cls[s0 + t];
+              flag[j] = ((uint32_t)t >= reg[c] && (uint32_t)t < reg[c + 1]) ? 0u : 1u;
+            }
+            nf += flag[j];
+          }
+          uint32_t tot;
+          uint32_t off = run_base + block_exclusive_scan(nf, &tot);
+#pragma unroll
+          for (int j = 0; j < kPer; ++j) {
+            if (flag[j]) {
+              const int64_t t = t0 + j;
+              fp[s0 + off] = (uint32_t)t;
+              fec[s0 + off] = (unsigned long long)ord[s0 + t] |
+                              ((unsigned long long)cls[s0 + t] << 32);
+              ++off;
+            }
+          }
+          __syncthreads();
+          if (tid == 0) run_base += tot;
+          __syncthreads();
+        }
+      }
+      if (tid == 0) {
+        for (int c = 0; c < K; ++c) reg[c] = c + 1 < K ? nxt[c + 1] : run_base;
+        // The walk: phase i starts a cycle at each of region i's remaining
+        // foreign slots (increasing); the carried element goes to the next
+        // unused foreign slot of its class's region, whose element is
+        // carried on, until an element of class i closes the cycle in the
+        // starting slot.  One list entry per step.
+        const uint32_t* fpp = fp + s0;
+        const unsigned long long* fe = fec + s0;
         uint32_t* o2 = ord2 + s0;
         for (int i = 0; i < K; ++i) {
-          for (;;) {
-            const int64_t p0 = tie_next(i, rot, reg, cl);   // the cycle's start slot
-            if (p0 < 0) break;
-            uint32_t el = o1[p0];
-            int c = cl[p0];
+          while (nxt[i] < reg[i]) {
+            const uint32_t k0 = nxt[i]++;
+            const uint32_t p0 = fpp[k0];
+            const unsigned long long e0 = fe[k0];
+            uint32_t el = (uint32_t)e0;
+            int c = (int)(e0 >> 32);
             for (;;) {
-              const int64_t p = tie_next(c, rot, reg, cl);
-              if (p < 0) {   // cannot happen: every region has as many foreign slots as it sends
+              if (nxt[c] >= reg[c]) {   // cannot happen: every region receives what it sends
                 atomicOr(&sc->flags, kFlagDupKeys);
                 break;
               }
+              const uint32_t k = nxt[c]++;
+              const uint32_t p = fpp[k];
+              const unsigned long long e2 = fe[k];
               o2[p] = el;
-              el = o1[p];
-              const int c2 = cl[p];
+              el = (uint32_t)e2;
+              const int c2 = (int)(e2 >> 32);
               if (c2 == i) {
                 o2[p0] = el;
                 break;
@@ -552,10 +604,11 @@ __global__ __launch_bounds__(kTieThreads) void tie_order_kernel(
 
 void launch_tie_order(hipStream_t s, int64_t n, int64_t nsys, const int64_t* soff,
                       const unsigned long long* ks, uint32_t* perm, unsigned long long* kin,
-                      uint32_t* ord, uint32_t* ord2, uint16_t* cls, DevScalars* sc) {
+                      uint32_t* ord, uint32_t* ord2, uint16_t* cls, uint32_t* fp,
+                      unsigned long long* fec, DevScalars* sc) {
   if (n < 2) return;
   tie_order_kernel<<<(unsigned)(nsys > 1 ? nsys : 1), kTieThreads, 0, s>>>(
-      n, nsys > 1 ? soff : nullptr, ks, perm, kin, ord, ord2, cls, sc);
+      n, nsys > 1 ? soff : nullptr, ks, perm, kin, ord, ord2, cls, fp, fec, sc);
 }
 
 int64_t scan_blocks(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
