@@ -83,6 +83,7 @@ struct TreeBuffers {
   uint32_t* node_rank;               // node index by twig rank
   uint32_t* nfirst;                  // first particle of each node
   uint8_t* ntype;                    // node kind: 0 leaf, 1 twig
+  uint8_t* kidtwig;                  // per twig: bit j set when child j is a twig
   uint32_t* parent;                  // parent node (roots: 0xFFFFFFFF)
   NodeLink* link;
   NodeGeom* geom;
@@ -126,6 +127,7 @@ struct WalkArgs {
   const double *x, *y, *z, *q;
   const NodeGeom* geom;
   const NodeLink* link;
+  const uint8_t* kidtwig;            // per twig: bit j set when child j is a twig
   const NodeMoments<float>* wm32;
   const NodeMoments<double>* wm64;
   const uint32_t* gfirst;

@@ -48,6 +48,7 @@ struct Ctx {
   // nodes
   uint32_t* tab = nullptr;          // BFS numbering table, bfs_table_words(ncap) words
   uint8_t* ntype = nullptr;         // node kinds (0 leaf, 1 twig)
+  uint8_t* kidtwig = nullptr;       // per twig: which children are twigs
   uint32_t *fc_rank = nullptr,
            *node_rank = nullptr, *nfirst = nullptr, *nparent = nullptr;
   NodeLink* link = nullptr;
@@ -276,12 +277,12 @@ void drop_graph(Ctx* c) {
 // Node arrays for mc nodes, and the radix histogram and scan blocks sized
 // for the larger of the particle and node arrays.  Drops the current tree.
 int alloc_nodes(Ctx* c, int64_t mc) {
-  // node indices (links, parents, stacks) are 32-bit, with 0xFFFFFFFF as the
-  // no-parent mark: the pool must stay below it
-  if ((double)mc >= 4294967295.0)
-    return fail(c, PEPC_GPU_EINVAL, "node pool exceeds 32-bit node indices");
+  // node indices (links, parents) are 32-bit, and the walk's stack entries
+  // keep the node's kind in the top bit: the pool must stay below 2^31
+  if ((double)mc >= 2147483648.0)
+    return fail(c, PEPC_GPU_EINVAL, "node pool exceeds 31-bit node indices");
   const int64_t nc = c->ncap;
-  ALLOC(c, c->fc_rank, mc + 1); ALLOC(c, c->ntype, mc); ALLOC(c, c->node_rank, mc); ALLOC(c, c->nfirst, mc);
+  ALLOC(c, c->fc_rank, mc + 1); ALLOC(c, c->ntype, mc); ALLOC(c, c->kidtwig, mc); ALLOC(c, c->node_rank, mc); ALLOC(c, c->nfirst, mc);
   ALLOC(c, c->nparent, mc);
   ALLOC(c, c->link, mc); ALLOC(c, c->geom, mc); ALLOC(c, c->mom64, mc * 16);
   ALLOC(c, c->wm32, mc); ALLOC(c, c->wm64, mc);
@@ -442,7 +443,7 @@ int grow_device_impl(Ctx* c, int64_t n, const double* x, const double* y, const 
   t.keys = c->k0;
   t.x = c->xs; t.y = c->ys; t.z = c->zs; t.q = c->qs;
   t.lo = c->lo; t.hi = c->hi; t.cnt = c->cnt;
-  t.tab = c->tab; t.fc_rank = c->fc_rank; t.ntype = c->ntype;
+  t.tab = c->tab; t.fc_rank = c->fc_rank; t.ntype = c->ntype; t.kidtwig = c->kidtwig;
   t.node_rank = c->node_rank; t.nfirst = c->nfirst; t.link = c->link; t.geom = c->geom;
   t.parent = c->nparent;
   t.mom64 = c->mom64; t.wm32 = c->wm32; t.wm64 = c->wm64;
@@ -473,7 +474,8 @@ int traverse_device_impl(Ctx* c, double* ex, double* ey, double* ez, double* pot
   WalkArgs a;
   a.n = c->n;
   a.x = c->xs; a.y = c->ys; a.z = c->zs; a.q = c->qs;
-  a.geom = c->geom; a.link = c->link; a.wm32 = c->wm32; a.wm64 = c->wm64;
+  a.geom = c->geom; a.link = c->link; a.kidtwig = c->kidtwig;
+  a.wm32 = c->wm32; a.wm64 = c->wm64;
   a.gfirst = c->gfirst; a.gcount = c->gcount; a.sc = c->sc;
   a.theta2 = c->cfg.theta2; a.eps2 = c->cfg.eps2; a.sign = c->cfg.sign;
   a.ex = ex; a.ey = ey; a.ez = ez; a.pot = pot; a.work = work;
@@ -595,7 +597,8 @@ int traverse_sinks_run(Ctx* c, const SinkBufs& b, int64_t m, const double* x, co
   WalkArgs a;
   a.n = c->n;
   a.x = c->xs; a.y = c->ys; a.z = c->zs; a.q = c->qs;
-  a.geom = c->geom; a.link = c->link; a.wm32 = c->wm32; a.wm64 = c->wm64;
+  a.geom = c->geom; a.link = c->link; a.kidtwig = c->kidtwig;
+  a.wm32 = c->wm32; a.wm64 = c->wm64;
   a.sc = c->sc;
   a.theta2 = c->cfg.theta2; a.eps2 = c->cfg.eps2; a.sign = c->cfg.sign;
   a.np2p = b.p2p; a.nm2p = b.m2p; a.gstat = nullptr; a.queue = c->queue; a.nqueue = c->nxcd;

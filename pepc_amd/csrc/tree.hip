@@ -483,6 +483,7 @@ struct UpArgs {
   const uint32_t* sys;
   const uint32_t* nfirst;
   const uint8_t* ntype;       // 0 leaf, 1 twig
+  uint8_t* kidtwig;           // bit j: child j is a twig (read by the walk's pushes)
   const uint32_t* node_rank;  // node index of each twig rank
   const uint32_t* fc_rank;    // first child of each twig rank (+1 sentinel)
   uint32_t* parent;           // parent node of each node
@@ -581,6 +582,7 @@ __device__ void twig_moments(int64_t i, uint32_t c0, uint32_t nc, int level, con
   m[M_XY] = xy; m[M_YZ] = yz; m[M_ZX] = zx; m[M_BMAX] = bmax; m[15] = 0.0;
   u.link[i].nchild = nc;
   u.link[i].count = count;
+  u.kidtwig[i] = (uint8_t)(~leafmask & ((1u << nc) - 1u));
   for (uint32_t j = 0; j < nc; ++j) u.parent[c0 + j] = (uint32_t)i;
   double2* out = reinterpret_cast<double2*>(u.mom + i * 16);
 #pragma unroll
@@ -805,6 +807,7 @@ void launch_upward(hipStream_t s, const TreeBuffers& t) {
   u.mom = t.mom64; u.geom = t.geom; u.wm32 = t.wm32; u.wm64 = t.wm64;
   u.boxes = t.box; u.sys = t.sys; u.nfirst = t.nfirst;
   u.node_rank = t.node_rank; u.fc_rank = t.fc_rank; u.ntype = t.ntype; u.parent = t.parent;
+  u.kidtwig = t.kidtwig;
   u.sc = t.sc;
   // A fixed grid near the resident capacity (six 256-thread blocks per CU of
   // MI355X) with a twig-stride loop: the large levels keep every CU busy and

@@ -56,6 +56,9 @@ constexpr int kStackCap = 512;      // entries the pop policy keeps below (FP64)
 constexpr int kStackCap32 = 768;    // the same for FP32, whose lists leave LDS for it
 constexpr int kStackSlack = 192;    // >= 7 * 21 + 8: depth-first overrun bound
 constexpr int kListP = 64;
+// Stack entries carry the node's kind in their top bit (set for a twig, and
+// for a root whatever it is), so a leaf's step skips the walk-moment loads.
+constexpr uint32_t kTwigBit = 0x80000000u;
 
 template <typename R> struct Prec;
 template <> struct Prec<float> {
@@ -312,7 +315,7 @@ void walk_kernel(WalkArgs a) {
     const double bhix = hix - vx, bhiy = hiy - vy, bhiz = hiz - vz;
     const double fox = ox - vx, foy = oy - vy, foz = oz - vz;   // FP32 node frame
     if constexpr (sizeof(R) == 8) { tx = sx; ty = sy; tz = sz; }
-    if (lane == 0) stack[0] = root;
+    if (lane == 0) stack[0] = root | kTwigBit;
     sp = 1;
     __syncthreads();
     while (sp > 0) {
@@ -326,19 +329,25 @@ void walk_kernel(WalkArgs a) {
       if (nm + m > (uint32_t)LM) { __syncthreads(); flush_m2p(); nm = 0; }
       if (np + m > (uint32_t)kListP) { __syncthreads(); flush_p2p(); np = 0; }
       const bool has = (uint32_t)lane < m;
-      const uint32_t node = has ? stack[sp - 1 - lane] : 0u;
+      const uint32_t entry = has ? stack[sp - 1 - lane] : 0u;
+      const uint32_t node = entry & ~kTwigBit;
+      const bool twig_entry = (entry & kTwigBit) != 0u;
       __syncthreads();
       ++nsteps;
 
       // All of a node's data is requested at once, before any decision, so a
-      // traversal step waits for one memory round trip (leaf moment slots are
-      // never written and their contents are not used).
+      // traversal step waits for one memory round trip.  A twig also brings
+      // its walk moments and its children's kinds; a leaf has neither.
       NodeGeom gm = {0.0, 0.0, 0.0, 0.0};
       NodeLink lk = {0u, 0u, 0u, 0u};
       V4 w0, w1, w2;
+      uint32_t kids = 0u;
       if (has) {
         gm = a.geom[node];
         lk = a.link[node];
+      }
+      if (has && twig_entry) {
+        kids = a.kidtwig[node];
         const V4* src;
         if constexpr (sizeof(R) == 4) src = reinterpret_cast<const V4*>(&a.wm32[node]);
         else src = reinterpret_cast<const V4*>(&a.wm64[node]);
@@ -461,7 +470,8 @@ void walk_kernel(WalkArgs a) {
       uint32_t tot;
       const uint32_t excl = wave_exclusive_scan16(nc, &tot);
       if (sp + tot > (uint32_t)(kCap + kStackSlack)) { overflow = true; break; }
-      for (uint32_t k = 0; k < nc; ++k) stack[sp + excl + k] = lk.first + k;
+      for (uint32_t k = 0; k < nc; ++k)
+        stack[sp + excl + k] = (lk.first + k) | (((kids >> k) & 1u) ? kTwigBit : 0u);
       sp += tot;
       __syncthreads();
     }
