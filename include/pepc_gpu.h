@@ -55,7 +55,7 @@ typedef struct {
     double eps2;            /* Plummer softening squared (default 0) */
     double sign;            /* +1 Coulomb; -1 attractive (gravity-like) */
     double node_factor;     /* node capacity per particle (np_mult analogue, default 2.5);
-                               node_factor * (n + n/8) must stay below 2^32 nodes (EINVAL) */
+                               node_factor * (n + n/8) must stay below 2^31 nodes (EINVAL) */
     int32_t mac_select;     /* 0 = Barnes-Hut theta2 d^2 > boxlength2(level); 1 = Bmax,
                                theta2 d^2 > min(bmax^2, 3 boxlength2); any other value
                                never accepts (N^2), as coulomb/module_interaction_specific.f90:256-275 */
