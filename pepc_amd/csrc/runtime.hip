@@ -243,7 +243,7 @@ void free_all(Ctx* c) {
   void* ptrs[] = {c->in[0], c->in[1], c->in[2], c->in[3], c->k0, c->k1, c->oldkeys, c->v0, c->v1,
                   c->xs, c->ys, c->zs, c->qs, c->lo, c->hi, c->cnt, c->gmark, c->dflag, c->dlist,
                   c->kd, c->gfirst, c->gcount, c->gstat, c->queue, c->np2p, c->nm2p, c->res[0], c->res[1],
-                  c->res[2], c->res[3], c->res[4], c->tab, c->ntype,
+                  c->res[2], c->res[3], c->res[4], c->tab, c->ntype, c->kidtwig,
                   c->fc_rank, c->node_rank, c->nfirst, c->link, c->geom,
                   c->mom64, c->wm32, c->wm64, c->hist, c->bsum, c->partial, c->box, c->sc,
                   c->sys, c->soff, c->sysroot, c->label, c->tlabel, c->ehist, c->ecount,
