@@ -73,9 +73,10 @@ __global__ void commit_kernel(int64_t n, StepState st) {
 
 // Energies in two deterministic passes (fixed summation order, so graph
 // replays and eager steps agree bit for bit): one block per chunk of at most
-// kEnergyChunk particles of one system writes its partial sums; then one thread per
-// system adds its chunks' partials in order into record rec = *counter
-// (records past cap are dropped).
+// kEnergyChunk particles of one system writes its partial sums; then one
+// block per system adds its chunks' partials (thread t takes chunks t,
+// t + kT, ... in order, then a fixed-shape wave and block reduction) into
+// record rec = *counter (records past cap are dropped).
 __global__ __launch_bounds__(kT) void energy_chunk_kernel(const int64_t* __restrict__ cbeg,
                                                          const double* __restrict__ pot, StepState st,
                                                          double* __restrict__ partial) {
@@ -99,24 +100,31 @@ __global__ __launch_bounds__(kT) void energy_chunk_kernel(const int64_t* __restr
     partial[2 * c + 1] = d;
   }
 }
-
-__global__ void energy_system_kernel(int nsys, const int* __restrict__ sys_chunk,
-                                     const double* __restrict__ partial, double* __restrict__ hist,
-                                     int64_t cap, const uint32_t* __restrict__ counter) {
+__global__ __launch_bounds__(kT) void energy_system_kernel(const int* __restrict__ sys_chunk,
+                                                          const double* __restrict__ partial,
+                                                          double* __restrict__ hist, int nsys,
+                                                          int64_t cap,
+                                                          const uint32_t* __restrict__ counter) {
+  const int sid = blockIdx.x;
   const int64_t rec = *counter;
-  for (int sid = blockIdx.x * blockDim.x + threadIdx.x; sid < nsys; sid += gridDim.x * blockDim.x) {
-    double a = 0.0, d = 0.0;
-    for (int c = sys_chunk[sid]; c < sys_chunk[sid + 1]; ++c) {
-      a += partial[2 * c];
-      d += partial[2 * c + 1];
-    }
-    if (rec < cap) {
-      hist[(rec * nsys + sid) * 2 + 0] = 0.5 * a;
-      hist[(rec * nsys + sid) * 2 + 1] = 0.5 * d;
-    }
+  double a = 0.0, d = 0.0;
+  for (int c = sys_chunk[sid] + (int)threadIdx.x; c < sys_chunk[sid + 1]; c += kT) {
+    a += partial[2 * c];
+    d += partial[2 * c + 1];
+  }
+  __shared__ double red[2][kT / kWave];
+  a = wave_sum(a);
+  d = wave_sum(d);
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) { red[0][w] = a; red[1][w] = d; }
+  __syncthreads();
+  if (threadIdx.x == 0 && rec < cap) {
+    double sa = 0.0, sd = 0.0;
+    for (int k = 0; k < kT / kWave; ++k) { sa += red[0][k]; sd += red[1][k]; }
+    hist[(rec * nsys + sid) * 2 + 0] = 0.5 * sa;
+    hist[(rec * nsys + sid) * 2 + 1] = 0.5 * sd;
   }
 }
-
 __global__ void counter_inc_kernel(uint32_t* counter) {
   if (threadIdx.x == 0) *counter += 1u;
 }
@@ -150,7 +158,7 @@ void launch_energy(hipStream_t s, int nsys, int nchunk, const int64_t* cbeg, con
                    const double* pot, const StepState& st, double* partial, double* hist,
                    int64_t cap, uint32_t* counter) {
   if (nchunk > 0) energy_chunk_kernel<<<nchunk, kT, 0, s>>>(cbeg, pot, st, partial);
-  energy_system_kernel<<<(nsys + 255) / 256, 256, 0, s>>>(nsys, sys_chunk, partial, hist, cap, counter);
+  if (nsys > 0) energy_system_kernel<<<nsys, kT, 0, s>>>(sys_chunk, partial, hist, nsys, cap, counter);
   counter_inc_kernel<<<1, 64, 0, s>>>(counter);
 }
 
