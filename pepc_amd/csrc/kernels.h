@@ -184,6 +184,14 @@ struct LatticeGeom {
   int simple;                                   // principal axes
   int correction;                               // extrinsic correction (kExtrinsic*)
   double volume;                                // |(t1 x t2) . t3|
+  // Fictitious-charge correction: the neighbour boxes' fictitious charges
+  // merged by lattice point, the points where all of them meet (and cancel)
+  // left out.  fpt[4k..4k+2] = integer lattice coordinates of point k,
+  // fpt[4k+3] = mask of the charges there; lattice_mu_kernel writes
+  // fpos[4k..4k+3] = {x, y, z, merged charge}.  nfpt = 0 otherwise.
+  const int* fpt;
+  double* fpos;
+  int nfpt;
 };
 // Extrinsic-to-intrinsic corrections, module_fmm_framework.f90:42-46.
 enum { kExtrinsicNone = 0, kExtrinsicRedlack = 1, kExtrinsicFictCharge = 2 };
@@ -204,7 +212,7 @@ void launch_lattice_inv(hipStream_t s, double* inv);
 // mu: kLatticeNM, dip: 4, fict: kLatticeFict doubles, coef: 8 * kLatticeNM.
 void launch_lattice_far(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
                         const double* q, const LatticeGeom& g, const double2* ml, const double* fact,
-                        const double* inv, const double* vbox, int nb, double sign, double2* part,
+                        const double* inv, double sign, double2* part,
                         double* dpart, double2* om, double* dip, double2* mu, double* fict,
                         double* coef, double* ex, double* ey, double* ez, double* pot);
 // The field part alone at m other points (mu, fict and coef of the sources
@@ -215,8 +223,8 @@ void launch_lattice_sources(hipStream_t s, int64_t n, const double* x, const dou
                             double2* om, double* dip, double2* mu, double* fict, double* coef);
 void launch_lattice_field(hipStream_t s, int64_t m, const double* x, const double* y, const double* z,
                           const LatticeGeom& g, const double* coef, const double* inv,
-                          const double* fict, const double* vbox, int nb, double sign, double* ex,
-                          double* ey, double* ez, double* pot);
+                          const double* fict, double sign, double* ex, double* ey, double* ez,
+                          double* pot);
 void launch_constrain_periodic(hipStream_t s, int64_t n, double* x, double* y, double* z,
                                const LatticeGeom& g);
 

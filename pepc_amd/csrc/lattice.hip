@@ -24,7 +24,9 @@
 // Work per launch: the coefficient kernels run once per lattice; the moments
 // kernel is N x 231 complex terms (one grid row per order m, recurrences in
 // Cartesian form instead of Legendre evaluations per term); the field kernel
-// is N x 231 terms against mu in LDS plus 4 x 27 fictitious-charge pairs.
+// is N x 231 terms against mu in LDS plus the neighbour boxes' fictitious
+// charges merged by lattice point (46 points for one mirror layer instead of
+// 4 x 27 pairs: where all four charges meet they cancel).
 #include "kernels.h"
 
 namespace pepc {
@@ -456,6 +458,19 @@ __global__ __launch_bounds__(kMuT) void lattice_mu_kernel(const double2* __restr
     nf = k;
   }
   __syncthreads();
+  // the neighbour boxes' fictitious charges merged by lattice point: charge
+  // f sits at origin + t_p (axis p) or at the origin, so every contribution
+  // to a point is one of this step's charges
+  for (int k = t; k < g.nfpt; k += kMuT) {
+    const int* e = g.fpt + 4 * k;
+    double qk = 0.0;
+    for (int f = 0; f < nf; ++f)
+      if ((e[3] >> f) & 1) qk += fq[f];
+    for (int d = 0; d < 3; ++d)
+      g.fpos[4 * k + d] = g.origin[d] + (double)e[0] * g.lat[d] + (double)e[1] * g.lat[3 + d] +
+                          (double)e[2] * g.lat[6 + d];
+    g.fpos[4 * k + 3] = qk;
+  }
   if (t < kNM) {
     int l, m;
     tdecode(t, l, m);
@@ -529,8 +544,9 @@ __global__ __launch_bounds__(kMuT) void lattice_mu_kernel(const double2* __restr
 // m, l = m..kLM by the recurrence).  Then the extrinsic correction
 // (:807-829): Redlack-Grindlay adds E += 2 pf D and phi += -2 pf r.D + pf Q
 // (pf = 2 pi / 3V, D the box dipole, Q the quadrupole trace, r relative to
-// the centre); the fictitious charges of every neighbour box (plain Coulomb
-// pair, no softening; 1/r by reciprocal square root) otherwise, when set.  Coefficients, recurrence constants, box vectors and charges are
+// the centre); the fictitious charges of every neighbour box, merged by
+// lattice point (plain Coulomb pair, no softening; 1/r by reciprocal square
+// root), otherwise, when set.  Coefficients, recurrence constants, box vectors and charges are
 // wave-uniform and come through the scalar cache.  Adds sign * (E, phi) to
 // the walk's results.
 constexpr int kFieldT = 256;
@@ -542,7 +558,6 @@ __global__ __launch_bounds__(kFieldT) void lattice_field_kernel(int64_t n, const
                                                                const double* __restrict__ coef,
                                                                const double* __restrict__ inv,
                                                                const double* __restrict__ fict,
-                                                               const double* __restrict__ vbox, int nb,
                                                                double sign, double* __restrict__ ex,
                                                                double* __restrict__ ey,
                                                                double* __restrict__ ez,
@@ -588,19 +603,14 @@ __global__ __launch_bounds__(kFieldT) void lattice_field_kernel(int64_t n, const
     e2 += 2.0 * pf * dz;
     phi += -2.0 * pf * (rx * dx + ry * dy + rz * dz) + pf * qt;
   }
-  const int nf = (int)fict[16];
-  for (int fi = 0; fi < nf; ++fi) {
-    const double fx = fict[4 * fi], fy = fict[4 * fi + 1], fz = fict[4 * fi + 2], fq = fict[4 * fi + 3];
-    for (int b = 0; b < nb; ++b) {
-      const double dx = (px - vbox[3 * b]) - fx;
-      const double dy = (py - vbox[3 * b + 1]) - fy;
-      const double dz = (pz - vbox[3 * b + 2]) - fz;
-      const double rd = rsqrt(dx * dx + dy * dy + dz * dz);
-      const double qrd = fq * rd;
-      const double rd3q = qrd * rd * rd;
-      phi += qrd;
-      e0 += rd3q * dx; e1 += rd3q * dy; e2 += rd3q * dz;
-    }
+  for (int k = 0; k < g.nfpt; ++k) {
+    const double* f = g.fpos + 4 * k;
+    const double dx = px - f[0], dy = py - f[1], dz = pz - f[2];
+    const double rd = rsqrt(dx * dx + dy * dy + dz * dz);
+    const double qrd = f[3] * rd;
+    const double rd3q = qrd * rd * rd;
+    phi += qrd;
+    e0 += rd3q * dx; e1 += rd3q * dy; e2 += rd3q * dz;
   }
   ex[i] += sign * e0;
   ey[i] += sign * e1;
@@ -704,21 +714,21 @@ void launch_lattice_sources(hipStream_t s, int64_t n, const double* x, const dou
 
 void launch_lattice_field(hipStream_t s, int64_t m, const double* x, const double* y, const double* z,
                           const LatticeGeom& g, const double* coef, const double* inv,
-                          const double* fict, const double* vbox, int nb, double sign, double* ex,
-                          double* ey, double* ez, double* pot) {
+                          const double* fict, double sign, double* ex, double* ey, double* ez,
+                          double* pot) {
   if (m <= 0) return;
   const unsigned fb = (unsigned)((m + kFieldT - 1) / kFieldT);
-  lattice_field_kernel<<<fb, kFieldT, 0, s>>>(m, x, y, z, g, coef, inv, fict, vbox, nb, sign, ex, ey,
-                                              ez, pot);
+  lattice_field_kernel<<<fb, kFieldT, 0, s>>>(m, x, y, z, g, coef, inv, fict, sign, ex, ey, ez,
+                                              pot);
 }
 
 void launch_lattice_far(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
                         const double* q, const LatticeGeom& g, const double2* ml, const double* fact,
-                        const double* inv, const double* vbox, int nb, double sign, double2* part,
+                        const double* inv, double sign, double2* part,
                         double* dpart, double2* om, double* dip, double2* mu, double* fict,
                         double* coef, double* ex, double* ey, double* ez, double* pot) {
   launch_lattice_sources(s, n, x, y, z, q, g, ml, fact, part, dpart, om, dip, mu, fict, coef);
-  launch_lattice_field(s, n, x, y, z, g, coef, inv, fict, vbox, nb, sign, ex, ey, ez, pot);
+  launch_lattice_field(s, n, x, y, z, g, coef, inv, fict, sign, ex, ey, ez, pot);
 }
 
 }  // namespace pepc

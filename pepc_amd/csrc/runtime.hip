@@ -7,9 +7,11 @@
 // sequences of kernel launches on the context's stream with no host round
 // trip; the host reads the scalars once, in pepc_gpu_sync().
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -105,6 +107,8 @@ struct Ctx {
   double* fact = nullptr;           // factorials 0..80
   double2 *ml = nullptr, *mstar = nullptr, *lstar = nullptr, *mu = nullptr, *lpart = nullptr;
   double *ldpart = nullptr, *fict = nullptr, *linv = nullptr, *ldip = nullptr, *lcoef = nullptr;
+  int* lfpt = nullptr;              // merged fictitious-charge points (LatticeGeom::fpt)
+  double* lfpos = nullptr;          // their positions and charges (LatticeGeom::fpos)
   double2* lom = nullptr;
   // sinks (traversal of other particles)
   int64_t scap = 0, sk_m = 0;
@@ -250,7 +254,7 @@ void free_all(Ctx* c) {
                   c->nadj, c->parent, c->leafof, c->aqueue, c->lcount, c->loff, c->lent,
                   c->ltotal, c->nparent, c->echunk, c->esyschunk, c->epartial, c->prec4,
                   c->vbox, c->fact, c->ml, c->mstar, c->lstar, c->mu, c->lpart, c->ldpart,
-                  c->fict, c->linv, c->ldip, c->lom, c->lcoef, c->sk_x, c->sk_y, c->sk_z,
+                  c->fict, c->linv, c->ldip, c->lom, c->lcoef, c->lfpt, c->lfpos, c->sk_x, c->sk_y, c->sk_z,
                   c->sk_res[0], c->sk_res[1], c->sk_res[2], c->sk_res[3], c->sk_res[4],
                   c->sk_io[0], c->sk_io[1], c->sk_io[2], c->sk_io[3], c->sk_io[4], c->sk_io[5],
                   c->sk_io[6], c->sk_io[7], c->sk_k0, c->sk_k1, c->sk_v0, c->sk_v1, c->sk_gfirst,
@@ -491,7 +495,7 @@ int traverse_device_impl(Ctx* c, double* ex, double* ey, double* ez, double* pot
   launch_walk(s, a, c->cfg.precision, c->walk_blocks, c->cfg.list_tile);
   if (c->periodic && c->far_field)
     launch_lattice_far(s, c->n, c->xs, c->ys, c->zs, c->qs, c->lgeom, c->ml, c->fact, c->linv,
-                       c->vbox, c->nbox, c->cfg.sign, c->lpart, c->ldpart, c->lom, c->ldip, c->mu,
+                       c->cfg.sign, c->lpart, c->ldpart, c->lom, c->ldip, c->mu,
                        c->fict, c->lcoef, ex, ey, ez, pot);
   RECORD(c, 3);
   launch_walk_stats(s, a);
@@ -629,8 +633,8 @@ int traverse_sinks_run(Ctx* c, const SinkBufs& b, int64_t m, const double* x, co
   if (c->periodic && c->far_field) {
     launch_lattice_sources(s, c->n, c->xs, c->ys, c->zs, c->qs, c->lgeom, c->ml, c->fact, c->lpart,
                            c->ldpart, c->lom, c->ldip, c->mu, c->fict, c->lcoef);
-    launch_lattice_field(s, m, tx, ty, tz, c->lgeom, c->lcoef, c->linv, c->fict, c->vbox, c->nbox,
-                         c->cfg.sign, out[0], out[1], out[2], out[3]);
+    launch_lattice_field(s, m, tx, ty, tz, c->lgeom, c->lcoef, c->linv, c->fict, c->cfg.sign,
+                         out[0], out[1], out[2], out[3]);
   }
   if (mode == PEPC_GPU_SINKS_GROUPED) {
     double* dst[5] = {ex, ey, ez, pot, work};
@@ -1741,6 +1745,43 @@ int pepc_gpu_set_periodic(void* ctx, const pepc_gpu_periodic* p) {
                                   c->lstar, 32, c->ml);
       HIPCHECK(c, hipGetLastError());
     }
+  }
+  // Fictitious charges (module_fmm_framework.f90:535-554): charge p of a box
+  // sits at origin + t_p, the last at the origin, and every neighbour box
+  // carries the set.  Merged by lattice point; where all of them meet they
+  // cancel (the last is minus the sum of the others) and are left out.
+  std::vector<int> fpt_h;
+  g.fpt = nullptr;
+  g.fpos = nullptr;
+  g.nfpt = 0;
+  if (far && corr == PEPC_GPU_EXTRINSIC_FICTCHARGE) {
+    std::vector<std::array<int, 3>> off;
+    for (int q = 0; q < 3; ++q) {
+      if (!g.per[q]) continue;
+      std::array<int, 3> e = {0, 0, 0};
+      e[q] = 1;
+      off.push_back(e);
+    }
+    off.push_back({0, 0, 0});
+    const int nf = (int)off.size(), all = (1 << nf) - 1;
+    std::map<std::array<int, 3>, int> mask;   // ordered: a fixed point order
+    for (int b = 0; b < nb; ++b)
+      for (int f = 0; f < nf; ++f) {
+        const std::array<int, 3> pt = {bx[3 * b] + off[f][0], bx[3 * b + 1] + off[f][1],
+                                       bx[3 * b + 2] + off[f][2]};
+        mask[pt] |= 1 << f;
+      }
+    for (const auto& kv : mask) {
+      if (kv.second == all) continue;
+      fpt_h.insert(fpt_h.end(), {kv.first[0], kv.first[1], kv.first[2], kv.second});
+    }
+    const int np = (int)fpt_h.size() / 4;
+    ALLOC(c, c->lfpt, 4 * (int64_t)np);
+    ALLOC(c, c->lfpos, 4 * (int64_t)np);
+    HIPCHECK(c, hipMemcpyAsync(c->lfpt, fpt_h.data(), sizeof(int) * fpt_h.size(), hipMemcpyHostToDevice, s));
+    g.fpt = c->lfpt;
+    g.fpos = c->lfpos;
+    g.nfpt = np;
   }
   HIPCHECK(c, hipStreamSynchronize(s));
   c->per = *p;
