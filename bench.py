@@ -97,6 +97,116 @@ def run_timestep(sys_, group, pack, steps, dt=1e-3):
     return ms, float(abs(tot[-1] - tot[0]) / abs(tot[0]))
 
 
+def _timed(fn, reps):
+    """Best device-synchronised wall time of fn() over reps runs (ms)."""
+    import torch
+    best = None
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) * 1e3
+        best = t if best is None else min(best, t)
+    return best
+
+
+def run_other_configs(steps, warmup):
+    """The other configurations of the solve, one record each (parity-test
+    cases, timed here for the measured-results table): the small FP64 system
+    with its 10 leapfrog steps, the clustered 8M gravity system, the batched
+    ensemble of 256 systems with its 50-step graph timestep, and the adjoint
+    (forward + backward through torch.autograd)."""
+    import torch
+
+    from pepc_amd import workloads
+    from pepc_amd.autograd import coulomb_field
+    from pepc_amd.solver import Solver
+    dev = torch.device("cuda", 0)
+    recs = []
+
+    # small system, FP64: one solve, then 10 leapfrog steps dt = 0.01
+    s = workloads.mini()
+    X = [torch.from_numpy(a).to(dev) for a in (s.x, s.y, s.z, s.q)]
+    sol = Solver(theta2=s.theta2, eps2=s.eps2, group_size=16, precision="fp64", sign=s.sign)
+    for _ in range(warmup):
+        sol.grow(*X, sync=False)
+        sol.traverse(sync=False)
+    ms = _timed(lambda: (sol.grow(*X, sync=False), sol.traverse(sync=False)), steps)
+    sol.state_set(s.x, s.y, s.z, s.v, s.q, s.m, max_records=16)
+    sol.run(1, 0.01, graph=True)
+    ms_step = _timed(lambda: sol.run(10, 0.01, graph=True, sync=False), 3) / 10
+    sol.close()
+    recs.append({"workload": "small_n32768_fp64", "n": s.n, "precision": "fp64", "group_size": 16,
+                 "solve_ms": ms, "updates_per_s": s.n / (ms * 1e-3),
+                 "leapfrog_ms_per_step": ms_step})
+
+    # clustered gravity, 8,388,608 particles, attractive sign, theta = 0.5
+    s = workloads.plummer()
+    X = [torch.from_numpy(a).to(dev) for a in (s.x, s.y, s.z, s.q)]
+    for g, pk in ((16, 32), (64, 64)):
+        sol = Solver(theta2=s.theta2, eps2=s.eps2, group_size=g, group_pack=pk, precision="fp32",
+                     sign=s.sign)
+        for _ in range(warmup):
+            sol.grow(*X, sync=False)
+            sol.traverse(sync=False)
+        ms = _timed(lambda: (sol.grow(*X, sync=False), sol.traverse(sync=False)), steps)
+        st = sol.sync()
+        recs.append({"workload": "plummer_n8388608_theta0.5", "n": s.n, "precision": "fp32",
+                     "group_size": g, "group_pack": pk, "solve_ms": ms,
+                     "updates_per_s": s.n / (ms * 1e-3), "interactions": int(st["interactions"]),
+                     "maxlevel": int(st["maxlevel"])})
+        sol.close()
+    del X
+
+    # batched ensemble: 256 systems, autotuned rule, one solve and the 50-step graph timestep
+    systems = workloads.ensemble()
+    x, y, z, q, m, v, off = workloads.concat(systems)
+    n = int(off[-1])
+    sol = Solver(theta2=0.36, eps2=1e-4, precision="fp32")
+    best, _ = sol.autotune(x, y, z, q, offsets=off, reps=1)
+    X = [torch.from_numpy(a).to(dev) for a in (x, y, z, q)]
+    for _ in range(warmup):
+        sol.grow_batch(*X, off, sync=False)
+        sol.traverse(sync=False)
+    ms = _timed(lambda: (sol.grow_batch(*X, off, sync=False), sol.traverse(sync=False)), steps)
+    sol.state_set(x, y, z, v, q, m, offsets=off, max_records=64)
+    sol.run(1, 1e-3, graph=True)
+    t50 = _timed(lambda: sol.run(50, 1e-3, graph=True, sync=False), 1)
+    cfg = sol.config()
+    sol.close()
+    recs.append({"workload": "ensemble_256_systems", "n": n, "systems": len(systems),
+                 "precision": "fp32", "group_size": cfg["group_size"],
+                 "group_pack": cfg["group_pack"], "solve_ms": ms,
+                 "updates_per_s": n / (ms * 1e-3), "timestep_50_ms": t50,
+                 "timestep_updates_per_s": 50 * n / (t50 * 1e-3)})
+    del X
+
+    # adjoint: forward + backward of L = sum |qE - F_t|^2 + sum (phi - phi_t)^2, FP64
+    s = workloads.adjoint()
+    kw = dict(theta2=s.theta2, eps2=s.eps2, group_size=64, group_pack=64, precision="fp64")
+    tgt = Solver(**kw)
+    Xt = [torch.from_numpy(a).to(dev) for a in (s.x, s.y, s.z)]
+    qt = torch.from_numpy(s.extra["q_target"]).to(dev)
+    et, pt = coulomb_field(*Xt, qt, tgt)
+    Ft, Pt = (qt[:, None] * et).detach(), pt.detach()
+    tgt.close()
+    sol = Solver(**kw)
+
+    def fwd_bwd():
+        Xg = [torch.from_numpy(a).to(dev).requires_grad_(True) for a in (s.x, s.y, s.z, s.q)]
+        e, pot = coulomb_field(*Xg, sol)
+        loss = ((Xg[3][:, None] * e - Ft) ** 2).sum() + ((pot - Pt) ** 2).sum()
+        loss.backward()
+
+    fwd_bwd()
+    ms = _timed(fwd_bwd, 3)
+    sol.close()
+    recs.append({"workload": "adjoint_n262144", "n": s.n, "precision": "fp64", "group_size": 64,
+                 "group_pack": 64, "forward_backward_ms": ms})
+    return recs
+
+
 def run_periodic(n, group, pack, precision, steps, warmup):
     """The periodic-boundary solve: a neutral +-1 plasma in the 3-D periodic
     unit cube (pepc-mini's charges), theta2 = 0.36, eps2 = 1e-4; each step is
@@ -292,6 +402,10 @@ def main():
     ap.add_argument("--periodic-only", action="store_true",
                     help="run only the periodic-boundary measurement (FP32, and FP64 unless "
                          "--no-fp64) and print its records")
+    ap.add_argument("--other-configs", action="store_true",
+                    help="time the other configurations (small FP64 system with its leapfrog "
+                         "steps, 8M clustered gravity, 256-system ensemble with its graph "
+                         "timestep, adjoint) and print one record each")
     ap.add_argument("--theta-legs", default="0.3",
                     help="comma-separated extra opening angles timed on the same system (FP32 "
                          "and FP64, fewer steps), empty for none")
@@ -306,6 +420,11 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.other_configs:
+        for rec in run_other_configs(args.steps, args.warmup):
+            print(json.dumps(rec), flush=True)
+        return
 
     if args.periodic_only:
         for prec in ("fp32",) if args.no_fp64 else ("fp32", "fp64"):
