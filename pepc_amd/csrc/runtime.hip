@@ -309,8 +309,8 @@ int ensure_particles(Ctx* c, int64_t n) {
   if (n <= c->ncap) return 0;
   const int64_t nc = n + n / 8 + 64;
   int64_t mc = (int64_t)(c->cfg.node_factor * (double)nc) + 64;
-  if (c->cfg.node_factor * (double)nc + 64.0 >= 4294967295.0)
-    return fail(c, PEPC_GPU_EINVAL, "node pool of node_factor * n nodes exceeds 32-bit node indices");
+  if (c->cfg.node_factor * (double)nc + 64.0 >= 2147483648.0)
+    return fail(c, PEPC_GPU_EINVAL, "node pool of node_factor * n nodes exceeds 31-bit node indices");
   if (mc < c->node_min) mc = c->node_min;
   if (mc < nc) mc = nc;   // every particle is a leaf (and the grow borrows node arrays of n entries)
   ALLOC(c, c->k0, nc); ALLOC(c, c->k1, nc); ALLOC(c, c->oldkeys, nc);

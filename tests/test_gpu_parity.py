@@ -176,12 +176,15 @@ def test_error_paths(gpu):
         sol.grow(s.x, s.y, s.z, s.q)
     assert ei.value.code == _lib.EOVERFLOW
     sol.close()
-    # a node pool beyond 32-bit node indices is refused before any allocation
-    sol = gpu(node_factor=1e6)
-    with pytest.raises(_lib.PepcGpuError) as ei:
-        sol.grow(s.x, s.y, s.z, s.q)
-    assert ei.value.code == _lib.EINVAL and "32-bit" in str(ei.value)
-    sol.close()
+    # a node pool beyond 31-bit node indices (the walk keeps the node kind in
+    # the top bit of its stack entries) is refused before any allocation,
+    # also between 2^31 and 2^32 nodes
+    for nf in (1e6, 1.2e5):
+        sol = gpu(node_factor=nf)
+        with pytest.raises(_lib.PepcGpuError) as ei:
+            sol.grow(s.x, s.y, s.z, s.q)
+        assert ei.value.code == _lib.EINVAL and "31-bit" in str(ei.value)
+        sol.close()
 
 
 @pytest.mark.parametrize("name", list(SYSTEMS))
