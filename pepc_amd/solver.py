@@ -83,16 +83,34 @@ class Solver:
             return t.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(self.device)
         return a.to(device=self.device, dtype=t.float64).contiguous()
 
+    def _grow_synced(self, launch):
+        """Runs a grow launch and waits for it.  A node pool too small for the
+        tree (the build counts its nodes before it stops) is enlarged to the
+        counted size and the grow runs again, as pepc_gpu_grow does."""
+        launch()
+        st = _lib.Stats()
+        rc = self.lib.pepc_gpu_sync(self.h, C.byref(st))
+        if rc == _lib.EOVERFLOW:
+            need = int(st.nnodes)
+            self._check(self.lib.pepc_gpu_reserve_nodes(self.h, need + need // 16 + 64))
+            launch()
+            rc = self.lib.pepc_gpu_sync(self.h, C.byref(st))
+        self._check(rc)
+        return st.as_dict()
+
     def grow(self, x, y, z, q, sync=True):
-        """Builds the tree.  Inputs: numpy arrays or device tensors (float64)."""
+        """Builds the tree.  Inputs: numpy arrays or device tensors (float64).
+        With sync, a node pool overflow enlarges the pool and grows again;
+        without, it is reported by the next sync."""
         self._bind_stream()
         xs = [self._dev(v) for v in (x, y, z, q)]
         self._inputs = xs      # keep alive until the stream has consumed them
         n = xs[0].numel()
-        self._check(self.lib.pepc_gpu_grow_device(self.h, n, *[_ptr(v) for v in xs]))
+        launch = lambda: self._check(self.lib.pepc_gpu_grow_device(self.h, n, *[_ptr(v) for v in xs]))
         self.n = n
         if sync:
-            return self.sync()
+            return self._grow_synced(launch)
+        launch()
         return None
 
     def grow_batch(self, x, y, z, q, offsets, sync=True):
@@ -105,12 +123,13 @@ class Solver:
         nsys = len(off) - 1
         if nsys < 1 or off[-1] != xs[0].numel():
             raise ValueError("grow_batch: offsets must have nsys + 1 entries ending at n")
-        self._check(self.lib.pepc_gpu_grow_batch_device(self.h, nsys, off.ctypes.data_as(C.c_void_p),
-                                                        *[_ptr(v) for v in xs]))
+        launch = lambda: self._check(self.lib.pepc_gpu_grow_batch_device(
+            self.h, nsys, off.ctypes.data_as(C.c_void_p), *[_ptr(v) for v in xs]))
         self.n = int(off[-1])
         self.offsets = off
         if sync:
-            return self.sync()
+            return self._grow_synced(launch)
+        launch()
         return None
 
     def traverse(self, out=None, sync=True):

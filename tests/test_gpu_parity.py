@@ -172,9 +172,15 @@ def test_error_paths(gpu):
     sol.close()
     sol = gpu(node_factor=1.0)   # too small a node pool for a real tree
     s = workloads.mini(n=20000)
+    sol.grow(s.x, s.y, s.z, s.q, sync=False)   # the device form reports it at the next sync
     with pytest.raises(_lib.PepcGpuError) as ei:
-        sol.grow(s.x, s.y, s.z, s.q)
+        sol.sync()
     assert ei.value.code == _lib.EOVERFLOW
+    sol.close()
+    # a synchronous grow enlarges the pool to the counted size and grows again
+    sol = gpu(node_factor=1.0)
+    st = sol.grow(s.x, s.y, s.z, s.q)
+    assert st["nleaf"] == s.n and st["nnodes"] > 1.0 * (s.n + s.n // 8 + 64)
     sol.close()
     # a node pool beyond 31-bit node indices (the walk keeps the node kind in
     # the top bit of its stack entries) is refused before any allocation,
