@@ -3,7 +3,9 @@
 group rule, interaction sign and softening, each solved through the C-ABI and
 by the oracle.  Keys, permutation, fixed-up positions, per-target P2P and M2P
 counts, per-group MAC counts and work must be bit-exact; FP64 fields and
-potentials agree to 1e-12 of their maximum."""
+potentials agree to 1e-12 of their maximum.  The FP32 + Kahan path keeps the
+integer results bit-exact and its fields within a per-target rounding
+bound."""
 import numpy as np
 import pytest
 
@@ -71,4 +73,73 @@ def test_random_parity(gpu, orc, seed):
     pot = res.pot.cpu().numpy()
     assert np.max(np.abs(e - w.e)) <= 1e-12 * max(np.abs(w.e).max(), 1e-300)
     assert np.max(np.abs(pot - w.pot)) <= 1e-12 * max(np.abs(w.pot).max(), 1e-300)
+    sol.close()
+
+
+def _rounding_scale(x, y, z, q, eps2, chunk=512):
+    """Per target, the sums a rounding bound scales with:
+    A_i = sum_j |q_j| / r_ij^3, B_i = sum_j |q_j| / r_ij^2 and
+    C_i = sum_j |q_j| / r_ij (r^2 = d^2 + eps2, self excluded), in FP64."""
+    n = len(x)
+    aq = np.abs(q)
+    A, B, C = (np.zeros(n) for _ in range(3))
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        r2 = ((x[s:e, None] - x[None]) ** 2 + (y[s:e, None] - y[None]) ** 2
+              + (z[s:e, None] - z[None]) ** 2 + eps2)
+        r2[np.arange(e - s), np.arange(s, e)] = np.inf
+        ir = 1.0 / np.sqrt(r2)
+        C[s:e] = ir @ aq
+        ir2 = ir * ir
+        B[s:e] = ir2 @ aq
+        A[s:e] = (ir2 * ir) @ aq
+    return A, B, C
+
+
+@pytest.mark.parametrize("seed", range(CASES))
+def test_random_parity_fp32(gpu, orc, seed):
+    """The FP32 + Kahan path on the same systems.  MAC decisions are FP64 in
+    both paths, so groups, per-target counts, MAC counts and work are
+    bit-exact.  Fields: coordinates live in FP32 relative to the group's first
+    particle, so a separation d carries an error of order u L (u = 2^-24, L
+    the system's extent), which moves d/r^3 by up to about 3 sqrt(3) u L / r^3,
+    and each term's arithmetic (rsqrt to about one ulp, cubed) and the
+    compensated sums add a few u times its size.  Per target,
+    |dE| <= K u (L A + B) and |dphi| <= K u (L B + C) with K = 16 (A, B, C as
+    in _rounding_scale; the sweep's largest ratio is 2.4, a lone pair); a
+    dropped or mis-signed term of the expansion exceeds this by orders of
+    magnitude.  Coincident particles without softening, a separation of one
+    key cell after the fix-up, reach a relative error of a few 1e-2 inside
+    that bound.  The median relative field error is below 1e-5."""
+    p, q, c = _case(seed)
+    x, y, z = (p[:, k].copy() for k in range(3))
+    print(f"[fp32 seed {seed}] n={len(q)} {c}", flush=True)
+    d = orc.domain_decompose(x, y, z, q)
+    t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
+    w = orc.walk_group(d.x, d.y, d.z, t, c["theta2"], c["eps2"], tile_size=c["G"],
+                       sign=c["sign"], keys=d.keys, pack=c["pack"])
+    sol = gpu(theta2=c["theta2"], eps2=c["eps2"], group_size=c["G"], precision="fp32",
+              sign=c["sign"], group_pack=c["pack"])
+    sol.grow(x, y, z, q)
+    res = sol.traverse()
+    wk = sol.read_walk()
+    assert np.array_equal(wk["gfirst"].astype(np.int64), w.tile_first)
+    assert np.array_equal(wk["p2p"].astype(np.int32), w.p2p)
+    assert np.array_equal(wk["m2p"].astype(np.int32), w.m2p)
+    assert np.array_equal(wk["gmac"].astype(np.int64), w.tile_mac)
+    assert np.array_equal(res.work.cpu().numpy(), w.work)
+    e = res.e.cpu().numpy().T
+    pot = res.pot.cpu().numpy()
+    A, B, C = _rounding_scale(d.x, d.y, d.z, d.q, c["eps2"])
+    L = float(np.max(d.box.as_arrays()[2]))
+    u = 2.0 ** -24
+    de = np.linalg.norm(e - w.e, axis=1)
+    dp = np.abs(pot - w.pot)
+    ke = de / (u * (L * A + B))
+    kp = dp / (u * (L * B + C))
+    rel = de / np.maximum(np.linalg.norm(w.e, axis=1), 1e-300)
+    print(f"  dE/(u(LA+B)) max {ke.max():.2e}  dphi/(u(LB+C)) max {kp.max():.2e}  "
+          f"rel median {np.median(rel):.1e} max {rel.max():.1e}", flush=True)
+    assert ke.max() <= 16.0 and kp.max() <= 16.0
+    assert np.median(rel) <= 1e-5
     sol.close()
