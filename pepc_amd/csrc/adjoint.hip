@@ -85,7 +85,66 @@ __device__ void adjoint_term(const double d[3], double r2, double gphi, const do
   }
 }
 
-__device__ __forceinline__ double wsum(double v) { return wave_sum(v); }
+// Lane k's value in every lane (k uniform): register reads, no LDS.
+__device__ __forceinline__ uint32_t lane_bcast(uint32_t v, int k) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, k);
+}
+__device__ __forceinline__ double lane_bcast(double v, int k) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
+  return __hiloint2double(hi, lo);
+}
+
+// v + (v of the lane that the DPP control selects), 64-bit as two halves.
+template <int kCtrl>
+__device__ __forceinline__ double dpp_add(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xF, 0xF, false);
+  return v + __hiloint2double(hi, lo);
+}
+
+// Exchanges between two registers, 64-bit as two halves:
+//   swap32: x = [x lanes 0-31, y lanes 0-31], y = [x lanes 32-63, y lanes 32-63];
+//   swap16: x = [x row 0, y row 0, x row 2, y row 2], y = [x row 1, y row 1,
+//           x row 3, y row 3] (rows of 16 lanes).
+__device__ __forceinline__ void swap32(double& x, double& y) {
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(x),
+                                                   (unsigned)__double2loint(y), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(x),
+                                                   (unsigned)__double2hiint(y), false, false);
+  x = __hiloint2double((int)hi[0], (int)lo[0]);
+  y = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+__device__ __forceinline__ void swap16(double& x, double& y) {
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(x),
+                                                   (unsigned)__double2loint(y), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(x),
+                                                   (unsigned)__double2hiint(y), false, false);
+  x = __hiloint2double((int)hi[0], (int)lo[0]);
+  y = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
+// Sum over the 16 lanes of each row, in every lane of the row.
+__device__ __forceinline__ double row_sum(double v) {
+  v = dpp_add<0xB1>(v);    // quad_perm [1,0,3,2]
+  v = dpp_add<0x4E>(v);    // quad_perm [2,3,0,1]
+  v = dpp_add<0x124>(v);   // row_ror:4
+  return dpp_add<0x128>(v);   // row_ror:8
+}
+
+// Wave sums of four values at once: the 32-lane halves of values 0 and 1
+// (and of 2 and 3) are exchanged and added, then the rows of the two
+// results, then each row is summed by DPP.  Row r of the result holds the sum
+// of value row_value(r) = 0, 2, 1, 3 for r = 0..3; every lane of the row has
+// it.
+__device__ __forceinline__ int row_value(int r) { return ((r & 1) << 1) | (r >> 1); }
+__device__ __forceinline__ double wsum4(double v0, double v1, double v2, double v3) {
+  swap32(v0, v1);
+  swap32(v2, v3);
+  double x = v0 + v1, y = v2 + v3;
+  swap16(x, y);
+  return row_sum(x + y);
+}
 
 // The adjoint buffers are plain device allocations (coarse grained), where the
 // hardware FP64 add atomic is exact.
@@ -134,11 +193,11 @@ __device__ bool traverse_group(uint32_t root, double xt, double yt, double zt, b
     __syncthreads();
     sp -= m;
     for (uint32_t k = 0; k < m; ++k) {
-      const uint32_t node = __shfl(mynode, (int)k);
-      const uint32_t first = __shfl(mylk.first, (int)k);
-      const uint32_t nchild = __shfl(mylk.nchild, (int)k);
-      const double cx = __shfl(mygm.cx, (int)k), cy = __shfl(mygm.cy, (int)k),
-                   cz = __shfl(mygm.cz, (int)k), b2 = __shfl(mygm.b2, (int)k);
+      const uint32_t node = lane_bcast(mynode, (int)k);
+      const uint32_t first = lane_bcast(mylk.first, (int)k);
+      const uint32_t nchild = lane_bcast(mylk.nchild, (int)k);
+      const double cx = lane_bcast(mygm.cx, (int)k), cy = lane_bcast(mygm.cy, (int)k),
+                   cz = lane_bcast(mygm.cz, (int)k), b2 = lane_bcast(mygm.b2, (int)k);
       if (nchild != 0u) {
         bool fail;
         {
@@ -176,8 +235,15 @@ struct AdjArgs {
   uint32_t* queue;
 };
 
+// The traversal is traverse_group's, written out so that each step also
+// stages the popped nodes' interaction data in LDS with the node records (a
+// twig's charge, dipole and quadrupole, a leaf's particle charge): the
+// entries of a step then read LDS only, and the step's one wait on memory
+// also drains the previous step's atomics.  A node's 13 sums (4 for a leaf's
+// particle) go out as one atomic instruction, lane j adding value j.
 __global__ __launch_bounds__(kWave) void adjoint_walk_kernel(AdjArgs a) {
   __shared__ uint32_t stack[kStackCap + kStackSlack];
+  __shared__ double smom[kWave][10];   // q, p[3], Qt[6] of popped node k
   __shared__ uint32_t gsh;
   const int lane = threadIdx.x;
   const uint32_t ngroups = a.sc->ngroups;
@@ -197,49 +263,100 @@ __global__ __launch_bounds__(kWave) void adjoint_walk_kernel(AdjArgs a) {
                          valid ? a.sign * a.gez[tid] : 0.0};
     double tD[3] = {0.0, 0.0, 0.0};
     const uint32_t root = a.sys ? a.sysroot[a.sys[a0]] : 0u;
-    auto entry = [&](uint32_t node, uint32_t first, uint32_t nchild, double cx, double cy,
-                     double cz) {
-      double d[3] = {xt - cx, yt - cy, zt - cz};
-      const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + eps2;
-      double D[3], gq, gp[3], M[6];
-      if (nchild == 0u) {   // leaf: P2P with source particle `first`
-        const double q = a.q[first];
-        const bool self = first == tid || !valid;
-        const double p0[3] = {0.0, 0.0, 0.0}, q0[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        adjoint_term(d, self ? 1.0 : r2, gphi, u, q, p0, q0, D, gq, gp, M);
-        if (self) { D[0] = D[1] = D[2] = 0.0; gq = 0.0; }
-        tD[0] += D[0]; tD[1] += D[1]; tD[2] += D[2];
-        const double s0 = wsum(D[0]), s1 = wsum(D[1]), s2 = wsum(D[2]), s3 = wsum(gq);
-        if (lane == 0) {
-          unsafeAtomicAdd(&a.gx[first], -s0);
-          unsafeAtomicAdd(&a.gy[first], -s1);
-          unsafeAtomicAdd(&a.gz[first], -s2);
-          unsafeAtomicAdd(&a.gq[first], s3);
+    bool overflow = false;
+    uint32_t sp = 1;
+    if (lane == 0) stack[0] = root;
+    __syncthreads();
+    while (sp > 0 && !overflow) {
+      int room = ((int)kStackCap - (int)sp) / 7;
+      room = room < 1 ? 1 : room;
+      uint32_t m = sp < (uint32_t)room ? sp : (uint32_t)room;
+      m = m > (uint32_t)kWave ? (uint32_t)kWave : m;
+      const bool has = (uint32_t)lane < m;
+      const uint32_t mynode = has ? stack[sp - 1 - lane] : 0u;
+      NodeLink mylk = {0u, 0u, 0u, 0u};
+      NodeGeom mygm = {0.0, 0.0, 0.0, 0.0};
+      if (has) { mylk = a.link[mynode]; mygm = a.geom[mynode]; }
+      if (has) {
+        if (mylk.nchild != 0u) {
+          const double* mm = a.mom64 + (size_t)mynode * 16;
+          smom[lane][0] = mm[3];
+#pragma unroll
+          for (int j = 0; j < 3; ++j) smom[lane][1 + j] = mm[5 + j];
+#pragma unroll
+          for (int j = 0; j < 6; ++j) smom[lane][4 + j] = mm[8 + j];
+        } else {
+          smom[lane][0] = a.q[mylk.first];
         }
-        return;
       }
-      const double* mm = a.mom64 + (size_t)node * 16;
-      const double q = mm[3];
-      const double p[3] = {mm[5], mm[6], mm[7]};
-      const double qt[6] = {mm[8], mm[9], mm[10], mm[11], mm[12], mm[13]};
-      adjoint_term(d, r2, gphi, u, q, p, qt, D, gq, gp, M);
-      if (!valid) {
-        D[0] = D[1] = D[2] = 0.0; gq = 0.0; gp[0] = gp[1] = gp[2] = 0.0;
+      __syncthreads();
+      sp -= m;
+      for (uint32_t k = 0; k < m && !overflow; ++k) {
+        const uint32_t node = lane_bcast(mynode, (int)k);
+        const uint32_t first = lane_bcast(mylk.first, (int)k);
+        const uint32_t nchild = lane_bcast(mylk.nchild, (int)k);
+        const double cx = lane_bcast(mygm.cx, (int)k), cy = lane_bcast(mygm.cy, (int)k),
+                     cz = lane_bcast(mygm.cz, (int)k), b2 = lane_bcast(mygm.b2, (int)k);
+        if (nchild != 0u) {
+          bool fail;
+          {
+#pragma clang fp contract(off)
+            const double dx = xt - cx, dy = yt - cy, dz = zt - cz;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            fail = valid && !(a.theta2 * d2 > b2);
+          }
+          if (__ballot(fail) != 0ull) {
+            if (sp + nchild > (uint32_t)(kStackCap + kStackSlack)) { overflow = true; break; }
+            if ((uint32_t)lane < nchild) stack[sp + lane] = first + (uint32_t)lane;
+            sp += nchild;
+            continue;
+          }
+        }
+        double d[3] = {xt - cx, yt - cy, zt - cz};
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + eps2;
+        double D[3], gq, gp[3], M[6];
+        if (nchild == 0u) {   // leaf: P2P with source particle `first`
+          const double q = smom[k][0];
+          const bool self = first == tid || !valid;
+          const double p0[3] = {0.0, 0.0, 0.0}, q0[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+          adjoint_term(d, self ? 1.0 : r2, gphi, u, q, p0, q0, D, gq, gp, M);
+          if (self) { D[0] = D[1] = D[2] = 0.0; gq = 0.0; }
+          tD[0] += D[0]; tD[1] += D[1]; tD[2] += D[2];
+          // values -D (to the source particle's position) and gq; lane 16 r
+          // adds value row_value(r)
+          const double sv = wsum4(-D[0], -D[1], -D[2], gq);
+          if ((lane & 15) == 0) {
+            const int j = row_value(lane >> 4);
+            double* dst = j == 0 ? a.gx : j == 1 ? a.gy : j == 2 ? a.gz : a.gq;
+            unsafeAtomicAdd(&dst[first], sv);
+          }
+          continue;
+        }
+        const double q = smom[k][0];
+        const double p[3] = {smom[k][1], smom[k][2], smom[k][3]};
+        const double qt[6] = {smom[k][4], smom[k][5], smom[k][6], smom[k][7], smom[k][8], smom[k][9]};
+        adjoint_term(d, r2, gphi, u, q, p, qt, D, gq, gp, M);
+        if (!valid) {
+          D[0] = D[1] = D[2] = 0.0; gq = 0.0; gp[0] = gp[1] = gp[2] = 0.0;
 #pragma unroll
-        for (int j = 0; j < 6; ++j) M[j] = 0.0;
+          for (int j = 0; j < 6; ++j) M[j] = 0.0;
+        }
+        tD[0] += D[0]; tD[1] += D[1]; tD[2] += D[2];
+        // the node's 13 sums (slots 0..12): register c holds values 4c..4c+3,
+        // and lane 16 r + c adds value 4c + row_value(r)
+        const double s0 = wsum4(gq, gp[0], gp[1], gp[2]);
+        const double s1 = wsum4(M[0], M[1], M[2], M[3]);
+        const double s2 = wsum4(M[4], M[5], D[0], D[1]);
+        const double s3 = wsum4(D[2], 0.0, 0.0, 0.0);
+        const int c = lane & 15;
+        const int j = 4 * c + row_value(lane >> 4);
+        if (c < 4 && j < 13) {
+          const double sv = c == 0 ? s0 : c == 1 ? s1 : c == 2 ? s2 : s3;
+          unsafeAtomicAdd(&a.nadj[(size_t)node * 16 + j], sv);
+        }
       }
-      tD[0] += D[0]; tD[1] += D[1]; tD[2] += D[2];
-      double v[13] = {gq, gp[0], gp[1], gp[2], M[0], M[1], M[2], M[3], M[4], M[5], D[0], D[1], D[2]};
-#pragma unroll
-      for (int j = 0; j < 13; ++j) v[j] = wsum(v[j]);
-      if (lane == 0) {
-        double* o = a.nadj + (size_t)node * 16;
-#pragma unroll
-        for (int j = 0; j < 13; ++j) unsafeAtomicAdd(&o[j], v[j]);
-      }
-    };
-    const bool overflow = traverse_group(root, xt, yt, zt, valid, a.theta2, a.link, a.geom, stack,
-                                         entry);
+      __syncthreads();
+    }
     if (overflow && lane == 0) atomicOr(&a.sc->flags, kFlagStackOverflow);
     if (valid) {
       unsafeAtomicAdd(&a.gx[tid], tD[0]);
