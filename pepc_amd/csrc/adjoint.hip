@@ -42,9 +42,12 @@ __device__ __forceinline__ bool broken(const DevScalars* sc) {
 // Derivatives of gphi * phi + u . E for one interaction (see header comment).
 // qt = {qx, qy, qz, xy, yz, zx} (Qt_xx = qx, Qt_xy = xy / 2, ...).  Outputs:
 // D = dL/dd, gq, gp, M = dL/dQt as the symmetric matrix {xx, yy, zz, xy, yz, zx}.
+// The adjoint is checked against the oracle to a tolerance, so its terms may
+// contract products and sums into fused multiply-adds.
 __device__ void adjoint_term(const double d[3], double r2, double gphi, const double u[3],
                              double q, const double p[3], const double qt[6], double D[3],
                              double& gq, double gp[3], double M[6]) {
+#pragma clang fp contract(fast)
   const double ri = rsqrt(r2);
   const double ri2 = ri * ri;
   const double r3 = ri * ri2, r5 = r3 * ri2, r7 = r5 * ri2, r9 = r7 * ri2;
@@ -83,6 +86,20 @@ __device__ void adjoint_term(const double d[3], double r2, double gphi, const do
           105.0 * r9 * d[k] * ud * dQd);
     D[k] = v;
   }
+}
+
+// The same for a leaf (p = Qt = 0): D = dL/dd and gq only.
+__device__ __forceinline__ void adjoint_term_p2p(const double d[3], double r2, double gphi,
+                                                 const double u[3], double q, double D[3],
+                                                 double& gq) {
+#pragma clang fp contract(fast)
+  const double ri = rsqrt(r2);
+  const double ri2 = ri * ri;
+  const double r3 = ri * ri2, r5 = r3 * ri2;
+  const double ud = u[0] * d[0] + u[1] * d[1] + u[2] * d[2];
+  gq = gphi * ri + ud * r3;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) D[k] = q * ((u[k] - gphi * d[k]) * r3 - 3.0 * ud * d[k] * r5);
 }
 
 // Lane k's value in every lane (k uniform): register reads, no LDS.
@@ -241,7 +258,7 @@ struct AdjArgs {
 // entries of a step then read LDS only, and the step's one wait on memory
 // also drains the previous step's atomics.  A node's 13 sums (4 for a leaf's
 // particle) go out as one atomic instruction, lane j adding value j.
-__global__ __launch_bounds__(kWave) void adjoint_walk_kernel(AdjArgs a) {
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void adjoint_walk_kernel(AdjArgs a) {
   __shared__ uint32_t stack[kStackCap + kStackSlack];
   __shared__ double smom[kWave][10];   // q, p[3], Qt[6] of popped node k
   __shared__ uint32_t gsh;
@@ -318,8 +335,7 @@ __global__ __launch_bounds__(kWave) void adjoint_walk_kernel(AdjArgs a) {
         if (nchild == 0u) {   // leaf: P2P with source particle `first`
           const double q = smom[k][0];
           const bool self = first == tid || !valid;
-          const double p0[3] = {0.0, 0.0, 0.0}, q0[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-          adjoint_term(d, self ? 1.0 : r2, gphi, u, q, p0, q0, D, gq, gp, M);
+          adjoint_term_p2p(d, self ? 1.0 : r2, gphi, u, q, D, gq);
           if (self) { D[0] = D[1] = D[2] = 0.0; gq = 0.0; }
           tD[0] += D[0]; tD[1] += D[1]; tD[2] += D[2];
           // values -D (to the source particle's position) and gq; lane 16 r
