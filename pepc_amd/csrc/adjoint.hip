@@ -265,24 +265,49 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void
   const int lane = threadIdx.x;
   const uint32_t ngroups = a.sc->ngroups;
   const double eps2 = a.eps2;
+  // Work units: a group's walk is cut at its root into one unit per child of
+  // the root (the acceptance decisions do not depend on the order of the
+  // walk, so the units' entries are the group's), which evens out the end of
+  // the launch; unit 0 takes the root itself when the root is an interaction.
+  const uint32_t split = ngroups <= (1u << 28) ? 8u : 1u;
+  const uint32_t nunits = ngroups * split;
   for (;;) {
     if (lane == 0) gsh = atomicAdd(a.queue, 1u);
     __syncthreads();
-    const uint32_t g = gsh;
+    const uint32_t unit = gsh;
     __syncthreads();
-    if (g >= ngroups) break;
+    if (unit >= nunits) break;
+    const uint32_t g = unit / split, part = unit - g * split;
     const uint32_t a0 = a.gfirst[g], cnt = a.gcount[g];
     const bool valid = (uint32_t)lane < cnt;
     const uint32_t tid = a0 + (valid ? (uint32_t)lane : 0u);
     const double xt = a.x[tid], yt = a.y[tid], zt = a.z[tid];
+    const uint32_t root = a.sys ? a.sysroot[a.sys[a0]] : 0u;
+    uint32_t start = root;
+    if (split > 1) {
+      const NodeLink rl = a.link[root];
+      bool open = false;
+      if (rl.nchild != 0u) {
+        const NodeGeom rg = a.geom[root];
+        bool fail;
+        {
+#pragma clang fp contract(off)
+          const double dx = xt - rg.cx, dy = yt - rg.cy, dz = zt - rg.cz;
+          const double d2 = dx * dx + dy * dy + dz * dz;
+          fail = valid && !(a.theta2 * d2 > rg.b2);
+        }
+        open = __ballot(fail) != 0ull;
+      }
+      if (open ? part >= rl.nchild : part != 0u) continue;   // no such unit
+      if (open) start = rl.first + part;
+    }
     const double gphi = valid ? a.sign * a.gpot[tid] : 0.0;
     const double u[3] = {valid ? a.sign * a.gex[tid] : 0.0, valid ? a.sign * a.gey[tid] : 0.0,
                          valid ? a.sign * a.gez[tid] : 0.0};
     double tD[3] = {0.0, 0.0, 0.0};
-    const uint32_t root = a.sys ? a.sysroot[a.sys[a0]] : 0u;
     bool overflow = false;
     uint32_t sp = 1;
-    if (lane == 0) stack[0] = root;
+    if (lane == 0) stack[0] = start;
     __syncthreads();
     while (sp > 0 && !overflow) {
       int room = ((int)kStackCap - (int)sp) / 7;

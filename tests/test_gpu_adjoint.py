@@ -131,7 +131,10 @@ def test_autograd_backward_and_gradcheck(gpu):
         e, pot = coulomb_field(x, y, z, qq, sol0)
         return e, pot
 
-    assert torch.autograd.gradcheck(f, inputs, eps=1e-6, atol=1e-5, rtol=1e-4)
+    # the adjoint sums its contributions with FP64 atomics from concurrent
+    # waves, so two backward passes agree to rounding, not bit for bit
+    assert torch.autograd.gradcheck(f, inputs, eps=1e-6, atol=1e-5, rtol=1e-4,
+                                    nondet_tol=1e-10)
     sol0.close()
 
 
