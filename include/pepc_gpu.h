@@ -289,8 +289,10 @@ int pepc_gpu_run(void *ctx, int64_t nsteps, double dt, int flags);
  * arrays, sorted order), writes gx,gy,gz = dL/dx and gq = dL/dq (sorted
  * order), FP64, with the tree and the interaction lists of the last grow held
  * fixed (the lists are piecewise constant in the positions).  Particles moved
- * by the duplicate-key fix-up are treated as fixed inputs.  Asynchronous on
- * the context's stream. */
+ * by the duplicate-key fix-up are treated as fixed inputs.  The sums are
+ * FP64 atomic additions from concurrent wavefronts, so two calls on the same
+ * inputs agree to rounding, not bit for bit.  Asynchronous on the context's
+ * stream. */
 int pepc_gpu_adjoint_device(void *ctx, const double *gex, const double *gey, const double *gez,
                             const double *gpot, double *gx, double *gy, double *gz,
                             double *gq);
