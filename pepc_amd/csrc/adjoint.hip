@@ -257,8 +257,11 @@ struct AdjArgs {
 // twig's charge, dipole and quadrupole, a leaf's particle charge): the
 // entries of a step then read LDS only, and the step's one wait on memory
 // also drains the previous step's atomics.  A node's 13 sums (4 for a leaf's
-// particle) go out as one atomic instruction, lane j adding value j.
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4))) void adjoint_walk_kernel(AdjArgs a) {
+// particle) are reduce-scattered across the wave (wsum4) and go out as one
+// atomic instruction, one value per lane.  Four waves per SIMD: the 128-VGPR
+// budget costs a few bytes of spill and hides more of each step's latency.
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4)))
+void adjoint_walk_kernel(AdjArgs a) {
   __shared__ uint32_t stack[kStackCap + kStackSlack];
   __shared__ double smom[kWave][10];   // q, p[3], Qt[6] of popped node k
   __shared__ uint32_t gsh;
