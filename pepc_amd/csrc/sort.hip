@@ -128,6 +128,7 @@ constexpr int kSortItems = 8;
 constexpr int kSortWaves = kSortThreads / kWave;
 constexpr int kSortTile = kSortThreads * kSortItems;   // 2048
 constexpr int kRadix = 256;
+constexpr int kScanHereBlocks = 64;   // up to 131,072 elements: no separate digit scan
 
 // The digit of element e is bits [shift, shift+nbits) of its key, or, when
 // dsrc is given, of dsrc[value] (used to order by system after the keys).
@@ -162,13 +163,14 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     unsigned long long* __restrict__ kout, uint32_t* __restrict__ vout,
     const uint32_t* __restrict__ dsrc, int64_t n,
     const uint32_t* __restrict__ ndev, int shift, int nbits, const uint32_t* __restrict__ hscan,
-    const uint32_t* __restrict__ dtot, int nblocks) {
+    const uint32_t* __restrict__ dtot, int nblocks, bool scan_here) {
   if (ndev && (int64_t)*ndev < n) n = *ndev;
   const int64_t tile0 = (int64_t)blockIdx.x * kSortTile;
   if (tile0 >= n) return;
   __shared__ uint32_t wcnt[kSortWaves][kRadix];
   __shared__ uint32_t lbase[kRadix];
   __shared__ uint32_t gbase[kRadix];
+  __shared__ uint32_t bbase[kRadix];   // digit d's elements in the blocks before this one
   __shared__ unsigned long long skey[kSortTile];
   __shared__ uint32_t sval[kSortTile];
   __shared__ uint8_t sdig[kSortTile];
@@ -219,8 +221,25 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     }
     uint32_t tot;
     lbase[d] = block_exclusive_scan(run, &tot);
+    // digit d's total and its count in earlier blocks: from the digit scan,
+    // or, for a few blocks (scan_here), from the histogram row of digit d
+    uint32_t all, before;
+    if (scan_here) {
+      const uint32_t* row = hscan + (int64_t)d * nblocks;
+      all = 0;
+      before = 0;
+      for (int b = 0; b < nblocks; ++b) {
+        const uint32_t v = row[b];
+        before += b < (int)blockIdx.x ? v : 0u;
+        all += v;
+      }
+    } else {
+      all = dtot[d];
+      before = hscan[(int64_t)d * nblocks + blockIdx.x];
+    }
+    bbase[d] = before;
     // global start of digit d: digits below d over all blocks
-    gbase[d] = block_exclusive_scan(dtot[d], &tot);
+    gbase[d] = block_exclusive_scan(all, &tot);
   }
   __syncthreads();
 #pragma unroll
@@ -238,8 +257,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   const int cnt = rem < kSortTile ? (int)rem : kSortTile;
   for (int i = threadIdx.x; i < cnt; i += kSortThreads) {
     const uint32_t d = sdig[i];
-    const uint32_t pos =
-        gbase[d] + hscan[(int64_t)d * nblocks + blockIdx.x] + ((uint32_t)i - lbase[d]);
+    const uint32_t pos = gbase[d] + bbase[d] + ((uint32_t)i - lbase[d]);
     kout[pos] = skey[i];
     vout[pos] = sval[i];
   }
@@ -633,10 +651,12 @@ void launch_radix_pass_dev(hipStream_t s, const unsigned long long* kin, const u
   const int nb = (int)sort_blocks(n);
   radix_hist_kernel<<<nb, kSortThreads, 0, s>>>(kin, vin, dsrc, n, ndev, shift, (1u << nbits) - 1u,
                                                 hist, nb);
-  // bsum[0..255] holds the digit totals
-  digit_scan_kernel<<<kRadix, kSortThreads, 0, s>>>(hist, nb, bsum);
+  // Few blocks: each scatter block sums its digits' rows itself, one launch
+  // fewer per pass.  Otherwise bsum[0..255] holds the digit totals.
+  const bool scan_here = nb <= kScanHereBlocks;
+  if (!scan_here) digit_scan_kernel<<<kRadix, kSortThreads, 0, s>>>(hist, nb, bsum);
   radix_scatter_kernel<<<nb, kSortThreads, 0, s>>>(kin, vin, kout, vout, dsrc, n, ndev, shift, nbits,
-                                                   hist, bsum, nb);
+                                                   hist, bsum, nb, scan_here);
 }
 
 void launch_radix_pass(hipStream_t s, const unsigned long long* kin, const uint32_t* vin,
