@@ -158,6 +158,9 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(
 // Ranks each element within its block (stable), stages the block's elements
 // in LDS in digit order, then writes them out so that consecutive threads
 // store consecutive positions of one digit's run (coalesced stores).
+// hscan is the digit-major table of per-block counts: exclusive-scanned along
+// each digit's row by digit_scan_kernel, with the row totals in dtot, or,
+// with scan_here, the histogram itself (dtot unused).
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const unsigned long long* __restrict__ kin, const uint32_t* __restrict__ vin,
     unsigned long long* __restrict__ kout, uint32_t* __restrict__ vout,
