@@ -103,3 +103,72 @@ def test_random_periodic_parity(gpu, P, seed):
         dp = dp - dp.mean()
     assert np.abs(dp).max() <= 1e-11 * np.abs(w.pot).max()
     sol.close()
+
+
+def _mirror_scale(x, y, z, q, vb, eps2, chunk=256):
+    """Per target, B_i = sum |q_j| / r^2 and A_i = sum |q_j| / r^3 over every
+    source image x_j + v of the mirror boxes v (r^2 = d^2 + eps2; the target's
+    own particle in the central box left out), in FP64."""
+    n = len(x)
+    aq = np.abs(q)
+    A, B = np.zeros(n), np.zeros(n)
+    for v in vb:
+        central = not np.any(v)
+        for a in range(0, n, chunk):
+            b = min(n, a + chunk)
+            d2 = ((x[a:b, None] - (x + v[0])) ** 2 + (y[a:b, None] - (y + v[1])) ** 2
+                  + (z[a:b, None] - (z + v[2])) ** 2)
+            r2 = d2 + eps2
+            if central:
+                r2[np.arange(b - a), np.arange(a, b)] = np.inf
+            ir2 = 1.0 / r2
+            B[a:b] += ir2 @ aq
+            A[a:b] += (ir2 * np.sqrt(ir2)) @ aq
+    return A, B
+
+
+FP32_CASES = 8
+
+
+@pytest.mark.parametrize("seed", range(FP32_CASES))
+def test_random_periodic_fp32(gpu, P, seed):
+    """The FP32 + Kahan periodic path (one mirror layer, the lattice far field
+    in FP64) on random boxes and origins: per-target counts over the mirror
+    boxes bit-exact (the MAC is FP64); per target the field error within
+    16 u (L A + B) with A, B summed over every source image and L the span of
+    the mirror boxes (nodes of a mirror box are placed in the group's frame,
+    up to that far away); the median relative field error below 1e-5."""
+    from oracle import oracle as orc
+    p, q, c = _case(100 + seed)
+    c = dict(c, layers=1, far=True, correction=seed % 3, periodicity=(1, 1, 1))
+    x, y, z = (p[:, k].copy() for k in range(3))
+    eps2 = 1e-4
+    print(f"[periodic fp32 seed {seed}] {c}", flush=True)
+    sol = gpu(theta2=c["theta2"], eps2=eps2, group_size=c["G"], precision="fp32")
+    sol.set_periodic(periodicity=c["periodicity"], lattice=c["box"], origin=c["origin"],
+                     layers=1, far_field=True, correction=c["correction"])
+    sol.grow(x, y, z, q)
+    res = sol.traverse()
+    d = orc.domain_decompose(x, y, z, q)
+    t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
+    vb = P.lattice_vectors(c["box"], P.neighbour_boxes(c["periodicity"], 1))
+    w = P.walk_group_boxes(d.x, d.y, d.z, t, c["theta2"], eps2, vb, tile_size=c["G"], keys=d.keys)
+    lat = P.Lattice(sol.lattice()[0], c["box"], c["origin"], layers=1, correction=c["correction"])
+    lat.timestep(d.x, d.y, d.z, d.q)
+    e_far, _ = lat.field(d.x, d.y, d.z)
+    wk = sol.read_walk()
+    assert np.array_equal(wk["p2p"].astype(np.int32), w.p2p)
+    assert np.array_equal(wk["m2p"].astype(np.int32), w.m2p)
+    want = w.e + e_far
+    e = res.e.cpu().numpy().T
+    A, B = _mirror_scale(d.x, d.y, d.z, d.q, np.asarray(vb), eps2)
+    L = 3.0 * max(c["box"])
+    u = 2.0 ** -24
+    de = np.linalg.norm(e - want, axis=1)
+    ratio = de / (u * (L * A + B))
+    rel = de / np.linalg.norm(want, axis=1)
+    print(f"  dE/(u(LA+B)) max {ratio.max():.2e}  rel median {np.median(rel):.1e} "
+          f"max {rel.max():.1e}", flush=True)
+    assert ratio.max() <= 16.0
+    assert np.median(rel) <= 1e-5
+    sol.close()
