@@ -99,3 +99,56 @@ def test_random_sinks_parity(gpu, orc, seed):
           f"max dphi/C {np.max(dp / np.maximum(C, 1e-300)):.1e}", flush=True)
     assert np.all(de <= 1e-12 * B) and np.all(dp <= 1e-12 * C)
     sol.close()
+
+
+def _sink_groups(sx, sy, sz, box, G, orc):
+    """The grouped mode's walk groups: sinks clamped into the source box,
+    keyed on its Hilbert curve (20 levels, cell = size / 2^20), stably sorted,
+    cut into runs of G.  Returns (order, tile_first, tile_count)."""
+    bmin, bmax, bsize = box.as_arrays()
+    ic = []
+    for v, d in ((sx, 0), (sy, 1), (sz, 2)):
+        w = np.minimum(np.maximum(v, bmin[d]), bmax[d])
+        k = ((w - bmin[d]) / (bsize[d] / 1048576.0)).astype(np.int64)
+        ic.append(np.clip(k, 0, (1 << 20) - 1))
+    keys = np.array([orc.hilbert_key(a, b, c) for a, b, c in zip(*ic)], dtype=np.uint64)
+    order = np.argsort(keys, kind="stable")
+    m = len(sx)
+    tf = np.arange(0, m, G, dtype=np.int64)
+    tc = np.minimum(G, m - tf).astype(np.int64)
+    return order, tf, tc
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_sinks_grouped_parity(gpu, orc, seed):
+    """Grouped mode (no reference counterpart: sinks sorted along the source
+    box's Hilbert curve, runs of group_size walking together) against the
+    oracle's walk of the same runs: per-sink counts and work bit-exact, FP64
+    fields and potentials to 1e-12 of each sink's own scale."""
+    from oracle import periodic as P
+    p, q, s, c = _case(seed)
+    G = int(np.random.default_rng(7000 + seed).choice([2, 8, 16, 32, 64]))
+    x, y, z = (p[:, k].copy() for k in range(3))
+    sx, sy, sz = (s[:, k].copy() for k in range(3))
+    print(f"[grouped sinks seed {seed}] n={len(q)} m={len(sx)} G={G} {c}", flush=True)
+    d = orc.domain_decompose(x, y, z, q)
+    t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
+    order, tf, tc = _sink_groups(sx, sy, sz, d.box, G, orc)
+    w = P.walk_sinks(sx[order], sy[order], sz[order], t, len(q), c["theta2"], c["eps2"],
+                     sign=c["sign"], tiles=(tf, tc))
+    sol = gpu(theta2=c["theta2"], eps2=c["eps2"], precision="fp64", sign=c["sign"], group_size=G)
+    sol.grow(x, y, z, q)
+    res = sol.traverse_sinks(sx, sy, sz, grouped=True)
+    p2p, m2p = sol.sink_counts(len(sx))
+    assert np.array_equal(p2p[order].astype(np.int32), w.p2p)
+    assert np.array_equal(m2p[order].astype(np.int32), w.m2p)
+    assert np.array_equal(res.work.cpu().numpy()[order], w.work - 1.0)
+    e = res.e.cpu().numpy().T[order]
+    pot = res.pot.cpu().numpy()[order]
+    B, C = _abs_sums(sx[order], sy[order], sz[order], d.x, d.y, d.z, d.q, c["eps2"])
+    de = np.abs(e - w.e).max(axis=1)
+    dp = np.abs(pot - w.pot)
+    print(f"  max dE/B {np.max(de / np.maximum(B, 1e-300)):.1e} "
+          f"max dphi/C {np.max(dp / np.maximum(C, 1e-300)):.1e}", flush=True)
+    assert np.all(de <= 1e-12 * B) and np.all(dp <= 1e-12 * C)
+    sol.close()
