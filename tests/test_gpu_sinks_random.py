@@ -9,7 +9,8 @@ bit-exact, FP64 fields and potentials to 1e-12 of each sink's sum of
 absolute terms (sum_j |q_j| / r^2 and sum_j |q_j| / r with r^2 = d^2 + eps2,
 zero separations left out), so that a sink one ulp from a source, whose
 field reaches 1e32 without softening, leaves every other sink checked at its
-own scale."""
+own scale.  The grouped mode is checked the same way against the oracle's
+walk of the runs it forms."""
 import numpy as np
 import pytest
 
