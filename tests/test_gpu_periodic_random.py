@@ -172,3 +172,101 @@ def test_random_periodic_fp32(gpu, P, seed):
     assert ratio.max() <= 16.0
     assert np.median(rel) <= 1e-5
     sol.close()
+
+
+SHEAR_CASES = 8
+
+
+@pytest.mark.parametrize("seed", range(SHEAR_CASES))
+def test_random_sheared_lattice(gpu, P, seed):
+    """Non-orthogonal lattices (rows t_lattice_1..3 of a triangular matrix
+    with random shear; the reference's general path, module_mirror_boxes.f90:
+    73-79, 224-236): particles fill the parallelepiped; one or two mirror
+    layers with the far field under each correction.  Counts over the mirror
+    boxes bit-exact, coefficients (rows 0, 2..40), mu, fictitious charges,
+    fields and potentials to the tolerances of the orthorhombic cases."""
+    from oracle import oracle as orc
+    rng = np.random.default_rng(900 + seed)
+    n = int(rng.integers(200, 2500))
+    T = np.diag(rng.uniform(0.8, 1.4, 3))
+    T[1, 0] = rng.uniform(-0.4, 0.4)
+    T[2, 0], T[2, 1] = rng.uniform(-0.4, 0.4, 2)
+    origin = tuple(float(v) for v in rng.uniform(-1.0, 1.0, 3))
+    layers = 1 + seed % 2
+    correction = seed % 3
+    theta2 = float(rng.choice([0.16, 0.36, 0.64]))
+    G = int(rng.choice([8, 16, 32, 64]))
+    p = np.asarray(origin) + rng.random((n, 3)) @ T
+    q = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+    x, y, z = (p[:, k].copy() for k in range(3))
+    eps2 = 1e-4
+    print(f"[sheared seed {seed}] n={n} layers={layers} correction={correction} G={G} "
+          f"theta2={theta2}\n  T={T.tolist()}", flush=True)
+    sol = gpu(theta2=theta2, eps2=eps2, group_size=G, precision="fp64")
+    sol.set_periodic(periodicity=(1, 1, 1), lattice=T, origin=origin, layers=layers,
+                     far_field=True, correction=correction)
+    sol.grow(x, y, z, q)
+    res = sol.traverse()
+    d = orc.domain_decompose(x, y, z, q)
+    t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
+    vb = P.lattice_vectors(T, P.neighbour_boxes((1, 1, 1), layers))
+    w = P.walk_group_boxes(d.x, d.y, d.z, t, theta2, eps2, vb, tile_size=G, keys=d.keys)
+    ml, _, _ = sol.lattice()
+    ref = P.lattice_coefficients(lattice=T, layers=layers)
+    _rows_close(ml, ref, [0] + list(range(2, 41)), 1e-11, 40)
+    lat = P.Lattice(ml, T, origin, layers=layers, correction=correction)
+    lat.timestep(d.x, d.y, d.z, d.q)
+    e_far, p_far = lat.field(d.x, d.y, d.z)
+    wk = sol.read_walk()
+    assert np.array_equal(wk["p2p"].astype(np.int32), w.p2p)
+    assert np.array_equal(wk["m2p"].astype(np.int32), w.m2p)
+    exact_pot = correction == 2 and layers == 1
+    _, mu, fict = sol.lattice()
+    _rows_close(mu, lat.step.mu[:231], range(0 if exact_pot else 1, 21), 1e-11, 20)
+    assert fict.shape == lat.step.fict.shape
+    assert np.abs(fict - lat.step.fict).max(initial=0.0) <= \
+        1e-13 * max(1.0, np.abs(lat.step.fict).max(initial=0.0))
+    want_e = w.e + e_far
+    want_p = w.pot + p_far
+    e = res.e.cpu().numpy().T
+    pot = res.pot.cpu().numpy()
+    scale = np.abs(want_e).max()
+    print(f"  max|dE|/max|E| = {np.abs(e - want_e).max() / scale:.2e}", flush=True)
+    assert np.abs(e - want_e).max() <= 1e-11 * scale
+    dp = pot - want_p
+    if not exact_pot:
+        dp = dp - dp.mean()
+    assert np.abs(dp).max() <= 1e-11 * np.abs(want_p).max()
+    sol.close()
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_constrain_sheared_lattice(gpu, seed):
+    """constrain_periodic on a non-orthogonal, fully periodic lattice (the
+    lattice-coordinate path of module_mirror_boxes.f90:174-210): every point
+    ends with lattice coordinates in [0, 1] (to rounding), moved by an integer
+    combination of the lattice vectors, and points already inside are left
+    bit for bit."""
+    import torch
+    rng = np.random.default_rng(950 + seed)
+    T = np.diag(rng.uniform(0.8, 1.4, 3))
+    T[1, 0] = rng.uniform(-0.4, 0.4)
+    T[2, 0], T[2, 1] = rng.uniform(-0.4, 0.4, 2)
+    origin = rng.uniform(-1.0, 1.0, 3)
+    sol = gpu(precision="fp64")
+    sol.set_periodic(periodicity=(1, 1, 1), lattice=T, origin=tuple(origin), far_field=False)
+    u = np.concatenate([rng.uniform(-3.0, 4.0, (2000, 3)), rng.uniform(0.01, 0.99, (500, 3))])
+    pts = origin + u @ T
+    tx, ty, tz = (torch.from_numpy(pts[:, k].copy()).cuda() for k in range(3))
+    sol.constrain_periodic(tx, ty, tz)
+    got = np.stack([tx.cpu().numpy(), ty.cpu().numpy(), tz.cpu().numpy()], 1)
+    lc = np.linalg.solve(T.T, (got - origin).T).T          # lattice coordinates
+    assert lc.min() >= -1e-12 and lc.max() <= 1.0 + 1e-12
+    shift = np.linalg.solve(T.T, (got - pts).T).T
+    assert np.abs(shift - np.round(shift)).max() <= 1e-9
+    inside = np.all((u >= 0.0) & (u <= 1.0), axis=1)
+    lc_in = np.linalg.solve(T.T, (pts - origin).T).T
+    still = inside & np.all((lc_in > 1e-9) & (lc_in < 1.0 - 1e-9), axis=1)
+    assert still.sum() >= 400
+    assert np.array_equal(got[still], pts[still])
+    sol.close()
