@@ -362,13 +362,17 @@ int pepc_gpu_read_energies(void *ctx, int64_t cap, double *out, int64_t *nrec);
  * 132-149, 766-833: lattice coefficients to order 40, central-box multipoles
  * to order 20, well separation ws = mirror_box_layers, the extrinsic-to-
  * intrinsic correction fmm_extrinsic_correction, :42-49, 807-829).  A
- * unit-cube lattice with one mirror layer uses the reference's pretabulated
- * coefficients (:114-121, :260-508); other lattices and layer counts compute
- * them (calc_lattice_coefficients, :154-213) on the device in set_periodic.
+ * unit-cube lattice with one mirror layer, periodic on all three axes, uses
+ * the reference's pretabulated coefficients (:114-121, :260-508); other
+ * lattices, layer counts and periodicities compute them
+ * (calc_lattice_coefficients, :154-213, over the periodic axes' neighbour
+ * boxes) on the device in set_periodic.  With one or two periodic axes the
+ * far field still runs, as the reference's does (do_periodic .and.
+ * include_far_field_if_periodic).
  * The on-device timestep wraps positions back into the box after each drift
  * (constrain_periodic, module_mirror_boxes.f90:174-210).
  *
- * Limits: far_field needs all three axes periodic; mirror_box_layers 1..3.
+ * Limits: mirror_box_layers 1..3.
  * Not with batched systems, the adjoint or frozen lists (EINVAL).  Particles
  * must lie in the central box (the reference warns, check_lattice_boundaries
  * :261-281). */

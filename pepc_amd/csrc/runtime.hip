@@ -1679,10 +1679,12 @@ int pepc_gpu_set_periodic(void* ctx, const pepc_gpu_periodic* p) {
   LatticeGeom g;
   memcpy(g.lat, p->t_lattice, sizeof(g.lat));
   if (!invert3(g.lat, g.latinv)) return fail(c, PEPC_GPU_EINVAL, "set_periodic: degenerate lattice");
+  // The far field runs for any periodicity, as the reference's does
+  // (do_periodic .and. include_far_field_if_periodic, coulomb/
+  // module_interaction_specific.f90:199-225): the lattice sums, the mirror
+  // boxes and the fictitious charges cover the periodic axes only.
   const bool all3 = p->periodicity[0] && p->periodicity[1] && p->periodicity[2];
   const bool far = p->far_field != 0;
-  if (far && !all3)
-    return fail(c, PEPC_GPU_EINVAL, "set_periodic: the far field needs all three axes periodic");
   const int corr = p->extrinsic_correction;
   if (far && corr != PEPC_GPU_EXTRINSIC_NONE && corr != PEPC_GPU_EXTRINSIC_REDLACK &&
       corr != PEPC_GPU_EXTRINSIC_FICTCHARGE)
@@ -1728,10 +1730,11 @@ int pepc_gpu_set_periodic(void* ctx, const pepc_gpu_periodic* p) {
     auto nsq = [](const double* v) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; };
     // system_is_unit_cube (module_mirror_boxes.f90:248-255): the reference
     // then loads its pretabulated coefficients (module_fmm_framework.f90:114-121).
-    // Those are the lattice beyond one mirror layer, so with more layers the
-    // coefficients are computed for the layers set (the reference loads the
-    // one-layer table for any layer count).
-    const bool unit = layers == 1 && g.simple && nsq(t) == 1.0 && nsq(t + 3) == 1.0 &&
+    // Those are the 3-D periodic lattice beyond one mirror layer, so with more
+    // layers or fewer periodic axes the coefficients are computed for the
+    // lattice set (the reference loads the table for any layer count and
+    // periodicity, and warns that it holds for the 3-D periodic unit box only).
+    const bool unit = layers == 1 && all3 && g.simple && nsq(t) == 1.0 && nsq(t + 3) == 1.0 &&
                       nsq(t + 6) == 1.0;
     if (unit) {
       mlh.assign(kLatticeNT, make_double2(0.0, 0.0));

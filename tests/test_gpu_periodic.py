@@ -357,8 +357,8 @@ def test_periodic_graph_replay_equals_eager(gpu):
 def test_periodic_errors(gpu):
     from pepc_amd._lib import PepcGpuError
     sol = gpu(precision="fp64")
-    with pytest.raises(PepcGpuError):
-        sol.set_periodic(periodicity=(1, 1, 0))                  # far field needs 3-D
+    sol.set_periodic(periodicity=(1, 1, 0))       # the far field runs with two periodic axes too
+    assert np.isfinite(sol.lattice()[0][1:]).all()
     with pytest.raises(PepcGpuError):
         sol.set_periodic(correction=3)                           # MEASUREMENT: not in the reference
     with pytest.raises(PepcGpuError):

@@ -270,3 +270,67 @@ def test_constrain_sheared_lattice(gpu, seed):
     assert still.sum() >= 400
     assert np.array_equal(got[still], pts[still])
     sol.close()
+
+
+PARTIAL_CASES = 8
+
+
+@pytest.mark.parametrize("seed", range(PARTIAL_CASES))
+def test_random_partial_periodicity_far_field(gpu, P, seed):
+    """One or two periodic axes with the far field, which the reference runs
+    for any periodicity (do_periodic .and. include_far_field_if_periodic):
+    lattice sums, mirror boxes and fictitious charges over the periodic axes
+    only.  Coefficients computed on the device against the oracle's (rows 0,
+    2..40), mu (rows 1..20), fictitious charges, per-target counts bit-exact,
+    fields to 1e-11 and potentials to 1e-11 up to a constant."""
+    from oracle import oracle as orc
+    rng = np.random.default_rng(1200 + seed)
+    n = int(rng.integers(200, 2500))
+    box = tuple(float(v) for v in rng.uniform(0.7, 1.6, 3))
+    origin = tuple(float(v) for v in rng.uniform(-1.0, 1.0, 3))
+    axes = [(1, 1, 0), (1, 0, 1), (0, 1, 1), (1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    per = axes[int(rng.integers(0, len(axes)))]
+    layers = 1 + seed % 2
+    correction = seed % 3
+    theta2 = float(rng.choice([0.16, 0.36, 0.64]))
+    G = int(rng.choice([8, 16, 32, 64]))
+    p = np.asarray(origin) + rng.random((n, 3)) * np.asarray(box)
+    q = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+    x, y, z = (p[:, k].copy() for k in range(3))
+    eps2 = 1e-4
+    print(f"[partial seed {seed}] n={n} per={per} layers={layers} correction={correction} "
+          f"G={G} theta2={theta2} box={box}", flush=True)
+    sol = gpu(theta2=theta2, eps2=eps2, group_size=G, precision="fp64")
+    sol.set_periodic(periodicity=per, lattice=box, origin=origin, layers=layers,
+                     far_field=True, correction=correction)
+    sol.grow(x, y, z, q)
+    res = sol.traverse()
+    d = orc.domain_decompose(x, y, z, q)
+    t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
+    vb = P.lattice_vectors(box, P.neighbour_boxes(per, layers))
+    w = P.walk_group_boxes(d.x, d.y, d.z, t, theta2, eps2, vb, tile_size=G, keys=d.keys)
+    ml, _, _ = sol.lattice()
+    ref = P.lattice_coefficients(lattice=box, periodicity=per, layers=layers)
+    _rows_close(ml, ref, [0] + list(range(2, 41)), 1e-11, 40)
+    lat = P.Lattice(ml, box, origin, periodicity=per, layers=layers, correction=correction)
+    lat.timestep(d.x, d.y, d.z, d.q)
+    e_far, p_far = lat.field(d.x, d.y, d.z)
+    wk = sol.read_walk()
+    assert np.array_equal(wk["p2p"].astype(np.int32), w.p2p)
+    assert np.array_equal(wk["m2p"].astype(np.int32), w.m2p)
+    _, mu, fict = sol.lattice()
+    _rows_close(mu, lat.step.mu[:231], range(1, 21), 1e-11, 20)
+    assert fict.shape == lat.step.fict.shape
+    assert np.abs(fict - lat.step.fict).max(initial=0.0) <= \
+        1e-13 * max(1.0, np.abs(lat.step.fict).max(initial=0.0))
+    want_e = w.e + e_far
+    want_p = w.pot + p_far
+    e = res.e.cpu().numpy().T
+    pot = res.pot.cpu().numpy()
+    scale = np.abs(want_e).max()
+    dp = pot - want_p
+    print(f"  max|dE|/max|E| = {np.abs(e - want_e).max() / scale:.2e}  "
+          f"potential constant {dp.mean():.2e}", flush=True)
+    assert np.abs(e - want_e).max() <= 1e-11 * scale
+    assert np.abs(dp - dp.mean()).max() <= 1e-11 * np.abs(want_p).max()
+    sol.close()
