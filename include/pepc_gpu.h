@@ -229,14 +229,18 @@ int pepc_gpu_read_walk(void *ctx, int64_t gcap, uint32_t *gfirst, uint32_t *gcou
 
 /* FP64 direct sum (src/utils/module_directsum.f90:34-172) on device arrays:
  * fields at the ntest targets testidx[] from all n sources, self excluded by
- * index.  Asynchronous on the context's stream. */
+ * index.  With periodic boundaries set on the context, directforce's
+ * periodic form: every source in every mirror box, self only in the central
+ * box (:80-93), plus the lattice far field at the targets from all sources
+ * (:130-137), after which pepc_gpu_lattice_get reports these sources'
+ * moments.  Asynchronous on the context's stream. */
 int pepc_gpu_direct_device(void *ctx, int64_t n, const double *x, const double *y,
                            const double *z, const double *q, int64_t ntest,
                            const int64_t *testidx, double eps2, double sign, double *ex,
                            double *ey, double *ez, double *pot);
 
 /* Host form of pepc_gpu_direct_device (directforce, src/utils/
- * module_directsum.f90:34-172, for open boundaries): host arrays in and
+ * module_directsum.f90:34-172, open or periodic): host arrays in and
  * out, indices checked against n (EINVAL), synchronous; device memory for
  * the sources is taken for the call and released after it. */
 int pepc_gpu_direct(void *ctx, int64_t n, const double *x, const double *y, const double *z,

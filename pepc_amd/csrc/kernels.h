@@ -221,10 +221,11 @@ void launch_lattice_sources(hipStream_t s, int64_t n, const double* x, const dou
                             const double* z, const double* q, const LatticeGeom& g,
                             const double2* ml, const double* fact, double2* part, double* dpart,
                             double2* om, double* dip, double2* mu, double* fict, double* coef);
+// Positions x[idx[i]] (idx == nullptr: x[i]) for outputs i < m.
 void launch_lattice_field(hipStream_t s, int64_t m, const double* x, const double* y, const double* z,
                           const LatticeGeom& g, const double* coef, const double* inv,
                           const double* fict, double sign, double* ex, double* ey, double* ez,
-                          double* pot);
+                          double* pot, const int64_t* idx = nullptr);
 void launch_constrain_periodic(hipStream_t s, int64_t n, double* x, double* y, double* z,
                                const LatticeGeom& g);
 
@@ -302,8 +303,10 @@ struct ReplayArgs {
 void launch_replay(hipStream_t s, const ReplayArgs& a, int blocks);
 
 // walk.hip: FP64 direct sum
+// nbox mirror-box vectors vbox (nullptr: open boundaries).
 void launch_direct(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
                    const double* q, int64_t ntest, const int64_t* testidx, double eps2, double sign,
-                   double* ex, double* ey, double* ez, double* pot);
+                   double* ex, double* ey, double* ez, double* pot, int nbox = 1,
+                   const double* vbox = nullptr);
 
 }  // namespace pepc

@@ -561,11 +561,13 @@ __global__ __launch_bounds__(kFieldT) void lattice_field_kernel(int64_t n, const
                                                                double sign, double* __restrict__ ex,
                                                                double* __restrict__ ey,
                                                                double* __restrict__ ez,
-                                                               double* __restrict__ pot) {
+                                                               double* __restrict__ pot,
+                                                               const int64_t* __restrict__ idx) {
 #pragma clang fp contract(fast)   // tolerance-checked sums: fused multiply-adds allowed
   const int64_t i = blockIdx.x * (int64_t)kFieldT + threadIdx.x;
   if (i >= n) return;
-  const double px = x[i], py = y[i], pz = z[i];
+  const int64_t at = idx ? idx[i] : i;   // position of output i
+  const double px = x[at], py = y[at], pz = z[at];
   const double rx = px - g.center[0], ry = py - g.center[1], rz = pz - g.center[2];
   const double r2 = rx * rx + ry * ry + rz * rz;
   double phi = 0.0, l10 = 0.0, l11r = 0.0, l11i = 0.0;
@@ -715,11 +717,11 @@ void launch_lattice_sources(hipStream_t s, int64_t n, const double* x, const dou
 void launch_lattice_field(hipStream_t s, int64_t m, const double* x, const double* y, const double* z,
                           const LatticeGeom& g, const double* coef, const double* inv,
                           const double* fict, double sign, double* ex, double* ey, double* ez,
-                          double* pot) {
+                          double* pot, const int64_t* idx) {
   if (m <= 0) return;
   const unsigned fb = (unsigned)((m + kFieldT - 1) / kFieldT);
   lattice_field_kernel<<<fb, kFieldT, 0, s>>>(m, x, y, z, g, coef, inv, fict, sign, ex, ey, ez,
-                                              pot);
+                                              pot, idx);
 }
 
 void launch_lattice_far(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,

@@ -1092,6 +1092,28 @@ int pepc_gpu_read_walk(void* p, int64_t gcap, uint32_t* gfirst, uint32_t* gcount
   return 0;
 }
 
+// directforce (src/utils/module_directsum.f90:34-172): with periodic
+// boundaries every source in every mirror box, self only in the central box
+// (:80-93), then the lattice far field at the targets from all sources
+// (calc_force_after_grow + calc_force_per_particle, :130-137), which leaves
+// the context's lattice moments at these sources.
+static void enqueue_direct(Ctx* c, hipStream_t s, int64_t n, const double* x, const double* y,
+                           const double* z, const double* q, int64_t ntest, const int64_t* testidx,
+                           double eps2, double sign, double* ex, double* ey, double* ez,
+                           double* pot) {
+  if (!c->periodic) {
+    launch_direct(s, n, x, y, z, q, ntest, testidx, eps2, sign, ex, ey, ez, pot);
+    return;
+  }
+  launch_direct(s, n, x, y, z, q, ntest, testidx, eps2, sign, ex, ey, ez, pot, c->nbox, c->vbox);
+  if (c->far_field && ntest > 0) {
+    launch_lattice_sources(s, n, x, y, z, q, c->lgeom, c->ml, c->fact, c->lpart, c->ldpart, c->lom,
+                           c->ldip, c->mu, c->fict, c->lcoef);
+    launch_lattice_field(s, ntest, x, y, z, c->lgeom, c->lcoef, c->linv, c->fict, sign, ex, ey, ez,
+                         pot, testidx);
+  }
+}
+
 int pepc_gpu_direct_device(void* p, int64_t n, const double* x, const double* y, const double* z,
                            const double* q, int64_t ntest, const int64_t* testidx, double eps2,
                            double sign, double* ex, double* ey, double* ez, double* pot) {
@@ -1099,7 +1121,7 @@ int pepc_gpu_direct_device(void* p, int64_t n, const double* x, const double* y,
   if (!c || !x || !y || !z || !q || !testidx || !ex || !ey || !ez || !pot || n < 1 || ntest < 0)
     return fail(c, PEPC_GPU_EINVAL, "direct: bad argument");
   HIPCHECK(c, hipSetDevice(c->cfg.device));
-  launch_direct(c->stream, n, x, y, z, q, ntest, testidx, eps2, sign, ex, ey, ez, pot);
+  enqueue_direct(c, c->stream, n, x, y, z, q, ntest, testidx, eps2, sign, ex, ey, ez, pot);
   HIPCHECK(c, hipGetLastError());
   return 0;
 }
@@ -1132,8 +1154,8 @@ int pepc_gpu_direct(void* p, int64_t n, const double* x, const double* y, const 
     e = hipMemcpyAsync(dst[k], src[k], sizeof(double) * n, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(dt, testidx, sizeof(int64_t) * ntest, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) {
-    launch_direct(s, n, dx, dy, dz, dq, ntest, dt, eps2, sign, r, r + ntest, r + 2 * ntest,
-                  r + 3 * ntest);
+    enqueue_direct(c, s, n, dx, dy, dz, dq, ntest, dt, eps2, sign, r, r + ntest, r + 2 * ntest,
+                   r + 3 * ntest);
     e = hipGetLastError();
   }
   double* out[4] = {ex, ey, ez, pot};

@@ -579,7 +579,8 @@ __global__ __launch_bounds__(256) void direct_kernel(int64_t n, const double* __
                                                     const int64_t* __restrict__ testidx, double eps2,
                                                     double sign, double* __restrict__ ex,
                                                     double* __restrict__ ey, double* __restrict__ ez,
-                                                    double* __restrict__ pot) {
+                                                    double* __restrict__ pot, int nbox,
+                                                    const double* __restrict__ vbox) {
 #pragma clang fp contract(off)
   __shared__ double sx[256], sy[256], sz[256], sq[256];
   const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -596,17 +597,37 @@ __global__ __launch_bounds__(256) void direct_kernel(int64_t n, const double* __
     __syncthreads();
     const int lim = (n - base) < 256 ? (int)(n - base) : 256;
     double part[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = 0; k < lim; ++k) {
-      if (base + k == tj) continue;
-      const double dx = xt - sx[k], dy = yt - sy[k], dz = zt - sz[k];
-      const double d2 = dx * dx + dy * dy + dz * dz;
-      const double r = sqrt(d2 + eps2);
-      const double rd = 1.0 / r;
-      const double rd3q = sq[k] * rd * rd * rd;
-      part[3] += sq[k] * rd;
-      part[0] += rd3q * dx;
-      part[1] += rd3q * dy;
-      part[2] += rd3q * dz;
+    if (!vbox) {
+      for (int k = 0; k < lim; ++k) {
+        if (base + k == tj) continue;
+        const double dx = xt - sx[k], dy = yt - sy[k], dz = zt - sz[k];
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        const double r = sqrt(d2 + eps2);
+        const double rd = 1.0 / r;
+        const double rd3q = sq[k] * rd * rd * rd;
+        part[3] += sq[k] * rd;
+        part[0] += rd3q * dx;
+        part[1] += rd3q * dy;
+        part[2] += rd3q * dz;
+      }
+    } else {
+      // every source in every mirror box (the target shifted by -v), self
+      // only in the central box (module_directsum.f90:80-93)
+      for (int k = 0; k < lim; ++k) {
+        for (int b = 0; b < nbox; ++b) {
+          const double vx = vbox[3 * b], vy = vbox[3 * b + 1], vz = vbox[3 * b + 2];
+          if (base + k == tj && vx == 0.0 && vy == 0.0 && vz == 0.0) continue;
+          const double dx = (xt - vx) - sx[k], dy = (yt - vy) - sy[k], dz = (zt - vz) - sz[k];
+          const double d2 = dx * dx + dy * dy + dz * dz;
+          const double r = sqrt(d2 + eps2);
+          const double rd = 1.0 / r;
+          const double rd3q = sq[k] * rd * rd * rd;
+          part[3] += sq[k] * rd;
+          part[0] += rd3q * dx;
+          part[1] += rd3q * dy;
+          part[2] += rd3q * dz;
+        }
+      }
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[c] += part[c];
@@ -688,10 +709,10 @@ void launch_walk_stats(hipStream_t s, const WalkArgs& a) {
 
 void launch_direct(hipStream_t s, int64_t n, const double* x, const double* y, const double* z,
                    const double* q, int64_t ntest, const int64_t* testidx, double eps2, double sign,
-                   double* ex, double* ey, double* ez, double* pot) {
+                   double* ex, double* ey, double* ez, double* pot, int nbox, const double* vbox) {
   if (ntest <= 0) return;
   direct_kernel<<<(unsigned)((ntest + 255) / 256), 256, 0, s>>>(n, x, y, z, q, ntest, testidx, eps2,
-                                                                sign, ex, ey, ez, pot);
+                                                                sign, ex, ey, ez, pot, nbox, vbox);
 }
 
 }  // namespace pepc

@@ -1022,10 +1022,8 @@ contains
       type(t_particle_results), intent(out) :: directresults(:)
       real(c_double), allocatable :: x(:), y(:), z(:), q(:), ex(:), ey(:), ez(:), pot(:)
       integer(c_int64_t), allocatable :: idx(:)
-      if (any(periodicity)) then
-         write (*, *) 'PEPC-GPU: directforce sums open boundaries only (no mirror boxes or lattice term)'
-         error stop 1
-      end if
+      ! with periodicity, the mirror boxes and the lattice far field set in
+      ! pepc_prepare apply (module_directsum.f90:80-93, 130-137)
       if (.not. c_associated(ctx)) then
          write (*, *) 'PEPC-GPU: directforce needs pepc_prepare first'
          error stop 1

@@ -334,3 +334,54 @@ def test_random_partial_periodicity_far_field(gpu, P, seed):
     assert np.abs(e - want_e).max() <= 1e-11 * scale
     assert np.abs(dp - dp.mean()).max() <= 1e-11 * np.abs(want_p).max()
     sol.close()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_periodic_direct_sum(gpu, P, seed):
+    """directforce's periodic form (module_directsum.f90:80-93, 130-137):
+    every source in every mirror box, self only in the central box, plus the
+    lattice far field at the targets.  Against the oracle's mirror-box walk
+    at theta = 0 (all pairs, self excluded in the central box) plus its
+    lattice field: fields to 1e-11, potentials to 1e-11 up to a constant."""
+    from oracle import oracle as orc
+    rng = np.random.default_rng(1400 + seed)
+    n = int(rng.integers(100, 1500))
+    box = tuple(float(v) for v in rng.uniform(0.7, 1.6, 3))
+    origin = tuple(float(v) for v in rng.uniform(-1.0, 1.0, 3))
+    axes = [(1, 1, 1), (1, 1, 0), (0, 1, 1), (1, 0, 0)]
+    per = axes[seed % len(axes)]
+    layers = 1 + (seed // 4) % 2
+    far = bool(rng.random() < 0.75)
+    correction = int(rng.integers(0, 3))
+    p = np.asarray(origin) + rng.random((n, 3)) * np.asarray(box)
+    q = rng.normal(size=n)
+    q -= q.mean()
+    x, y, z = (p[:, k].copy() for k in range(3))
+    idx = rng.choice(n, size=int(rng.integers(20, 300)), replace=True)
+    eps2 = 1e-4
+    print(f"[periodic direct seed {seed}] n={n} per={per} layers={layers} far={far} "
+          f"correction={correction} ntest={len(idx)}", flush=True)
+    sol = gpu(eps2=eps2, precision="fp64")
+    sol.set_periodic(periodicity=per, lattice=box, origin=origin, layers=layers, far_field=far,
+                     correction=correction)
+    e, pot = sol.direct(x, y, z, q, idx)
+    e, pot = e.cpu().numpy(), pot.cpu().numpy()
+    d = orc.domain_decompose(x, y, z, q)
+    t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
+    vb = P.lattice_vectors(box, P.neighbour_boxes(per, layers))
+    w = P.walk_group_boxes(d.x, d.y, d.z, t, 0.0, eps2, vb, tile_size=16, keys=d.keys)
+    inv = np.argsort(d.perm)
+    want_e, want_p = w.e[inv][idx], w.pot[inv][idx]
+    if far:
+        lat = P.Lattice(sol.lattice()[0], box, origin, periodicity=per, layers=layers,
+                        correction=correction)
+        lat.timestep(d.x, d.y, d.z, d.q)
+        ef, pf = lat.field(x[idx], y[idx], z[idx])
+        want_e, want_p = want_e + ef, want_p + pf
+    scale = np.abs(want_e).max()
+    dp = pot - want_p
+    print(f"  max|dE|/max|E| = {np.abs(e - want_e).max() / scale:.2e}  "
+          f"potential constant {dp.mean():.2e}", flush=True)
+    assert np.abs(e - want_e).max() <= 1e-11 * scale
+    assert np.abs(dp - dp.mean()).max() <= 1e-11 * np.abs(want_p).max()
+    sol.close()
