@@ -64,28 +64,31 @@ __device__ void adjoint_term(const double d[3], double r2, double gphi, const do
   const double tr = qt[0] + qt[1] + qt[2];
   const double dQd = d[0] * Qd[0] + d[1] * Qd[1] + d[2] * Qd[2];
   const double uQd = u[0] * Qd[0] + u[1] * Qd[1] + u[2] * Qd[2];
+  // The terms of phi = q G + p.grad G + Qt : grad grad G and E = -grad phi,
+  // differentiated and collected by vector, with
+  //   s1 = gphi r^-3 + 3 ud r^-5,  s2 = 3 gphi r^-5 + 15 ud r^-7:
+  //   gp = u r^-3 - s1 d,  M_ab = s2 d_a d_b - 3 r^-5 (u_a d_b + u_b d_a) - s1 delta_ab,
+  //   D = cu u - s1 p + 2 s2 Qd - 6 r^-5 Qu + cd d.
+  const double s1 = gphi * r3 + 3.0 * ud * r5;
+  const double s2 = 3.0 * gphi * r5 + 15.0 * ud * r7;
+  const double s3 = 15.0 * gphi * r7 + 105.0 * ud * r9;
+  const double pt = pd + tr;
+  const double t5 = 3.0 * r5;
   gq = gphi * ri + ud * r3;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) gp[k] = (u[k] - gphi * d[k]) * r3 - 3.0 * ud * d[k] * r5;
+  for (int k = 0; k < 3; ++k) gp[k] = u[k] * r3 - s1 * d[k];
   const int ma[6] = {0, 1, 2, 0, 1, 2}, mb[6] = {0, 1, 2, 1, 2, 0};
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     const int a = ma[k], b = mb[k];
-    const double dab = a == b ? 1.0 : 0.0;
-    M[k] = gphi * (-dab * r3 + 3.0 * d[a] * d[b] * r5) -
-           (3.0 * r5 * (u[a] * d[b] + u[b] * d[a] + dab * ud) - 15.0 * r7 * ud * d[a] * d[b]);
+    const double v = s2 * d[a] * d[b] - t5 * (u[a] * d[b] + u[b] * d[a]);
+    M[k] = a == b ? v - s1 : v;
   }
+  const double cu = q * r3 - t5 * pt + 15.0 * r7 * dQd;
+  const double cd = -q * s1 + pt * s2 - t5 * up - s3 * dQd + 30.0 * r7 * uQd;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    double v = q * gp[k];
-    v += gphi * (-p[k] * r3 + 3.0 * d[k] * pd * r5) -
-         (3.0 * r5 * (u[k] * pd + p[k] * ud + d[k] * up) - 15.0 * r7 * d[k] * ud * pd);
-    v += gphi * (3.0 * r5 * (2.0 * Qd[k] + tr * d[k]) - 15.0 * r7 * d[k] * dQd) -
-         (3.0 * r5 * (u[k] * tr + 2.0 * Qu[k]) -
-          15.0 * r7 * (u[k] * dQd + 2.0 * ud * Qd[k] + 2.0 * d[k] * uQd + d[k] * ud * tr) +
-          105.0 * r9 * d[k] * ud * dQd);
-    D[k] = v;
-  }
+  for (int k = 0; k < 3; ++k)
+    D[k] = cu * u[k] - s1 * p[k] + 2.0 * s2 * Qd[k] - 2.0 * t5 * Qu[k] + cd * d[k];
 }
 
 // The same for a leaf (p = Qt = 0): D = dL/dd and gq only.
