@@ -112,3 +112,42 @@ def test_equal_keys_distinct_positions(gpu, orc):
     assert st["dup_fixed"] == d.nfixed > 0
     _compare(orc, sol, None, d, "one-cell runs")
     sol.close()
+
+
+@pytest.mark.parametrize("runs", [(257,), (300, 700), (1500, 260, 2)])
+def test_long_runs_of_equal_keys(gpu, orc, runs):
+    """Runs of identical positions longer than the reference sort's
+    insertion threshold (256, sl_tune_intern.h:30): rs_rec_ip recurses
+    through every digit with the whole run in one bucket, and the fix-up then
+    spreads the run over consecutive keys.  Keys, perm, fixed-up positions,
+    tree size, per-target counts, work bit-exact; FP64 fields to 1e-12."""
+    r = np.random.default_rng(70 + len(runs))
+    n = 4000 + sum(runs)
+    p = r.random((n, 3))
+    at = r.permutation(n)
+    k = 0
+    for length in runs:
+        src = at[k]
+        p[at[k + 1:k + length]] = p[src]
+        k += length
+    q = r.choice(np.array([-1.0, 1.0]), n)
+    x, y, z = (p[:, j].copy() for j in range(3))
+    d = orc.domain_decompose(x, y, z, q)
+    t = orc.tree_build(d.keys, d.x, d.y, d.z, d.q, d.box)
+    w = orc.walk_group(d.x, d.y, d.z, t, 0.36, 1e-4, tile_size=16, keys=d.keys)
+    print(f"[long runs {runs}] n={n} moved by the fix-up: {d.nfixed}", flush=True)
+    assert d.nfixed >= sum(runs) - 2 * len(runs)
+    sol = gpu(theta2=0.36, eps2=1e-4, group_size=16, precision="fp64", node_factor=8.0)
+    st = sol.grow(x, y, z, q)
+    res = sol.traverse()
+    assert st["dup_fixed"] == d.nfixed
+    _compare(orc, sol, None, d, f"runs {runs}")
+    assert st["nnodes"] == t.nnodes and st["ntwig"] == t.ntwig
+    wk = sol.read_walk()
+    assert np.array_equal(wk["p2p"].astype(np.int32), w.p2p)
+    assert np.array_equal(wk["m2p"].astype(np.int32), w.m2p)
+    assert np.array_equal(res.work.cpu().numpy(), w.work)
+    e = res.e.cpu().numpy().T
+    assert np.max(np.abs(e - w.e)) <= 1e-12 * np.abs(w.e).max()
+    assert np.max(np.abs(res.pot.cpu().numpy() - w.pot)) <= 1e-12 * np.abs(w.pot).max()
+    sol.close()
